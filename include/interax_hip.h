@@ -1,0 +1,212 @@
+/*
+ * interax_hip: the C ABI of the MI355X (gfx950) HIP platform for the InteraxFF pair and three-body forces.
+ *
+ * This is the boundary a host binding calls.  It replaces, for one platform, the four kernel interfaces of the
+ * reference plugin (openmmapi/include/InteraxKernels.h:51-189):
+ *
+ *   CalcInteraxDispersionForceKernel  (InteraxKernels.h:51-81)   -> ixh_pair_* with IXH_TERM_DISPERSION
+ *   CalcInteraxExchangeForceKernel    (InteraxKernels.h:87-117)  -> ixh_pair_* with IXH_TERM_EXCHANGE
+ *   CalcInteraxShortRangeForceKernel  (InteraxKernels.h:123-153) -> ixh_pair_* with IXH_TERM_SHORT_RANGE
+ *   CalcInteraxThreeBodyForceKernel   (InteraxKernels.h:159-189) -> ixh_threebody_*
+ *
+ * Each interface's initialize() maps to *_create(), execute() to *_execute() and copyParametersToContext() to
+ * *_set_parameters().  Pair forces that share the nonbonded method, cutoff, switching function, soft-core radius,
+ * exclusions and alchemical flags are fused into one handle and evaluated by one kernel; the `terms` argument of
+ * ixh_pair_execute() selects which of them an evaluation includes (the force groups of the reference's
+ * getState(groups) call).
+ *
+ * Conventions.
+ *   - Every function returns IXH_OK (0) or an error code; ixh_last_error() gives the message of the last error on
+ *     the calling thread.  No C++ exception crosses this boundary.
+ *   - Host arrays passed in are copied; the caller keeps ownership.  The library owns the device memory of its
+ *     handles.  The device buffers of ixh_buffers are borrowed for the duration of one call.
+ *   - A handle is not re-entrant: calls on one handle must not overlap.  Distinct handles may run concurrently on
+ *     distinct streams.
+ *   - Units are those of OpenMM: nm, kJ/mol, kJ/mol/nm.
+ *   - Forces are accumulated in 64-bit fixed point with 2^32 units per kJ/mol/nm, as OpenMM's GPU platforms do,
+ *     which makes them independent of the order of accumulation.  Energies are accumulated in FP64.
+ */
+#ifndef INTERAX_HIP_H
+#define INTERAX_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IXH_VERSION_MAJOR 0
+#define IXH_VERSION_MINOR 1
+
+/* Status codes. */
+#define IXH_OK 0
+#define IXH_ERROR_INVALID_ARGUMENT 1   /* a setting or parameter the reference rejects (same messages) */
+#define IXH_ERROR_HIP 2                /* a HIP runtime error */
+#define IXH_ERROR_STATE 3              /* e.g. "The number of particles has changed" */
+#define IXH_ERROR_UNSUPPORTED 4
+#define IXH_ERROR_CAPACITY 5           /* a request that does not fit the device memory budget */
+
+/* Precision of the evaluation: OpenMM's "Precision" platform property (single, mixed, double). */
+#define IXH_SINGLE 0    /* FP32 positions and arithmetic */
+#define IXH_MIXED 1     /* FP64 positions (float4 posq plus float4 correction), FP32 arithmetic */
+#define IXH_DOUBLE 2    /* FP64 positions and arithmetic */
+
+/* Nonbonded methods (InteraxPairForce.h:76-93). */
+#define IXH_NO_CUTOFF 0
+#define IXH_CUTOFF_NON_PERIODIC 1
+#define IXH_CUTOFF_PERIODIC 2
+
+/* Pair terms, one per pair force class. */
+#define IXH_TERM_DISPERSION 1
+#define IXH_TERM_EXCHANGE 2
+#define IXH_TERM_SHORT_RANGE 4
+
+/* Overlap methods (InteraxExchangeForce.h OverlapMethod). */
+#define IXH_OVERLAP_EXACT 0
+#define IXH_OVERLAP_INTERAX_ORDER2 2
+
+/* Three-body damping methods (InteraxThreeBodyForce.h DampingMethod). */
+#define IXH_DAMPING_NONE 0
+#define IXH_DAMPING_TANG_TOENNIES 1
+
+/* execute() flags */
+#define IXH_INCLUDE_FORCES 1
+#define IXH_INCLUDE_ENERGY 2
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* ixh_last_error(void);
+/* Library version. */
+int ixh_version(int* major, int* minor);
+/* Number of visible HIP devices. */
+int ixh_device_count(int* count);
+
+/*
+ * The per-Context device buffers of a platform: what OpenMM's platform data owns for a Context and hands to a
+ * kernel's execute() (the posq array, the fixed-point force buffer, the energy accumulator, the box and the
+ * context parameter).  All pointers are device pointers.
+ */
+typedef struct {
+    int num_particles;
+    int precision;                 /* IXH_SINGLE / IXH_MIXED: posq is float4; IXH_DOUBLE: posq is double4 */
+    const void* posq;              /* x, y, z, (unused) per particle, in the particle order of the System */
+    const void* posq_correction;   /* IXH_MIXED: float4 low-order parts, position = posq + correction; else NULL */
+    long long* force;              /* component c of particle i at force[c*force_stride + i], 2^32 per kJ/mol/nm */
+    int force_stride;              /* >= num_particles */
+    double* energy;                /* the energy of the call is ADDED to energy[0] */
+    double box[9];                 /* periodic box vectors a, b, c (rows), in OpenMM's reduced form */
+    double lambda;                 /* current value of the alchemical context parameter */
+    void* stream;                  /* hipStream_t, or NULL for the default stream */
+} ixh_buffers;
+
+/*
+ * Device state: a stand-in for a platform's per-Context buffers, for hosts that do not bring their own (tests,
+ * benchmarks, the server).  It allocates the buffers of ixh_buffers and converts positions and forces.
+ */
+typedef struct ixh_state ixh_state;
+int ixh_state_create(int device, int num_particles, int precision, ixh_state** out);
+int ixh_state_destroy(ixh_state* state);
+/* Upload positions (3 doubles per particle). */
+int ixh_state_set_positions(ixh_state* state, const double* positions);
+/* Set the periodic box vectors (9 doubles, rows a, b, c). */
+int ixh_state_set_box(ixh_state* state, const double* box);
+/* Set the value of the alchemical context parameter used by the next calls. */
+int ixh_state_set_lambda(ixh_state* state, double lambda);
+/* Zero the force buffer and the energy accumulator (asynchronous, on the state's stream). */
+int ixh_state_clear(ixh_state* state);
+/* Wait for the state's stream and download the forces (3 doubles per particle) / the energy. */
+int ixh_state_get_forces(ixh_state* state, double* forces);
+int ixh_state_get_energy(ixh_state* state, double* energy);
+/* Wait for all work queued on the state's stream. */
+int ixh_state_synchronize(ixh_state* state);
+/* The buffers of the state, to pass to the execute() functions. */
+int ixh_state_get_buffers(ixh_state* state, ixh_buffers* out);
+
+/*
+ * A fused set of pair forces that share their geometry settings (InteraxPairForce.h:70-239).  The per-particle
+ * parameter arrays of the terms not in `terms` may be NULL.
+ */
+typedef struct {
+    int num_particles;
+    int terms;                      /* IXH_TERM_* bitmask of the forces in this handle */
+    int method;                     /* IXH_NO_CUTOFF, IXH_CUTOFF_NON_PERIODIC, IXH_CUTOFF_PERIODIC */
+    double cutoff;
+    int use_switching_function;
+    double switching_distance;
+    double softcore_radius;
+    const unsigned char* alchemical;   /* num_particles flags, or NULL for none */
+    int num_exclusions;
+    const int* exclusions;             /* 2*num_exclusions particle indices */
+    int precision;                     /* IXH_SINGLE, IXH_MIXED or IXH_DOUBLE */
+    /* InteraxDispersionForce: per particle (m1, m2, m3, alpha, width) */
+    const double* dispersion;
+    double damping_scale;
+    int use_long_range_correction;
+    /* InteraxExchangeForce: per particle (amplitude, width) */
+    const double* exchange;
+    int exchange_overlap_method;
+    /* InteraxShortRangeForce: per particle (k, l, width) */
+    const double* short_range;
+    int short_range_overlap_method;
+    /* Periodic box used to validate the cutoff (the System's default box); may be NULL if not periodic. */
+    const double* default_box;
+} ixh_pair_desc;
+
+typedef struct ixh_pair ixh_pair;
+/* CalcInterax*ForceKernel::initialize (ReferenceInteraxKernels.cpp:44-50, 104-112, 147-155). */
+int ixh_pair_create(int device, const ixh_pair_desc* desc, ixh_pair** out);
+int ixh_pair_destroy(ixh_pair* pair);
+/*
+ * CalcInterax*ForceKernel::copyParametersToContext (ReferenceInteraxKernels.cpp:98-112, 142-148, 185-191): new
+ * per-particle parameters and damping scale.  The number of particles and the overlap methods must not change.
+ * The long range correction is recomputed.
+ */
+int ixh_pair_set_parameters(ixh_pair* pair, const ixh_pair_desc* desc);
+/*
+ * CalcInterax*ForceKernel::execute (ReferenceInteraxKernels.cpp:79-96, 127-140, 170-183): add the forces and
+ * energy of the selected terms to the buffers.  The long range correction is not included: it is returned by
+ * ixh_pair_long_range_energy(), as the GPU platforms return it from execute().  Asynchronous on buffers->stream.
+ */
+int ixh_pair_execute(ixh_pair* pair, const ixh_buffers* buffers, int terms, int flags);
+/* Dispersion long range correction: coefficients (InteraxDispersionForceImpl.cpp:86-149) and the energy
+   (coefficient + lambda*alchemicalCoefficient)/V (ReferenceInteraxKernels.cpp:93-94). */
+int ixh_pair_long_range_coefficients(ixh_pair* pair, double* coefficient, double* alchemical_coefficient);
+int ixh_pair_long_range_energy(ixh_pair* pair, const double* box, double lambda, double* energy);
+/* Number of interacting pairs of the last execute() (waits for it). */
+int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
+
+/* InteraxThreeBodyForce (InteraxThreeBodyForce.h:75-258). */
+typedef struct {
+    int num_particles;                 /* of the System */
+    int num_sites;
+    const int* site_offsets;           /* num_sites+1 offsets into site_particles */
+    const int* site_particles;
+    const double* alpha;
+    const double* frequency;
+    const double* width;
+    const int* molecule;               /* negative: a molecule of its own */
+    const unsigned char* alchemical;   /* or NULL */
+    double cutoff;
+    double scale_factor;
+    int damping_method;                /* IXH_DAMPING_NONE or IXH_DAMPING_TANG_TOENNIES */
+    double damping_scale;
+    double centroid_damping_radius;
+    int periodic;
+    int precision;
+} ixh_threebody_desc;
+
+typedef struct ixh_threebody ixh_threebody;
+/* CalcInteraxThreeBodyForceKernel::initialize, with the checks of InteraxThreeBodyForceImpl.cpp:42-62. */
+int ixh_threebody_create(int device, const ixh_threebody_desc* desc, ixh_threebody** out);
+int ixh_threebody_destroy(ixh_threebody* threebody);
+/* copyParametersToContext (ReferenceInteraxKernels.cpp:215-217): alpha, frequency, width, scale factor, damping
+   scale and a nonzero centroid damping radius may change. */
+int ixh_threebody_set_parameters(ixh_threebody* threebody, const ixh_threebody_desc* desc);
+/* execute (ReferenceInteraxKernels.cpp:210-213). */
+int ixh_threebody_execute(ixh_threebody* threebody, const ixh_buffers* buffers, int flags);
+/* Number of interacting triples of the last execute() (waits for it). */
+int ixh_threebody_last_triple_count(ixh_threebody* threebody, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

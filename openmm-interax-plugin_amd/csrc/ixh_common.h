@@ -1,0 +1,151 @@
+// Shared host and device helpers of the interax_hip library (gfx950).
+#ifndef IXH_COMMON_H
+#define IXH_COMMON_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "interax_hip.h"
+
+namespace ixh {
+
+// ------------------------------------------------------------------------------------------------------------
+// Errors: every C entry point catches ixh::Error and stores its message for ixh_last_error().
+// ------------------------------------------------------------------------------------------------------------
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int code, const std::string& message) : std::runtime_error(message), code(code) {}
+};
+
+void set_last_error(const std::string& message);
+
+#define IXH_HIP_CHECK(call)                                                                                   \
+    do {                                                                                                      \
+        hipError_t ixh_status_ = (call);                                                                      \
+        if (ixh_status_ != hipSuccess)                                                                        \
+            throw ::ixh::Error(IXH_ERROR_HIP, std::string(#call) + ": " + hipGetErrorString(ixh_status_));    \
+    } while (0)
+
+// Run a body, translating exceptions into a status code.
+template <class F>
+int guarded(F&& body) {
+    try {
+        body();
+        return IXH_OK;
+    }
+    catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    }
+    catch (const std::exception& e) {
+        set_last_error(e.what());
+        return IXH_ERROR_HIP;
+    }
+    catch (...) {
+        set_last_error("unknown error");
+        return IXH_ERROR_HIP;
+    }
+}
+
+// Device memory owned by a handle.
+template <class T>
+struct DeviceArray {
+    T* ptr = nullptr;
+    size_t count = 0;
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() { release(); }
+    void release() {
+        if (ptr != nullptr)
+            (void) hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    // Grow to at least n elements (contents are not kept).
+    void reserve(size_t n) {
+        if (n <= count && ptr != nullptr)
+            return;
+        release();
+        size_t bytes = (n == 0 ? 1 : n)*sizeof(T);
+        IXH_HIP_CHECK(hipMalloc((void**) &ptr, bytes));
+        count = n;
+    }
+    void upload(const T* host, size_t n, hipStream_t stream) {
+        reserve(n);
+        if (n > 0)
+            IXH_HIP_CHECK(hipMemcpyAsync(ptr, host, n*sizeof(T), hipMemcpyHostToDevice, stream));
+    }
+};
+
+// Sets the device for the lifetime of the object and restores the previous one.
+struct DeviceGuard {
+    int previous = 0;
+    explicit DeviceGuard(int device) {
+        IXH_HIP_CHECK(hipGetDevice(&previous));
+        if (previous != device)
+            IXH_HIP_CHECK(hipSetDevice(device));
+    }
+    ~DeviceGuard() { (void) hipSetDevice(previous); }
+};
+
+constexpr double FIXED_POINT_SCALE = 4294967296.0;   // 2^32 units per kJ/mol/nm, as OpenMM's GPU platforms
+
+// ------------------------------------------------------------------------------------------------------------
+// Device helpers
+// ------------------------------------------------------------------------------------------------------------
+
+// Wave64 lane rotation: lane l receives the value of lane (l+1) mod 64.  DPP wave_rol:1 is a GFX9-family control
+// (CDNA keeps the wave-wide shifts and rotations that RDNA dropped).
+__device__ __forceinline__ float wave_rotate_down(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int wave_rotate_down(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x134, 0xF, 0xF, false);
+}
+__device__ __forceinline__ unsigned wave_rotate_down(unsigned v) {
+    return (unsigned) __builtin_amdgcn_update_dpp(0, (int) v, 0x134, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double wave_rotate_down(double v) {
+    long long bits = __builtin_bit_cast(long long, v);
+    int lo = (int) (bits & 0xffffffffLL), hi = (int) (bits >> 32);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x134, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x134, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (long long) (((unsigned long long) (unsigned) hi << 32) | (unsigned) lo));
+}
+
+// Convert a force component to 64-bit fixed point.
+__device__ __forceinline__ long long to_fixed(double v) {
+    return (long long) (v*FIXED_POINT_SCALE);
+}
+__device__ __forceinline__ long long to_fixed(float v) {
+    return (long long) ((double) v*FIXED_POINT_SCALE);
+}
+
+__device__ __forceinline__ void add_fixed(long long* p, long long v) {
+    if (v != 0)
+        atomicAdd((unsigned long long*) p, (unsigned long long) v);
+}
+
+// Fast FP32 intrinsics: v_rcp_f32, v_rsq_f32, v_exp_f32 (2^x).
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0/x; }
+__device__ __forceinline__ double fast_rsqrt(double x) { return 1.0/sqrt(x); }
+__device__ __forceinline__ double fast_exp2(double x) { return exp2(x); }
+
+// Wave-wide sum of a double, result valid in every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int offset = 32; offset > 0; offset >>= 1)
+        v += __shfl_xor(v, offset, 64);
+    return v;
+}
+
+}  // namespace ixh
+
+#endif

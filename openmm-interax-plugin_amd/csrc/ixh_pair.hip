@@ -1,0 +1,1422 @@
+// The fused InteraxFF pair forces on gfx950: neighbour list, exclusion masks, the rotating-tile pair kernel and the
+// dispersion long range correction.
+//
+// Reference behaviour (the contract): ReferenceInteraxPairIxn.h:53-99 (pairs, cutoff, soft core, switching,
+// alchemical scaling), ReferenceInteraxKernels.cpp:41-198 (the three force kernels), InteraxPairForce.cpp:113-146
+// (validation), InteraxDispersionForceImpl.cpp:86-149 (long range correction).
+//
+// Layout in HBM.  Particles are sorted along a Morton curve of their fractional coordinates into blocks of 64 (one
+// wavefront); per sorted slot the kernel reads one 16-byte position in the frame of its block and three 16-byte
+// parameter vectors.  A tile is a pair of blocks (I <= J) whose bounding boxes are within cutoff + skin; it is
+// evaluated by one wavefront as 64 rotation steps: at step s lane l pairs its particle I*64+l with J*64+(l+s)%64,
+// reading j from LDS, so every lane owns a distinct i and a distinct j at every step and the j force accumulators
+// follow their particle around the wave with a DPP rotation.  Excluded pairs, padding slots and the lower triangle
+// of diagonal tiles are removed by a 64x64 bit mask per tile (only tiles that need one have one).
+//
+// Determinism.  Each tile converts its FP32 force sums to 64-bit fixed point before they are added with integer
+// atomics, and writes its FP64 energy to its own slot; the slots are summed in tile order.  Results therefore do
+// not depend on the launch shape, on which wave ran which tile, or on other systems sharing a launch.
+#include "ixh_pair_math.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+#include <map>
+#include <numeric>
+#include <tuple>
+#include <vector>
+
+using namespace ixh;
+
+namespace {
+
+constexpr int BLOCK = 64;
+constexpr double SKIN = 0.1;   // neighbour list padding (nm); rebuilt when a particle has moved skin/2
+
+template <typename Real>
+struct Vec4T;
+template <>
+struct Vec4T<float> {
+    typedef float4 type;
+};
+template <>
+struct Vec4T<double> {
+    typedef double4 type;
+};
+
+// ------------------------------------------------------------------------------------------------------------
+// Geometry helpers
+// ------------------------------------------------------------------------------------------------------------
+
+struct BoxD {
+    double ax, bx, by, cx, cy, cz;   // reduced form: a = (ax,0,0), b = (bx,by,0), c = (cx,cy,cz)
+};
+
+// Minimum image of d, reduced along c, b, a in turn, with every operation rounded as in
+// ReferenceForce::getDeltaRPeriodic (no fused multiply-add), so that the result is bit-identical to the reference.
+__device__ __forceinline__ void reference_min_image(double& dx, double& dy, double& dz, const BoxD& b) {
+    double s2 = floor(__dadd_rn(__ddiv_rn(dz, b.cz), 0.5));
+    dx = __dsub_rn(dx, __dmul_rn(b.cx, s2));
+    dy = __dsub_rn(dy, __dmul_rn(b.cy, s2));
+    dz = __dsub_rn(dz, __dmul_rn(b.cz, s2));
+    double s1 = floor(__dadd_rn(__ddiv_rn(dy, b.by), 0.5));
+    dx = __dsub_rn(dx, __dmul_rn(b.bx, s1));
+    dy = __dsub_rn(dy, __dmul_rn(b.by, s1));
+    double s0 = floor(__dadd_rn(__ddiv_rn(dx, b.ax), 0.5));
+    dx = __dsub_rn(dx, __dmul_rn(b.ax, s0));
+}
+
+template <typename Real>
+__device__ __forceinline__ void min_image(Real& dx, Real& dy, Real& dz, Real ax, Real bx, Real by, Real cx, Real cy, Real cz,
+                                          Real iax, Real iby, Real icz) {
+    Real s2 = floor(dz*icz+Real(0.5));
+    dx -= s2*cx;
+    dy -= s2*cy;
+    dz -= s2*cz;
+    Real s1 = floor(dy*iby+Real(0.5));
+    dx -= s1*bx;
+    dy -= s1*by;
+    Real s0 = floor(dx*iax+Real(0.5));
+    dx -= s0*ax;
+}
+
+// FP64 position of particle i from the platform's posq layout.
+__device__ __forceinline__ void load_position(const void* posq, const float4* correction, int precision, int i,
+                                              double& x, double& y, double& z) {
+    if (precision == IXH_DOUBLE) {
+        double4 p = ((const double4*) posq)[i];
+        x = p.x;
+        y = p.y;
+        z = p.z;
+    }
+    else {
+        float4 p = ((const float4*) posq)[i];
+        x = p.x;
+        y = p.y;
+        z = p.z;
+        if (correction != nullptr) {
+            float4 c = correction[i];
+            x += c.x;
+            y += c.y;
+            z += c.z;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Neighbour list kernels (run when the list is rebuilt)
+// ------------------------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned spread_bits(unsigned v) {
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+struct KeyArgs {
+    const void* posq;
+    const float4* correction;
+    int precision;
+    int n;
+    int periodic;
+    BoxD box;
+    double lo[3], scale[3];   // nonperiodic: fractional coordinate = (x - lo)*scale
+};
+
+__global__ void k_morton_keys(KeyArgs a, unsigned* keys, int* values, double* pos3) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= a.n)
+        return;
+    double x, y, z;
+    load_position(a.posq, a.correction, a.precision, i, x, y, z);
+    pos3[3*i] = x;
+    pos3[3*i+1] = y;
+    pos3[3*i+2] = z;
+    double f[3];
+    if (a.periodic) {
+        double fz = z/a.box.cz;
+        double fy = (y-fz*a.box.cy)/a.box.by;
+        double fx = (x-fz*a.box.cx-fy*a.box.bx)/a.box.ax;
+        f[0] = fx-floor(fx);
+        f[1] = fy-floor(fy);
+        f[2] = fz-floor(fz);
+    }
+    else {
+        f[0] = (x-a.lo[0])*a.scale[0];
+        f[1] = (y-a.lo[1])*a.scale[1];
+        f[2] = (z-a.lo[2])*a.scale[2];
+    }
+    unsigned c[3];
+    for (int k = 0; k < 3; k++) {
+        int v = (int) (f[k]*1024.0);
+        c[k] = (unsigned) min(max(v, 0), 1023);
+    }
+    keys[i] = spread_bits(c[0]) | (spread_bits(c[1]) << 1) | (spread_bits(c[2]) << 2);
+    values[i] = i;
+}
+
+__global__ void k_bounding_box(const void* posq, const float4* correction, int precision, int n, double* out) {
+    // one workgroup of 256 threads: min and max of the coordinates
+    __shared__ double sMin[3][256], sMax[3][256];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double p[3];
+        load_position(posq, correction, precision, i, p[0], p[1], p[2]);
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], p[k]);
+            hi[k] = fmax(hi[k], p[k]);
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        sMin[k][threadIdx.x] = lo[k];
+        sMax[k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride)
+            for (int k = 0; k < 3; k++) {
+                sMin[k][threadIdx.x] = fmin(sMin[k][threadIdx.x], sMin[k][threadIdx.x+stride]);
+                sMax[k][threadIdx.x] = fmax(sMax[k][threadIdx.x], sMax[k][threadIdx.x+stride]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) {
+            out[k] = sMin[k][0];
+            out[3+k] = sMax[k][0];
+        }
+}
+
+// Slot bookkeeping after the sort: padding slots get -1; slotOf is the inverse permutation.
+__global__ void k_finish_perm(int* perm, int n, int padded, int* slotOf) {
+    int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= padded)
+        return;
+    if (s >= n)
+        perm[s] = -1;
+    else
+        slotOf[perm[s]] = s;
+}
+
+template <typename Real>
+__global__ void k_gather_params(const int* perm, int padded, const typename Vec4T<Real>::type* pA, const typename Vec4T<Real>::type* pB,
+                                const typename Vec4T<Real>::type* pC, const int* flags, typename Vec4T<Real>::type* sA,
+                                typename Vec4T<Real>::type* sB, typename Vec4T<Real>::type* sC, int* sFlags) {
+    int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= padded)
+        return;
+    int i = perm[s];
+    typedef typename Vec4T<Real>::type R4;
+    if (i < 0) {
+        R4 zero = {0, 0, 0, 0};
+        sA[s] = zero;
+        sB[s] = zero;
+        sC[s] = zero;
+        sFlags[s] = -1;
+        return;
+    }
+    sA[s] = pA[i];
+    sB[s] = pB[i];
+    sC[s] = pC[i];
+    sFlags[s] = flags[i];
+}
+
+// Bounding box of each block, in the frame of its first particle (periodic images taken about it).
+struct BlockBoundsArgs {
+    const double* pos3;
+    const int* perm;
+    int numBlocks;
+    int periodic;
+    BoxD box;
+};
+
+__global__ void k_block_bounds(BlockBoundsArgs a, double* bounds /* 6 per block: center, half extent */) {
+    int b = blockIdx.x*blockDim.x+threadIdx.x;
+    if (b >= a.numBlocks)
+        return;
+    int first = a.perm[b*BLOCK];
+    double o[3] = {a.pos3[3*first], a.pos3[3*first+1], a.pos3[3*first+2]};
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int k = 0; k < BLOCK; k++) {
+        int i = a.perm[b*BLOCK+k];
+        if (i < 0)
+            break;
+        double d[3] = {a.pos3[3*i]-o[0], a.pos3[3*i+1]-o[1], a.pos3[3*i+2]-o[2]};
+        if (a.periodic)
+            reference_min_image(d[0], d[1], d[2], a.box);
+        for (int c = 0; c < 3; c++) {
+            lo[c] = fmin(lo[c], d[c]);
+            hi[c] = fmax(hi[c], d[c]);
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        bounds[6*b+c] = o[c]+0.5*(lo[c]+hi[c]);
+        bounds[6*b+3+c] = 0.5*(hi[c]-lo[c]);
+    }
+}
+
+// Smallest distance between the boxes of blocks I and J, over the periodic images within one lattice vector of the
+// reduced centre difference (a conservative test: it never drops a pair that can be within range).
+__device__ double block_distance2(const double* bounds, int I, int J, int periodic, const BoxD& box) {
+    double d[3] = {bounds[6*J]-bounds[6*I], bounds[6*J+1]-bounds[6*I+1], bounds[6*J+2]-bounds[6*I+2]};
+    double h[3] = {bounds[6*I+3]+bounds[6*J+3], bounds[6*I+4]+bounds[6*J+4], bounds[6*I+5]+bounds[6*J+5]};
+    if (!periodic) {
+        double s = 0;
+        for (int c = 0; c < 3; c++) {
+            double g = fmax(0.0, fabs(d[c])-h[c]);
+            s += g*g;
+        }
+        return s;
+    }
+    reference_min_image(d[0], d[1], d[2], box);
+    double best = 1e300;
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) {
+                double e[3] = {d[0]+i*box.ax+j*box.bx+k*box.cx, d[1]+j*box.by+k*box.cy, d[2]+k*box.cz};
+                double s = 0;
+                for (int c = 0; c < 3; c++) {
+                    double g = fmax(0.0, fabs(e[c])-h[c]);
+                    s += g*g;
+                }
+                best = fmin(best, s);
+            }
+    return best;
+}
+
+// Count (pass 0) or write (pass 1) the tiles of each row I: J >= I with box distance below the list cutoff.
+__global__ void k_find_tiles(const double* bounds, int numBlocks, int periodic, BoxD box, double listCutoff2, int* rowCount,
+                             const int* rowStart, int2* tiles) {
+    int I = blockIdx.x;
+    int lane = threadIdx.x;   // 64 threads
+    int written = 0;
+    for (int J0 = I; J0 < numBlocks; J0 += 64) {
+        int J = J0+lane;
+        bool hit = (J < numBlocks) && block_distance2(bounds, I, J, periodic, box) < listCutoff2;
+        unsigned long long ballot = __ballot(hit);
+        if (tiles != nullptr && hit) {
+            int pos = written + __popcll(ballot & ((1ULL << lane)-1));
+            tiles[rowStart[I]+pos] = make_int2(I, J);
+        }
+        written += __popcll(ballot);
+    }
+    if (tiles == nullptr && lane == 0)
+        rowCount[I] = written;
+}
+
+// Index of tile (I, J) in the row-sorted tile list, or -1.
+__device__ __forceinline__ int find_tile(const int2* tiles, const int* rowStart, int I, int J) {
+    int lo = rowStart[I], hi = rowStart[I+1];
+    while (lo < hi) {
+        int mid = (lo+hi) >> 1;
+        int v = tiles[mid].y;
+        if (v == J)
+            return mid;
+        if (v < J)
+            lo = mid+1;
+        else
+            hi = mid;
+    }
+    return -1;
+}
+
+// Mark the tiles that need a mask: those holding an excluded pair or padding slots.
+__global__ void k_mark_mask_tiles(const int* exclusions, int numExclusions, const int* slotOf, const int2* tiles,
+                                  const int* rowStart, int numTiles, int paddingBlock, int* needMask) {
+    int e = blockIdx.x*blockDim.x+threadIdx.x;
+    if (e < numExclusions) {
+        int p = exclusions[2*e], q = exclusions[2*e+1];
+        if (p != q) {
+            int bp = slotOf[p]/BLOCK, bq = slotOf[q]/BLOCK;
+            int t = find_tile(tiles, rowStart, min(bp, bq), max(bp, bq));
+            if (t >= 0)
+                needMask[t] = 1;
+        }
+    }
+    if (e < numTiles && paddingBlock >= 0) {
+        int2 tile = tiles[e];
+        if (tile.x == paddingBlock || tile.y == paddingBlock)
+            needMask[e] = 1;
+    }
+}
+
+__global__ void k_init_masks(const int2* tiles, const int* maskIndex, int numTiles, int paddingBlock, int validInLast,
+                             unsigned long long* masks) {
+    int t = blockIdx.x;
+    int row = threadIdx.x;   // 64 threads
+    if (t >= numTiles || maskIndex[t] < 0)
+        return;
+    int2 tile = tiles[t];
+    unsigned long long m = 0;
+    if (tile.y == paddingBlock)
+        m |= ~0ULL << validInLast;
+    if (tile.x == paddingBlock && row >= validInLast)
+        m = ~0ULL;
+    masks[(size_t) maskIndex[t]*BLOCK+row] = m;
+}
+
+__global__ void k_set_exclusion_bits(const int* exclusions, int numExclusions, const int* slotOf, const int2* tiles,
+                                     const int* rowStart, const int* maskIndex, unsigned long long* masks) {
+    int e = blockIdx.x*blockDim.x+threadIdx.x;
+    if (e >= numExclusions)
+        return;
+    int p = exclusions[2*e], q = exclusions[2*e+1];
+    if (p == q)
+        return;
+    int sp = slotOf[p], sq = slotOf[q];
+    if (sp > sq) {
+        int tmp = sp;
+        sp = sq;
+        sq = tmp;
+    }
+    int I = sp/BLOCK, J = sq/BLOCK;
+    int t = find_tile(tiles, rowStart, I, J);
+    if (t < 0)
+        return;
+    unsigned long long* m = masks+(size_t) maskIndex[t]*BLOCK;
+    atomicOr(&m[sp%BLOCK], 1ULL << (sq%BLOCK));
+    if (I == J)
+        atomicOr(&m[sq%BLOCK], 1ULL << (sp%BLOCK));
+}
+
+// Largest squared displacement since the list was built exceeds (skin/2)^2?
+__global__ void k_check_displacement(const void* posq, const float4* correction, int precision, int n, const double* buildPos,
+                                     int periodic, BoxD box, double limit2, int* flag) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n)
+        return;
+    double x, y, z;
+    load_position(posq, correction, precision, i, x, y, z);
+    double dx = x-buildPos[3*i], dy = y-buildPos[3*i+1], dz = z-buildPos[3*i+2];
+    if (periodic)
+        reference_min_image(dx, dy, dz, box);
+    if (dx*dx+dy*dy+dz*dz > limit2)
+        *flag = 1;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Per-evaluation kernels
+// ------------------------------------------------------------------------------------------------------------
+
+// Block frames: the centre of block b is the position of its first particle; every particle is stored relative to
+// the centre of its block (nearest image), in the arithmetic type.
+template <typename Real>
+__global__ void k_local_positions(const void* posq, const float4* correction, int precision, const int* perm, int padded,
+                                  int periodic, BoxD box, double* centers, typename Vec4T<Real>::type* local) {
+    int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= padded)
+        return;
+    int b = s/BLOCK;
+    int first = perm[b*BLOCK];
+    double cx, cy, cz;
+    load_position(posq, correction, precision, first, cx, cy, cz);
+    if (s%BLOCK == 0) {
+        centers[3*b] = cx;
+        centers[3*b+1] = cy;
+        centers[3*b+2] = cz;
+    }
+    int i = perm[s];
+    typedef typename Vec4T<Real>::type R4;
+    if (i < 0) {
+        R4 far = {Real(0), Real(0), Real(0), Real(0)};
+        local[s] = far;
+        return;
+    }
+    double x, y, z;
+    load_position(posq, correction, precision, i, x, y, z);
+    double dx = x-cx, dy = y-cy, dz = z-cz;
+    if (periodic)
+        reference_min_image(dx, dy, dz, box);
+    R4 v = {Real(dx), Real(dy), Real(dz), Real(0)};
+    local[s] = v;
+}
+
+template <typename Real>
+struct TileArgs {
+    typedef typename Vec4T<Real>::type R4;
+    const int2* tiles;
+    int numTiles;
+    const int* maskIndex;
+    const unsigned long long* masks;
+    const R4* local;
+    const double* centers;
+    const R4* pA;    // m1, m2, m3, alpha
+    const R4* pB;    // dispersion width, exchange amplitude, exchange width, 1/exchange width
+    const R4* pC;    // k, l, short-range width, 1/short-range width
+    const int* flags;   // original index | alchemical << 31, -1 for padding
+    int n;
+    // geometry
+    int method;
+    int periodic;
+    BoxD box;
+    Real boxR[9];   // ax, bx, by, cx, cy, cz, 1/ax, 1/by, 1/cz
+    Real cutoff2;
+    double cutoff;
+    Real band;      // |r^2 - rc^2| < band*rc^2 is decided exactly in FP64
+    int useSwitch;
+    Real switchDistance, invSwitchWidth;
+    int useSoftcore;
+    Real softcoreTerm;   // (1 - lambda) R^6
+    Real lambda;
+    PairSettings settings;
+    // the platform buffers
+    const void* posq;
+    const float4* correction;
+    int precision;
+    long long* force;
+    int stride;
+    double* tileEnergy;
+    int* tileCount;
+};
+
+// Exact membership test of the reference (ReferenceInteraxPairIxn.h:64-70) in FP64.
+__device__ __noinline__ bool exact_within_cutoff(const void* posq, const float4* correction, int precision, int periodic,
+                                                 BoxD box, double cutoff, int oi, int oj) {
+    double xi, yi, zi, xj, yj, zj;
+    load_position(posq, correction, precision, oi, xi, yi, zi);
+    load_position(posq, correction, precision, oj, xj, yj, zj);
+    double dx = __dsub_rn(xj, xi), dy = __dsub_rn(yj, yi), dz = __dsub_rn(zj, zi);
+    if (periodic)
+        reference_min_image(dx, dy, dz, box);
+    double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+    return __dsqrt_rn(r2) < cutoff;
+}
+
+template <typename Real, int TERMS>
+__global__ void __launch_bounds__(256) k_pair_tiles(TileArgs<Real> a) {
+    typedef typename Vec4T<Real>::type R4;
+    __shared__ R4 sPos[4][BLOCK], sA[4][BLOCK], sB[4][BLOCK], sC[4][BLOCK];
+    __shared__ int sFlags[4][BLOCK];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const long long wave = (long long) blockIdx.x*4+w;
+    const long long numWaves = (long long) gridDim.x*4;
+    const long long T = a.numTiles;
+    const long long begin = T*wave/numWaves, end = T*(wave+1)/numWaves;
+
+    int curI = -1;
+    R4 pi = {0, 0, 0, 0}, iA = {0, 0, 0, 0}, iB = {0, 0, 0, 0}, iC = {0, 0, 0, 0};
+    int flagI = -1;
+    long long fixI[3] = {0, 0, 0};
+    PairParticle<Real> P, Q;
+
+    for (long long t = begin; t < end; t++) {
+        const int2 tile = a.tiles[t];
+        const int I = tile.x, J = tile.y;
+        if (I != curI) {
+            if (curI >= 0 && flagI >= 0) {
+                int oi = flagI & 0x7fffffff;
+                add_fixed(&a.force[oi], fixI[0]);
+                add_fixed(&a.force[a.stride+oi], fixI[1]);
+                add_fixed(&a.force[2*a.stride+oi], fixI[2]);
+            }
+            fixI[0] = fixI[1] = fixI[2] = 0;
+            curI = I;
+            int slot = I*BLOCK+lane;
+            pi = a.local[slot];
+            iA = a.pA[slot];
+            iB = a.pB[slot];
+            iC = a.pC[slot];
+            flagI = a.flags[slot];
+            P.m1 = iA.x; P.m2 = iA.y; P.m3 = iA.z; P.alpha = iA.w;
+            P.wd = iB.x; P.amp = iB.y; P.wx = iB.z; P.iwx = iB.w;
+            P.k = iC.x; P.l = iC.y; P.ws = iC.z; P.iws = iC.w;
+        }
+        // shift of block J into the frame of block I
+        double sx = a.centers[3*J]-a.centers[3*I], sy = a.centers[3*J+1]-a.centers[3*I+1], sz = a.centers[3*J+2]-a.centers[3*I+2];
+        if (a.periodic)
+            reference_min_image(sx, sy, sz, a.box);
+        {
+            int slot = J*BLOCK+lane;
+            R4 pj = a.local[slot];
+            pj.x += Real(sx);
+            pj.y += Real(sy);
+            pj.z += Real(sz);
+            sPos[w][lane] = pj;
+            sA[w][lane] = a.pA[slot];
+            sB[w][lane] = a.pB[slot];
+            sC[w][lane] = a.pC[slot];
+            sFlags[w][lane] = a.flags[slot];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int mi = a.maskIndex[t];
+        const unsigned long long mrow = (mi >= 0 ? a.masks[(size_t) mi*BLOCK+lane] : 0ULL);
+        const bool diag = (I == J);
+        const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
+        const bool alchI = (flagI < 0 ? false : (flagI >> 31) != 0);
+        const int origI = flagI & 0x7fffffff;
+        Real fi[3] = {0, 0, 0};
+        Real fj[3] = {0, 0, 0};
+        double energy = 0;
+        int count = 0;
+        for (int s = sBegin; s < sEnd; s++) {
+            const int jl = (lane+s) & 63;
+            bool skip = ((mrow >> jl) & 1ULL) != 0;
+            if (diag && s == 32 && lane >= 32)
+                skip = true;
+            const R4 pj = sPos[w][jl];
+            Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
+            if (a.periodic)
+                min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
+                                a.boxR[7], a.boxR[8]);
+            const Real r2 = dx*dx+dy*dy+dz*dz;
+            bool interact = !skip;
+            if (a.method != IXH_NO_CUTOFF && interact) {
+                Real margin = a.band*a.cutoff2;
+                if (r2 >= a.cutoff2+margin)
+                    interact = false;
+                else if (r2 > a.cutoff2-margin) {
+                    int fl = sFlags[w][jl];
+                    interact = exact_within_cutoff(a.posq, a.correction, a.precision, a.periodic, a.box, a.cutoff, origI,
+                                                   fl & 0x7fffffff);
+                }
+            }
+            if (interact) {
+                const int flagJ = sFlags[w][jl];
+                const R4 jA = sA[w][jl], jB = sB[w][jl], jC = sC[w][jl];
+                Q.m1 = jA.x; Q.m2 = jA.y; Q.m3 = jA.z; Q.alpha = jA.w;
+                Q.wd = jB.x; Q.amp = jB.y; Q.wx = jB.z; Q.iwx = jB.w;
+                Q.k = jC.x; Q.l = jC.y; Q.ws = jC.z; Q.iws = jC.w;
+                const bool alchJ = (flagJ >> 31) != 0;
+                const bool alchPair = (alchI != alchJ);
+                const Real invR = fast_rsqrt(r2);
+                const Real r = r2*invR;
+                Real rEff = r, invREff = invR;
+                if (a.useSoftcore && alchPair) {
+                    // ReferenceInteraxPairIxn.h:72-77
+                    Real r6 = r2*r2*r2;
+                    rEff = pow(r6+a.softcoreTerm, Real(1.0/6.0));
+                    invREff = Real(1)/rEff;
+                }
+                Real e, dEdR;
+                const bool iFirst = origI < (flagJ & 0x7fffffff);
+                pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
+                if (rEff != r) {
+                    Real ratio = r*invREff;
+                    Real ratio2 = ratio*ratio;
+                    dEdR *= ratio2*ratio2*ratio;
+                }
+                if (a.useSwitch && r > a.switchDistance) {
+                    Real x = (r-a.switchDistance)*a.invSwitchWidth;
+                    Real sw = 1+x*x*x*(Real(-10)+x*(Real(15)-x*Real(6)));
+                    Real dsw = x*x*(Real(-30)+x*(Real(60)-x*Real(30)))*a.invSwitchWidth;
+                    dEdR = dEdR*sw + e*dsw;
+                    e *= sw;
+                }
+                const Real scale = (alchPair ? a.lambda : Real(1));
+                energy += (double) (scale*e);
+                const Real fs = scale*dEdR*invR;
+                fi[0] += dx*fs;
+                fi[1] += dy*fs;
+                fi[2] += dz*fs;
+                fj[0] -= dx*fs;
+                fj[1] -= dy*fs;
+                fj[2] -= dz*fs;
+                count++;
+            }
+            fj[0] = wave_rotate_down(fj[0]);
+            fj[1] = wave_rotate_down(fj[1]);
+            fj[2] = wave_rotate_down(fj[2]);
+        }
+        // lane l now holds the accumulator of j = (l + sEnd) % 64
+        {
+            const int jl = (lane+sEnd) & 63;
+            const int flagJ = sFlags[w][jl];
+            if (flagJ >= 0) {
+                int oj = flagJ & 0x7fffffff;
+                add_fixed(&a.force[oj], to_fixed(fj[0]));
+                add_fixed(&a.force[a.stride+oj], to_fixed(fj[1]));
+                add_fixed(&a.force[2*a.stride+oj], to_fixed(fj[2]));
+            }
+        }
+        fixI[0] += to_fixed(fi[0]);
+        fixI[1] += to_fixed(fi[1]);
+        fixI[2] += to_fixed(fi[2]);
+        double tileEnergy = wave_sum(energy);
+        int tileCount = (int) wave_sum((double) count);
+        if (lane == 0) {
+            a.tileEnergy[t] = tileEnergy;
+            a.tileCount[t] = tileCount;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (curI >= 0 && flagI >= 0) {
+        int oi = flagI & 0x7fffffff;
+        add_fixed(&a.force[oi], fixI[0]);
+        add_fixed(&a.force[a.stride+oi], fixI[1]);
+        add_fixed(&a.force[2*a.stride+oi], fixI[2]);
+    }
+}
+
+// Sum the tile energies in tile order (one workgroup, fixed reduction tree) and add them to the energy buffer.
+__global__ void k_reduce_tiles(const double* tileEnergy, const int* tileCount, int numTiles, double* energy, long long* pairCount,
+                               int addEnergy) {
+    __shared__ double sE[256];
+    __shared__ long long sC[256];
+    double e = 0;
+    long long c = 0;
+    // contiguous chunks per thread, so the sum is a fixed function of the tile order
+    int per = (numTiles+255)/256;
+    int begin = threadIdx.x*per, end = min(numTiles, begin+per);
+    for (int t = begin; t < end; t++) {
+        e += tileEnergy[t];
+        c += tileCount[t];
+    }
+    sE[threadIdx.x] = e;
+    sC[threadIdx.x] = c;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) {
+            sE[threadIdx.x] += sE[threadIdx.x+stride];
+            sC[threadIdx.x] += sC[threadIdx.x+stride];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (addEnergy)
+            energy[0] += sE[0];
+        *pairCount = sC[0];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Long range correction (InteraxDispersionForceImpl.cpp:86-149), FP64 on the device
+// ------------------------------------------------------------------------------------------------------------
+
+struct LrcArgs {
+    const double* classes;   // 7 per class: m1, m2, m3, alpha, width, alchemical, count
+    long long numClasses;
+    double cutoff, switchDistance, switchWidth, dampingScale;
+    int useSwitch;
+    double nodes[32], weights[32];
+};
+
+__device__ double lrc_pair_energy(double r, const double* c1, const double* c2, double beta) {
+    PairParticle<double> P, Q;
+    P.m1 = c1[0]; P.m2 = c1[1]; P.m3 = c1[2]; P.alpha = c1[3]; P.wd = c1[4];
+    Q.m1 = c2[0]; Q.m2 = c2[1]; Q.m3 = c2[2]; Q.alpha = c2[3]; Q.wd = c2[4];
+    PairSettings set;
+    set.damping_scale = beta;
+    set.damping_scale_f = (float) beta;
+    set.shared_overlap = 0;
+    double e, d;
+    pair_energy<double, IXH_TERM_DISPERSION>(r, 1.0/r, P, Q, true, set, e, d);
+    return e;
+}
+
+__global__ void k_long_range(LrcArgs a, double* partial /* 2 per thread */) {
+    long long tid = (long long) blockIdx.x*blockDim.x+threadIdx.x;
+    long long nthreads = (long long) gridDim.x*blockDim.x;
+    long long C = a.numClasses;
+    long long totalPairs = C*(C+1)/2;
+    long long begin = totalPairs*tid/nthreads, end = totalPairs*(tid+1)/nthreads;
+    double sum = 0, alch = 0;
+    if (begin < end) {
+        // decode begin into (i, j), pairs ordered i <= j, row i has C - i entries
+        long long i = 0, rowStart = 0;
+        {
+            // solve rowStart(i) = i*C - i(i-1)/2 <= begin
+            double cc = (double) C;
+            double guess = floor(((2*cc+1)-sqrt((2*cc+1)*(2*cc+1)-8.0*(double) begin))/2);
+            i = (long long) fmax(0.0, guess-2);
+            while (i+1 < C && (i+1)*C-(i+1)*i/2 <= begin)
+                i++;
+            rowStart = i*C-i*(i-1)/2;
+        }
+        long long j = i+(begin-rowStart);
+        for (long long p = begin; p < end; p++) {
+            const double* c1 = a.classes+7*i;
+            const double* c2 = a.classes+7*j;
+            if (c1[3]*c2[3] != 0) {
+                double integral = 0;
+                for (int k = 0; k < 32; k++) {
+                    double u = a.nodes[k];
+                    integral += a.weights[k]*lrc_pair_energy(a.cutoff/u, c1, c2, a.dampingScale)*a.cutoff*a.cutoff*a.cutoff/(u*u*u*u);
+                    if (a.useSwitch) {
+                        double x = a.nodes[k];
+                        double r = a.switchDistance+x*a.switchWidth;
+                        double oneMinusSwitch = x*x*x*(10+x*(-15+x*6));
+                        integral += a.weights[k]*a.switchWidth*oneMinusSwitch*lrc_pair_energy(r, c1, c2, a.dampingScale)*r*r;
+                    }
+                }
+                double count = (i == j ? 0.5*c1[6]*(c1[6]+1) : c1[6]*c2[6]);
+                if (c1[5] != 0 || c2[5] != 0)
+                    alch += count*integral;
+                else
+                    sum += count*integral;
+            }
+            j++;
+            if (j >= C) {
+                i++;
+                j = i;
+            }
+        }
+    }
+    partial[2*tid] = sum;
+    partial[2*tid+1] = alch;
+}
+
+void gauss_legendre(int n, double* nodes, double* weights) {
+    // InteraxDispersionForceImpl.cpp:49-73
+    for (int i = 0; i < n; i++) {
+        double x = std::cos(M_PI*(i+0.75)/(n+0.5));
+        double derivative = 0;
+        for (int iteration = 0; iteration < 100; iteration++) {
+            double p0 = 1, p1 = x;
+            for (int k = 2; k <= n; k++) {
+                double p2 = ((2*k-1)*x*p1-(k-1)*p0)/k;
+                p0 = p1;
+                p1 = p2;
+            }
+            derivative = n*(x*p1-p0)/(x*x-1);
+            double step = p1/derivative;
+            x -= step;
+            if (std::fabs(step) < 1e-15)
+                break;
+        }
+        nodes[i] = 0.5*(1-x);
+        weights[i] = 1/((1-x*x)*derivative*derivative);
+    }
+}
+
+BoxD make_box(const double* box) {
+    BoxD b;
+    b.ax = box[0];
+    b.bx = box[3];
+    b.by = box[4];
+    b.cx = box[6];
+    b.cy = box[7];
+    b.cz = box[8];
+    return b;
+}
+
+int grid_for(long long n, int threads) {
+    return (int) std::max(1LL, (n+threads-1)/threads);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------------
+// The handle
+// ------------------------------------------------------------------------------------------------------------
+
+struct ixh_pair {
+    int device = 0;
+    int n = 0;
+    int terms = 0;
+    int method = IXH_NO_CUTOFF;
+    double cutoff = 1.0;
+    int useSwitch = 0;
+    double switchDistance = 0;
+    double softcoreRadius = 0;
+    int precision = IXH_MIXED;
+    bool anyAlchemical = false;
+    int exchangeMethod = 0, shortRangeMethod = 0;
+    std::vector<unsigned char> alchemical;
+    std::vector<int> exclusions;
+    std::vector<double> dispersion, exchange, shortRange;
+    double dampingScale = 1;
+    int useLRC = 0;
+    double lrcCoefficient = 0, lrcAlchemical = 0;
+    PairSettings settings;
+
+    int numBlocks = 0, padded = 0;
+    bool built = false;
+    int numTiles = 0;
+    std::vector<int> hostRowStart;
+    // parameters in particle order and in slot order
+    DeviceArray<char> paramA, paramB, paramC, sortedA, sortedB, sortedC, local;
+    DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionsDev, rowStart, rowCount, maskIndex, needMask, tileCount;
+    DeviceArray<unsigned> keys, keysOut;
+    DeviceArray<int> values;
+    DeviceArray<double> pos3, buildPos, bounds, centers, tileEnergy, bbox;
+    DeviceArray<int2> tiles;
+    DeviceArray<unsigned long long> masks;
+    DeviceArray<char> sortTemp;
+    DeviceArray<int> flagDev;
+    DeviceArray<long long> pairCount;
+    int* hostFlag = nullptr;
+    hipStream_t lastStream = nullptr;
+};
+
+namespace {
+
+void check_desc(const ixh_pair_desc* d) {
+    if (d == nullptr)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: null descriptor");
+    if (d->num_particles < 0)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: negative number of particles");
+    if (d->terms <= 0 || d->terms > 7)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: no pair terms selected");
+    if (d->precision != IXH_SINGLE && d->precision != IXH_MIXED && d->precision != IXH_DOUBLE)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: illegal precision");
+    // InteraxPairForce.cpp:56-60, 97-101, 113-136
+    if (d->method < 0 || d->method > 2)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxPairForce: Illegal value for nonbonded method");
+    if (d->softcore_radius < 0)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxPairForce: The soft-core radius cannot be negative");
+    for (int e = 0; e < d->num_exclusions; e++) {
+        int p1 = d->exclusions[2*e], p2 = d->exclusions[2*e+1];
+        if (p1 < 0 || p1 >= d->num_particles || p2 < 0 || p2 >= d->num_particles)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxPairForce: Illegal particle index for an exclusion");
+    }
+    if (d->method != IXH_NO_CUTOFF) {
+        if (d->cutoff <= 0)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxPairForce: The cutoff distance must be positive");
+        if (d->use_switching_function && (d->switching_distance < 0 || d->switching_distance >= d->cutoff))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxPairForce: Switching distance must satisfy 0 <= r_switch < r_cutoff");
+        if (d->method == IXH_CUTOFF_PERIODIC && d->default_box != nullptr) {
+            const double* b = d->default_box;
+            if (d->cutoff > 0.5*b[0] || d->cutoff > 0.5*b[4] || d->cutoff > 0.5*b[8])
+                throw Error(IXH_ERROR_INVALID_ARGUMENT,
+                            "InteraxPairForce: The cutoff distance cannot be greater than half the periodic box size.");
+        }
+    }
+    auto checkMethod = [](int m) {
+        if (m != IXH_OVERLAP_EXACT && m != IXH_OVERLAP_INTERAX_ORDER2)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxExchangeForce: Illegal value for overlap method");
+    };
+    if (d->terms & IXH_TERM_DISPERSION) {
+        if (d->dispersion == nullptr && d->num_particles > 0)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: missing dispersion parameters");
+    }
+    if (d->terms & IXH_TERM_EXCHANGE) {
+        if (d->exchange == nullptr && d->num_particles > 0)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: missing exchange parameters");
+        checkMethod(d->exchange_overlap_method);
+    }
+    if (d->terms & IXH_TERM_SHORT_RANGE) {
+        if (d->short_range == nullptr && d->num_particles > 0)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: missing short range parameters");
+        checkMethod(d->short_range_overlap_method);
+    }
+}
+
+OverlapScheme make_scheme(int method, bool doublePrecision) {
+    // InteraxPairForceImpl.h:155-160 with InteraxPairTerms.h:139-159
+    OverlapScheme s;
+    if (method == IXH_OVERLAP_EXACT) {
+        s.order = 6;
+        s.about_smaller = 1;
+        s.max_delta = 0.0;
+        s.max_t = (doublePrecision ? 0.02 : 0.2);
+        s.max_tx = (doublePrecision ? 0.3 : 1.6);
+    }
+    else {
+        s.order = 2;
+        s.about_smaller = 0;
+        s.max_delta = 0.01*0.0529177210903;
+        s.max_t = 0.0;
+        s.max_tx = HUGE_VAL;
+    }
+    s.max_delta_f = (float) s.max_delta;
+    s.max_t_f = (float) s.max_t;
+    s.max_tx_f = (float) s.max_tx;
+    return s;
+}
+
+// Copy the per-particle parameters of a descriptor into the handle and onto the device.
+void load_parameters(ixh_pair* p, const ixh_pair_desc* d, hipStream_t stream) {
+    int n = p->n;
+    if (d->terms & IXH_TERM_DISPERSION) {
+        p->dispersion.assign(d->dispersion, d->dispersion+5*(size_t) n);
+        p->dampingScale = d->damping_scale;
+    }
+    if (d->terms & IXH_TERM_EXCHANGE)
+        p->exchange.assign(d->exchange, d->exchange+2*(size_t) n);
+    if (d->terms & IXH_TERM_SHORT_RANGE)
+        p->shortRange.assign(d->short_range, d->short_range+3*(size_t) n);
+    bool doublePrecision = (p->precision == IXH_DOUBLE);
+    p->settings.exchange_scheme = make_scheme(p->exchangeMethod, doublePrecision);
+    p->settings.short_range_scheme = make_scheme(p->shortRangeMethod, doublePrecision);
+    p->settings.damping_scale = p->dampingScale;
+    p->settings.damping_scale_f = (float) p->dampingScale;
+    // One overlap per pair when both forces have the same widths and scheme.
+    bool shared = (p->terms & IXH_TERM_EXCHANGE) && (p->terms & IXH_TERM_SHORT_RANGE) && p->exchangeMethod == p->shortRangeMethod;
+    for (int i = 0; shared && i < n; i++)
+        shared = (p->exchange[2*i+1] == p->shortRange[3*i+2]);
+    p->settings.shared_overlap = shared ? 1 : 0;
+
+    std::vector<double> A(4*(size_t) n, 0.0), B(4*(size_t) n, 0.0), C(4*(size_t) n, 0.0);
+    std::vector<int> flags(n);
+    for (int i = 0; i < n; i++) {
+        if (p->terms & IXH_TERM_DISPERSION) {
+            for (int k = 0; k < 4; k++)
+                A[4*i+k] = p->dispersion[5*i+k];
+            B[4*i] = p->dispersion[5*i+4];
+        }
+        if (p->terms & IXH_TERM_EXCHANGE) {
+            B[4*i+1] = p->exchange[2*i];
+            B[4*i+2] = p->exchange[2*i+1];
+            B[4*i+3] = (p->exchange[2*i+1] != 0 ? 1.0/p->exchange[2*i+1] : 0.0);
+        }
+        if (p->terms & IXH_TERM_SHORT_RANGE) {
+            C[4*i] = p->shortRange[3*i];
+            C[4*i+1] = p->shortRange[3*i+1];
+            C[4*i+2] = p->shortRange[3*i+2];
+            C[4*i+3] = (p->shortRange[3*i+2] != 0 ? 1.0/p->shortRange[3*i+2] : 0.0);
+        }
+        flags[i] = i | (p->alchemical[i] ? (int) 0x80000000u : 0);
+    }
+    if (p->precision == IXH_DOUBLE) {
+        p->paramA.upload((const char*) A.data(), sizeof(double)*A.size(), stream);
+        p->paramB.upload((const char*) B.data(), sizeof(double)*B.size(), stream);
+        p->paramC.upload((const char*) C.data(), sizeof(double)*C.size(), stream);
+    }
+    else {
+        std::vector<float> Af(A.begin(), A.end()), Bf(B.begin(), B.end()), Cf(C.begin(), C.end());
+        p->paramA.upload((const char*) Af.data(), sizeof(float)*Af.size(), stream);
+        p->paramB.upload((const char*) Bf.data(), sizeof(float)*Bf.size(), stream);
+        p->paramC.upload((const char*) Cf.data(), sizeof(float)*Cf.size(), stream);
+    }
+    p->flags.upload(flags.data(), flags.size(), stream);
+    IXH_HIP_CHECK(hipStreamSynchronize(stream));
+    p->built = false;   // the slot-ordered copies are refreshed by the next build
+}
+
+// Long range correction coefficients on the device.
+void compute_long_range(ixh_pair* p, hipStream_t stream) {
+    p->lrcCoefficient = 0;
+    p->lrcAlchemical = 0;
+    if (!(p->terms & IXH_TERM_DISPERSION) || p->method != IXH_CUTOFF_PERIODIC || !p->useLRC)
+        return;
+    int n = p->n;
+    // classes of identical (m1, m2, m3, alpha, width, alchemical), in the order of a std::map over the tuple
+    std::map<std::tuple<double, double, double, double, double, bool>, int> counts;
+    for (int i = 0; i < n; i++) {
+        const double* q = &p->dispersion[5*(size_t) i];
+        counts[std::make_tuple(q[0], q[1], q[2], q[3], q[4], (bool) p->alchemical[i])]++;
+    }
+    std::vector<double> classes;
+    classes.reserve(7*counts.size());
+    for (auto& c : counts) {
+        classes.push_back(std::get<0>(c.first));
+        classes.push_back(std::get<1>(c.first));
+        classes.push_back(std::get<2>(c.first));
+        classes.push_back(std::get<3>(c.first));
+        classes.push_back(std::get<4>(c.first));
+        classes.push_back(std::get<5>(c.first) ? 1.0 : 0.0);
+        classes.push_back((double) c.second);
+    }
+    DeviceArray<double> classesDev, partial;
+    classesDev.upload(classes.data(), classes.size(), stream);
+    LrcArgs a;
+    a.classes = classesDev.ptr;
+    a.numClasses = (long long) counts.size();
+    a.cutoff = p->cutoff;
+    a.useSwitch = p->useSwitch;
+    a.switchDistance = (p->useSwitch ? p->switchDistance : p->cutoff);
+    a.switchWidth = p->cutoff-a.switchDistance;
+    a.dampingScale = p->dampingScale;
+    gauss_legendre(32, a.nodes, a.weights);
+    long long pairs = a.numClasses*(a.numClasses+1)/2;
+    int threads = 256;
+    int blocks = (int) std::min<long long>(2048, std::max<long long>(1, (pairs+threads*64-1)/(threads*64)));
+    partial.reserve(2*(size_t) blocks*threads);
+    hipLaunchKernelGGL(k_long_range, dim3(blocks), dim3(threads), 0, stream, a, partial.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    std::vector<double> host(2*(size_t) blocks*threads);
+    IXH_HIP_CHECK(hipMemcpyAsync(host.data(), partial.ptr, sizeof(double)*host.size(), hipMemcpyDeviceToHost, stream));
+    IXH_HIP_CHECK(hipStreamSynchronize(stream));
+    double sum = 0, alch = 0;
+    for (size_t k = 0; k < host.size(); k += 2) {
+        sum += host[k];
+        alch += host[k+1];
+    }
+    double N = n;
+    double normalization = 2*M_PI*N*N/(0.5*N*(N+1));
+    p->lrcCoefficient = normalization*sum;
+    p->lrcAlchemical = normalization*alch;
+}
+
+// Build the slot order, the tiles and the masks from the current positions.
+void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
+    int n = p->n;
+    bool periodic = (p->method == IXH_CUTOFF_PERIODIC);
+    BoxD box = make_box(b->box);
+    const float4* corr = (const float4*) b->posq_correction;
+    p->pos3.reserve(3*(size_t) n);
+    p->perm.reserve(p->padded);
+    p->slotOf.reserve(n);
+    if (p->method == IXH_NO_CUTOFF) {
+        // all pairs: the particle order is kept
+        std::vector<int> perm(p->padded);
+        for (int s = 0; s < p->padded; s++)
+            perm[s] = (s < n ? s : -1);
+        p->perm.upload(perm.data(), perm.size(), stream);
+        std::vector<int> slotOf(n);
+        std::iota(slotOf.begin(), slotOf.end(), 0);
+        p->slotOf.upload(slotOf.data(), slotOf.size(), stream);
+        std::vector<int2> tiles;
+        p->hostRowStart.assign(p->numBlocks+1, 0);
+        for (int I = 0; I < p->numBlocks; I++) {
+            p->hostRowStart[I] = (int) tiles.size();
+            for (int J = I; J < p->numBlocks; J++)
+                tiles.push_back(make_int2(I, J));
+        }
+        p->hostRowStart[p->numBlocks] = (int) tiles.size();
+        p->numTiles = (int) tiles.size();
+        p->tiles.upload(tiles.data(), tiles.size(), stream);
+    }
+    else {
+        KeyArgs k;
+        k.posq = b->posq;
+        k.correction = corr;
+        k.precision = b->precision;
+        k.n = n;
+        k.periodic = periodic;
+        k.box = box;
+        if (!periodic) {
+            p->bbox.reserve(6);
+            hipLaunchKernelGGL(k_bounding_box, dim3(1), dim3(256), 0, stream, b->posq, corr, b->precision, n, p->bbox.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+            double bb[6];
+            IXH_HIP_CHECK(hipMemcpyAsync(bb, p->bbox.ptr, sizeof(bb), hipMemcpyDeviceToHost, stream));
+            IXH_HIP_CHECK(hipStreamSynchronize(stream));
+            for (int c = 0; c < 3; c++) {
+                k.lo[c] = bb[c];
+                double extent = bb[3+c]-bb[c];
+                k.scale[c] = (extent > 0 ? 1.0/(extent*(1+1e-9)) : 0.0);
+            }
+        }
+        p->keys.reserve(n);
+        p->keysOut.reserve(n);
+        p->values.reserve(n);
+        hipLaunchKernelGGL(k_morton_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, k, p->keys.ptr, p->values.ptr, p->pos3.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        size_t tempBytes = 0;
+        IXH_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tempBytes, p->keys.ptr, p->keysOut.ptr, p->values.ptr, p->perm.ptr,
+                                                (size_t) n, 0, 30, stream));
+        p->sortTemp.reserve(tempBytes);
+        IXH_HIP_CHECK(rocprim::radix_sort_pairs(p->sortTemp.ptr, tempBytes, p->keys.ptr, p->keysOut.ptr, p->values.ptr, p->perm.ptr,
+                                                (size_t) n, 0, 30, stream));
+    }
+    hipLaunchKernelGGL(k_finish_perm, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, n, p->padded, p->slotOf.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
+    p->sortedA.reserve(r4*p->padded);
+    p->sortedB.reserve(r4*p->padded);
+    p->sortedC.reserve(r4*p->padded);
+    p->sortedFlags.reserve(p->padded);
+    if (p->precision == IXH_DOUBLE)
+        hipLaunchKernelGGL(k_gather_params<double>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
+                           (const double4*) p->paramA.ptr, (const double4*) p->paramB.ptr, (const double4*) p->paramC.ptr, p->flags.ptr,
+                           (double4*) p->sortedA.ptr, (double4*) p->sortedB.ptr, (double4*) p->sortedC.ptr, p->sortedFlags.ptr);
+    else
+        hipLaunchKernelGGL(k_gather_params<float>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
+                           (const float4*) p->paramA.ptr, (const float4*) p->paramB.ptr, (const float4*) p->paramC.ptr, p->flags.ptr,
+                           (float4*) p->sortedA.ptr, (float4*) p->sortedB.ptr, (float4*) p->sortedC.ptr, p->sortedFlags.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+
+    if (p->method != IXH_NO_CUTOFF) {
+        // tiles from block bounding boxes
+        p->bounds.reserve(6*(size_t) p->numBlocks);
+        BlockBoundsArgs bb;
+        bb.pos3 = p->pos3.ptr;
+        bb.perm = p->perm.ptr;
+        bb.numBlocks = p->numBlocks;
+        bb.periodic = periodic;
+        bb.box = box;
+        hipLaunchKernelGGL(k_block_bounds, dim3(grid_for(p->numBlocks, 64)), dim3(64), 0, stream, bb, p->bounds.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        double listCutoff = p->cutoff+SKIN;
+        p->rowCount.reserve(p->numBlocks);
+        hipLaunchKernelGGL(k_find_tiles, dim3(p->numBlocks), dim3(64), 0, stream, p->bounds.ptr, p->numBlocks, (int) periodic, box,
+                           listCutoff*listCutoff, p->rowCount.ptr, (const int*) nullptr, (int2*) nullptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        std::vector<int> counts(p->numBlocks);
+        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), p->rowCount.ptr, sizeof(int)*counts.size(), hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        p->hostRowStart.assign(p->numBlocks+1, 0);
+        for (int I = 0; I < p->numBlocks; I++)
+            p->hostRowStart[I+1] = p->hostRowStart[I]+counts[I];
+        p->numTiles = p->hostRowStart[p->numBlocks];
+        p->rowStart.upload(p->hostRowStart.data(), p->hostRowStart.size(), stream);
+        p->tiles.reserve(p->numTiles);
+        hipLaunchKernelGGL(k_find_tiles, dim3(p->numBlocks), dim3(64), 0, stream, p->bounds.ptr, p->numBlocks, (int) periodic, box,
+                           listCutoff*listCutoff, p->rowCount.ptr, (const int*) p->rowStart.ptr, p->tiles.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
+    else
+        p->rowStart.upload(p->hostRowStart.data(), p->hostRowStart.size(), stream);
+
+    // masks
+    int T = p->numTiles;
+    int paddingBlock = (n%BLOCK != 0 ? p->numBlocks-1 : -1);
+    int validInLast = n-(p->numBlocks-1)*BLOCK;
+    int numExclusions = (int) (p->exclusions.size()/2);
+    p->needMask.reserve(T);
+    p->maskIndex.reserve(T);
+    IXH_HIP_CHECK(hipMemsetAsync(p->needMask.ptr, 0, sizeof(int)*(size_t) std::max(T, 1), stream));
+    int markCount = std::max(numExclusions, T);
+    hipLaunchKernelGGL(k_mark_mask_tiles, dim3(grid_for(markCount, 256)), dim3(256), 0, stream, p->exclusionsDev.ptr, numExclusions,
+                       p->slotOf.ptr, p->tiles.ptr, p->rowStart.ptr, T, paddingBlock, p->needMask.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    std::vector<int> need(T);
+    IXH_HIP_CHECK(hipMemcpyAsync(need.data(), p->needMask.ptr, sizeof(int)*(size_t) T, hipMemcpyDeviceToHost, stream));
+    IXH_HIP_CHECK(hipStreamSynchronize(stream));
+    std::vector<int> index(T);
+    int numMasks = 0;
+    for (int t = 0; t < T; t++)
+        index[t] = (need[t] ? numMasks++ : -1);
+    p->maskIndex.upload(index.data(), index.size(), stream);
+    p->masks.reserve((size_t) std::max(numMasks, 1)*BLOCK);
+    if (numMasks > 0) {
+        hipLaunchKernelGGL(k_init_masks, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->maskIndex.ptr, T, paddingBlock, validInLast,
+                           p->masks.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        if (numExclusions > 0)
+            hipLaunchKernelGGL(k_set_exclusion_bits, dim3(grid_for(numExclusions, 256)), dim3(256), 0, stream, p->exclusionsDev.ptr,
+                               numExclusions, p->slotOf.ptr, p->tiles.ptr, p->rowStart.ptr, p->maskIndex.ptr, p->masks.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
+    p->tileEnergy.reserve(std::max(T, 1));
+    p->tileCount.reserve(std::max(T, 1));
+    if (p->method != IXH_NO_CUTOFF) {
+        p->buildPos.reserve(3*(size_t) n);
+        IXH_HIP_CHECK(hipMemcpyAsync(p->buildPos.ptr, p->pos3.ptr, sizeof(double)*3*(size_t) n, hipMemcpyDeviceToDevice, stream));
+    }
+    IXH_HIP_CHECK(hipStreamSynchronize(stream));
+    p->built = true;
+}
+
+template <typename Real>
+void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, hipStream_t stream) {
+    typedef typename Vec4T<Real>::type R4;
+    TileArgs<Real> a;
+    a.tiles = p->tiles.ptr;
+    a.numTiles = p->numTiles;
+    a.maskIndex = p->maskIndex.ptr;
+    a.masks = p->masks.ptr;
+    a.local = (const R4*) p->local.ptr;
+    a.centers = p->centers.ptr;
+    a.pA = (const R4*) p->sortedA.ptr;
+    a.pB = (const R4*) p->sortedB.ptr;
+    a.pC = (const R4*) p->sortedC.ptr;
+    a.flags = p->sortedFlags.ptr;
+    a.n = p->n;
+    a.method = p->method;
+    a.periodic = (p->method == IXH_CUTOFF_PERIODIC);
+    a.box = make_box(b->box);
+    a.boxR[0] = Real(b->box[0]);
+    a.boxR[1] = Real(b->box[3]);
+    a.boxR[2] = Real(b->box[4]);
+    a.boxR[3] = Real(b->box[6]);
+    a.boxR[4] = Real(b->box[7]);
+    a.boxR[5] = Real(b->box[8]);
+    a.boxR[6] = Real(a.periodic ? 1.0/b->box[0] : 0.0);
+    a.boxR[7] = Real(a.periodic ? 1.0/b->box[4] : 0.0);
+    a.boxR[8] = Real(a.periodic ? 1.0/b->box[8] : 0.0);
+    a.cutoff = p->cutoff;
+    a.cutoff2 = Real(p->cutoff*p->cutoff);
+    a.band = Real(sizeof(Real) == 4 ? 1e-5 : 1e-12);
+    a.useSwitch = (p->method != IXH_NO_CUTOFF && p->useSwitch);
+    a.switchDistance = Real(p->switchDistance);
+    a.invSwitchWidth = Real(a.useSwitch ? 1.0/(p->cutoff-p->switchDistance) : 0.0);
+    double lambda = (p->anyAlchemical ? b->lambda : 1.0);
+    a.lambda = Real(lambda);
+    a.useSoftcore = (p->softcoreRadius > 0 && p->anyAlchemical);
+    double R3 = p->softcoreRadius*p->softcoreRadius*p->softcoreRadius;
+    a.softcoreTerm = Real((1-lambda)*R3*R3);
+    a.settings = p->settings;
+    a.posq = b->posq;
+    a.correction = (const float4*) b->posq_correction;
+    a.precision = b->precision;
+    a.force = b->force;
+    a.stride = b->force_stride;
+    a.tileEnergy = p->tileEnergy.ptr;
+    a.tileCount = p->tileCount.ptr;
+    int blocks = std::max(1, std::min((p->numTiles+3)/4, 4096));
+    if (p->numTiles > 0) {
+        switch (terms) {
+#define IXH_CASE(T) case T: hipLaunchKernelGGL((k_pair_tiles<Real, T>), dim3(blocks), dim3(256), 0, stream, a); break;
+            IXH_CASE(1) IXH_CASE(2) IXH_CASE(3) IXH_CASE(4) IXH_CASE(5) IXH_CASE(6) IXH_CASE(7)
+#undef IXH_CASE
+            default:
+                throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: illegal terms");
+        }
+        IXH_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_reduce_tiles, dim3(1), dim3(256), 0, stream, p->tileEnergy.ptr, p->tileCount.ptr, p->numTiles, b->energy,
+                       p->pairCount.ptr, addEnergy);
+    IXH_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixh_pair_create(int device, const ixh_pair_desc* d, ixh_pair** out) {
+    return guarded([&] {
+        if (out == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_create: null output pointer");
+        check_desc(d);
+        DeviceGuard guard(device);
+        ixh_pair* p = new ixh_pair();
+        try {
+            p->device = device;
+            p->n = d->num_particles;
+            p->terms = d->terms;
+            p->method = d->method;
+            p->cutoff = d->cutoff;
+            p->useSwitch = (d->method != IXH_NO_CUTOFF && d->use_switching_function);
+            p->switchDistance = d->switching_distance;
+            p->softcoreRadius = d->softcore_radius;
+            p->precision = d->precision;
+            p->exchangeMethod = d->exchange_overlap_method;
+            p->shortRangeMethod = d->short_range_overlap_method;
+            p->useLRC = d->use_long_range_correction;
+            p->alchemical.assign(p->n, 0);
+            if (d->alchemical != nullptr)
+                for (int i = 0; i < p->n; i++)
+                    p->alchemical[i] = (d->alchemical[i] != 0);
+            p->anyAlchemical = std::any_of(p->alchemical.begin(), p->alchemical.end(), [](unsigned char c) { return c != 0; });
+            p->exclusions.assign(d->exclusions, d->exclusions+2*(size_t) d->num_exclusions);
+            p->numBlocks = std::max(1, (p->n+BLOCK-1)/BLOCK);
+            p->padded = p->numBlocks*BLOCK;
+            hipStream_t stream = nullptr;
+            p->exclusionsDev.upload(p->exclusions.data(), p->exclusions.size(), stream);
+            p->flagDev.reserve(1);
+            p->pairCount.reserve(1);
+            IXH_HIP_CHECK(hipHostMalloc((void**) &p->hostFlag, sizeof(int), hipHostMallocDefault));
+            load_parameters(p, d, stream);
+            compute_long_range(p, stream);
+        }
+        catch (...) {
+            if (p->hostFlag != nullptr)
+                (void) hipHostFree(p->hostFlag);
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int ixh_pair_destroy(ixh_pair* p) {
+    return guarded([&] {
+        if (p == nullptr)
+            return;
+        DeviceGuard guard(p->device);
+        (void) hipDeviceSynchronize();
+        if (p->hostFlag != nullptr)
+            (void) hipHostFree(p->hostFlag);
+        delete p;
+    });
+}
+
+int ixh_pair_set_parameters(ixh_pair* p, const ixh_pair_desc* d) {
+    return guarded([&] {
+        if (p == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_set_parameters: null handle");
+        check_desc(d);
+        // ReferenceInteraxKernels.cpp:98-102, 142-148, 185-191
+        if (d->num_particles != p->n)
+            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The number of particles has changed");
+        if (d->terms != p->terms)
+            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The set of pair forces has changed");
+        if (((p->terms & IXH_TERM_EXCHANGE) && d->exchange_overlap_method != p->exchangeMethod) ||
+            ((p->terms & IXH_TERM_SHORT_RANGE) && d->short_range_overlap_method != p->shortRangeMethod))
+            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The overlap method has changed");
+        DeviceGuard guard(p->device);
+        if (p->lastStream != nullptr)
+            IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
+        hipStream_t stream = nullptr;
+        load_parameters(p, d, stream);
+        compute_long_range(p, stream);
+    });
+}
+
+int ixh_pair_execute(ixh_pair* p, const ixh_buffers* b, int terms, int flags) {
+    return guarded([&] {
+        if (p == nullptr || b == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: null argument");
+        if (b->num_particles != p->n)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: the buffers hold a different number of particles");
+        if (b->precision != p->precision && !(b->precision != IXH_DOUBLE && p->precision != IXH_DOUBLE))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: the buffers have a different precision");
+        terms &= p->terms;
+        if (terms == 0 || p->n == 0)
+            return;
+        if (p->method == IXH_CUTOFF_PERIODIC) {
+            const double* bx = b->box;
+            if (p->cutoff > 0.5*bx[0] || p->cutoff > 0.5*bx[4] || p->cutoff > 0.5*bx[8])
+                throw Error(IXH_ERROR_INVALID_ARGUMENT,
+                            "InteraxPairForce: The cutoff distance cannot be greater than half the periodic box size.");
+        }
+        DeviceGuard guard(p->device);
+        hipStream_t stream = (hipStream_t) b->stream;
+        p->lastStream = stream;
+        const float4* corr = (const float4*) b->posq_correction;
+        bool rebuild = !p->built;
+        if (!rebuild && p->method != IXH_NO_CUTOFF) {
+            IXH_HIP_CHECK(hipMemsetAsync(p->flagDev.ptr, 0, sizeof(int), stream));
+            hipLaunchKernelGGL(k_check_displacement, dim3(grid_for(p->n, 256)), dim3(256), 0, stream, b->posq, corr, b->precision, p->n,
+                               p->buildPos.ptr, (int) (p->method == IXH_CUTOFF_PERIODIC), make_box(b->box), 0.25*SKIN*SKIN, p->flagDev.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+            IXH_HIP_CHECK(hipMemcpyAsync(p->hostFlag, p->flagDev.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+            IXH_HIP_CHECK(hipStreamSynchronize(stream));
+            rebuild = (*p->hostFlag != 0);
+        }
+        if (rebuild)
+            build_list(p, b, stream);
+        size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
+        p->local.reserve(r4*p->padded);
+        p->centers.reserve(3*(size_t) p->numBlocks);
+        int periodic = (p->method == IXH_CUTOFF_PERIODIC);
+        if (p->precision == IXH_DOUBLE) {
+            hipLaunchKernelGGL(k_local_positions<double>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
+                               p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (double4*) p->local.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+            launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, stream);
+        }
+        else {
+            hipLaunchKernelGGL(k_local_positions<float>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
+                               p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (float4*) p->local.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+            launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, stream);
+        }
+    });
+}
+
+int ixh_pair_long_range_coefficients(ixh_pair* p, double* coefficient, double* alchemical) {
+    return guarded([&] {
+        if (p == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_long_range_coefficients: null handle");
+        if (coefficient != nullptr)
+            *coefficient = p->lrcCoefficient;
+        if (alchemical != nullptr)
+            *alchemical = p->lrcAlchemical;
+    });
+}
+
+int ixh_pair_long_range_energy(ixh_pair* p, const double* box, double lambda, double* energy) {
+    return guarded([&] {
+        if (p == nullptr || energy == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_long_range_energy: null argument");
+        double e = 0;
+        if (p->lrcCoefficient != 0 || p->lrcAlchemical != 0) {
+            if (box == nullptr)
+                throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_long_range_energy: null box");
+            double lam = (p->anyAlchemical ? lambda : 1.0);
+            e = (p->lrcCoefficient+lam*p->lrcAlchemical)/(box[0]*box[4]*box[8]);
+        }
+        *energy = e;
+    });
+}
+
+int ixh_pair_last_pair_count(ixh_pair* p, long long* count) {
+    return guarded([&] {
+        if (p == nullptr || count == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_last_pair_count: null argument");
+        DeviceGuard guard(p->device);
+        IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
+        IXH_HIP_CHECK(hipMemcpy(count, p->pairCount.ptr, sizeof(long long), hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,657 @@
+// The InteraxFF Axilrod-Teller-Muto three-body force on gfx950.
+//
+// Reference behaviour (the contract): ReferenceInteraxThreeBodyIxn.cpp:39-135 (sites, neighbours, triples, force
+// sharing), InteraxThreeBodyTerms.h:51-146 (damping and the triple), InteraxThreeBodyForceImpl.h:69-72 (C9) and
+// InteraxThreeBodyForceImpl.cpp:42-62 (validation).
+//
+// Pipeline per evaluation:
+//   1. site centroids in FP64, summed in the reference's order;
+//   2. half neighbour lists (sites above, other molecule, |d|^2 < rc^2 decided in FP64 with the reference's
+//      minimum image), one wavefront per site with ballot compaction, so each list is in ascending site order;
+//   3. one wavefront per site A walks the pairs (B, C) of its list, tests B-C in FP32 (FP64 near the cutoff or near
+//      a change of periodic image) and evaluates the triple in FP32; the forces on B and C go to LDS as 64-bit
+//      fixed point (integer atomics, so their order does not matter), the force on A stays in registers;
+//   4. the site forces are shared equally among the particles of each site.
+#include "ixh_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace ixh;
+
+namespace {
+
+struct BoxT {
+    double ax, bx, by, cx, cy, cz;
+};
+
+// ReferenceInteraxThreeBodyIxn.cpp:74-82, each operation rounded separately as on the host.
+__device__ __forceinline__ void ref_delta(const double* from, const double* to, int periodic, const BoxT& b, double d[3]) {
+    d[0] = __dsub_rn(to[0], from[0]);
+    d[1] = __dsub_rn(to[1], from[1]);
+    d[2] = __dsub_rn(to[2], from[2]);
+    if (periodic) {
+        double s2 = floor(__dadd_rn(__ddiv_rn(d[2], b.cz), 0.5));
+        d[0] = __dsub_rn(d[0], __dmul_rn(b.cx, s2));
+        d[1] = __dsub_rn(d[1], __dmul_rn(b.cy, s2));
+        d[2] = __dsub_rn(d[2], __dmul_rn(b.cz, s2));
+        double s1 = floor(__dadd_rn(__ddiv_rn(d[1], b.by), 0.5));
+        d[0] = __dsub_rn(d[0], __dmul_rn(b.bx, s1));
+        d[1] = __dsub_rn(d[1], __dmul_rn(b.by, s1));
+        double s0 = floor(__dadd_rn(__ddiv_rn(d[0], b.ax), 0.5));
+        d[0] = __dsub_rn(d[0], __dmul_rn(b.ax, s0));
+    }
+}
+
+__device__ __forceinline__ double ref_norm2(const double d[3]) {
+    return __dadd_rn(__dadd_rn(__dmul_rn(d[0], d[0]), __dmul_rn(d[1], d[1])), __dmul_rn(d[2], d[2]));
+}
+
+__device__ __forceinline__ void load_pos(const void* posq, const float4* correction, int precision, int i, double p[3]) {
+    if (precision == IXH_DOUBLE) {
+        double4 v = ((const double4*) posq)[i];
+        p[0] = v.x;
+        p[1] = v.y;
+        p[2] = v.z;
+    }
+    else {
+        float4 v = ((const float4*) posq)[i];
+        p[0] = v.x;
+        p[1] = v.y;
+        p[2] = v.z;
+        if (correction != nullptr) {
+            float4 c = correction[i];
+            p[0] += c.x;
+            p[1] += c.y;
+            p[2] += c.z;
+        }
+    }
+}
+
+// 1. ReferenceInteraxThreeBodyIxn.cpp:87-93 (OpenMM's Vec3 divides by multiplying with the reciprocal)
+__global__ void k_sites(const void* posq, const float4* correction, int precision, const int* offsets, const int* particles,
+                        int numSites, double* sites) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= numSites)
+        return;
+    double s[3] = {0, 0, 0};
+    for (int k = offsets[i]; k < offsets[i+1]; k++) {
+        double p[3];
+        load_pos(posq, correction, precision, particles[k], p);
+        s[0] = __dadd_rn(s[0], p[0]);
+        s[1] = __dadd_rn(s[1], p[1]);
+        s[2] = __dadd_rn(s[2], p[2]);
+    }
+    double scale = 1.0/(double) (offsets[i+1]-offsets[i]);
+    sites[3*i] = __dmul_rn(s[0], scale);
+    sites[3*i+1] = __dmul_rn(s[1], scale);
+    sites[3*i+2] = __dmul_rn(s[2], scale);
+}
+
+// 2. Half neighbour lists: pass 0 counts, pass 1 writes (ascending order).
+__global__ void k_neighbors(const double* sites, const int* molecule, int numSites, int periodic, BoxT box, double cutoff2,
+                            int* count, const int* start, int* nbrIndex, double4* nbrDelta) {
+    int i = blockIdx.x;
+    int lane = threadIdx.x;
+    const double* si = sites+3*i;
+    int mi = molecule[i];
+    int written = 0;
+    for (int j0 = i+1; j0 < numSites; j0 += 64) {
+        int j = j0+lane;
+        bool hit = false;
+        double d[3] = {0, 0, 0};
+        if (j < numSites && molecule[j] != mi) {
+            ref_delta(si, sites+3*j, periodic, box, d);
+            hit = ref_norm2(d) < cutoff2;
+        }
+        unsigned long long ballot = __ballot(hit);
+        if (start != nullptr && hit) {
+            int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
+            nbrIndex[pos] = j;
+            nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
+        }
+        written += __popcll(ballot);
+    }
+    if (start == nullptr && lane == 0)
+        count[i] = written;
+}
+
+template <typename Real>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+    typedef float4 type;
+};
+template <>
+struct Vec4Of<double> {
+    typedef double4 type;
+};
+
+template <typename Real>
+struct TripleArgs {
+    const double* sites;
+    const int* molecule;
+    const double* siteParams;    // alpha, frequency, width, alchemical per site
+    const int* start;
+    const int* nbrIndex;
+    const double4* nbrDelta;
+    int numSites;
+    int periodic;
+    BoxT box;
+    Real boxR[9];                // ax, bx, by, cx, cy, cz, 1/ax, 1/by, 1/cz
+    Real cutoff2r;
+    double cutoff2;
+    Real band;                   // relative band around the cutoff and the image boundaries decided in FP64
+    Real scaleFactor;
+    Real lambda;
+    int useDamping;
+    Real dampingScale;
+    Real invDampingRadius;       // 0: no centroid damping
+    int capacity;                // LDS entries
+    long long* siteForce;        // 3 x numSites fixed point
+    double* siteEnergy;
+    int* siteCount;
+};
+
+// InteraxThreeBodyTerms.h:51-65
+template <typename Real>
+__device__ __forceinline__ void tt3(Real x, Real& f, Real& df) {
+    Real expX = fast_exp2(-x*Real(1.4426950408889634));
+    Real term3 = x*x*x*Real(1.0/6.0);
+    df = expX*term3;
+    if (x < Real(2)) {
+        Real term = term3, tail = 0;
+        for (int k = 4; k <= 30; k++) {
+            term *= x*Real(1.0/k);
+            tail += term;
+        }
+        f = expX*tail;
+    }
+    else
+        f = Real(1)-expX*(Real(1)+x+x*x*Real(0.5)+term3);
+}
+
+// InteraxThreeBodyForceImpl.h:69-72
+template <typename Real>
+__device__ __forceinline__ Real c9_of(Real a1, Real w1, Real a2, Real w2, Real a3, Real w3) {
+    return Real(1.5)*a1*a2*a3*w1*w2*w3*(w1+w2+w3)/((w1+w2)*(w2+w3)*(w3+w1));
+}
+
+// 3. One wavefront per site A.
+template <typename Real>
+__global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
+    typedef typename Vec4Of<Real>::type R4;
+    extern __shared__ double sharedStorage[];
+    R4* sDelta = (R4*) sharedStorage;                               // AB vectors of the neighbours
+    R4* sParam = sDelta+t.capacity;                                 // alpha, frequency, width, alchemical
+    unsigned long long* sForce = (unsigned long long*) (sParam+t.capacity);   // 3 per neighbour
+    int* sIndex = (int*) (sForce+3*(size_t) t.capacity);
+    int* sMol = sIndex+t.capacity;
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int begin = t.start[i], m = t.start[i+1]-begin;
+    for (int a = lane; a < m; a += 64) {
+        int j = t.nbrIndex[begin+a];
+        double4 d = t.nbrDelta[begin+a];
+        R4 dr = {Real(d.x), Real(d.y), Real(d.z), Real(0)};
+        sDelta[a] = dr;
+        R4 pr = {Real(t.siteParams[4*j]), Real(t.siteParams[4*j+1]), Real(t.siteParams[4*j+2]), Real(t.siteParams[4*j+3])};
+        sParam[a] = pr;
+        sIndex[a] = j;
+        sMol[a] = t.molecule[j];
+        sForce[3*a] = sForce[3*a+1] = sForce[3*a+2] = 0;
+    }
+    __syncthreads();
+    const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
+    const bool alchI = t.siteParams[4*i+3] != 0;
+    Real fA[3] = {0, 0, 0};
+    double energy = 0;
+    int count = 0;
+    const long long numPairs = (long long) m*(m-1)/2;
+    for (long long p0 = 0; p0 < numPairs; p0 += 64) {
+        long long p = p0+lane;
+        if (p < numPairs) {
+            // p = b(b-1)/2 + a, 0 <= a < b < m
+            int b = (int) ((1.0+sqrt(1.0+8.0*(double) p))*0.5);
+            while ((long long) b*(b-1)/2 > p)
+                b--;
+            while ((long long) (b+1)*b/2 <= p)
+                b++;
+            int a = (int) (p-(long long) b*(b-1)/2);
+            bool ok = (sMol[a] != sMol[b]);
+            R4 va = sDelta[a], vb = sDelta[b];
+            Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
+            if (ok) {
+                bool exact = false;
+                if (t.periodic) {
+                    // the reference's minimum image of BC; near a change of image the choice is made in FP64
+                    Real u2 = c[2]*t.boxR[8]+Real(0.5);
+                    Real s2 = floor(u2);
+                    c[0] -= s2*t.boxR[3];
+                    c[1] -= s2*t.boxR[4];
+                    c[2] -= s2*t.boxR[5];
+                    Real u1 = c[1]*t.boxR[7]+Real(0.5);
+                    Real s1 = floor(u1);
+                    c[0] -= s1*t.boxR[1];
+                    c[1] -= s1*t.boxR[2];
+                    Real u0 = c[0]*t.boxR[6]+Real(0.5);
+                    Real s0 = floor(u0);
+                    c[0] -= s0*t.boxR[0];
+                    Real f2 = u2-s2, f1 = u1-s1, f0 = u0-s0;
+                    exact = (f2 < t.band || f2 > 1-t.band || f1 < t.band || f1 > 1-t.band || f0 < t.band || f0 > 1-t.band);
+                }
+                Real r2 = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+                if (exact || fabs(r2-t.cutoff2r) < t.band*t.cutoff2r) {
+                    double d[3];
+                    ref_delta(t.sites+3*sIndex[a], t.sites+3*sIndex[b], t.periodic, t.box, d);
+                    ok = ref_norm2(d) < t.cutoff2;
+                    c[0] = Real(d[0]);
+                    c[1] = Real(d[1]);
+                    c[2] = Real(d[2]);
+                }
+                else
+                    ok = r2 < t.cutoff2r;
+            }
+            R4 pa = sParam[a], pb = sParam[b];
+            Real c9 = 0;
+            if (ok) {
+                // ReferenceInteraxThreeBodyIxn.cpp:116-120
+                c9 = t.scaleFactor*c9_of<Real>(aI, wI, pa.x, pa.y, pb.x, pb.y);
+                if (alchI || pa.w != 0 || pb.w != 0)
+                    c9 *= t.lambda;
+                ok = (c9 != 0);
+            }
+            if (ok) {
+                // InteraxThreeBodyTerms.h:97-146 with a = AB, b = AC, c = BC
+                const Real A[3] = {va.x, va.y, va.z}, B[3] = {vb.x, vb.y, vb.z};
+                Real u = A[0]*A[0]+A[1]*A[1]+A[2]*A[2];
+                Real v = B[0]*B[0]+B[1]*B[1]+B[2]*B[2];
+                Real w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+                Real pp = A[0]*B[0]+A[1]*B[1]+A[2]*B[2];
+                Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
+                Real s = B[0]*c[0]+B[1]*c[1]+B[2]*c[2];
+                Real invP = Real(1)/(u*v*w);
+                Real invP32 = invP*sqrt(invP);
+                Real invP52 = invP32*invP;
+                Real pqs = pp*q*s;
+                Real e = c9*(invP32-3*pqs*invP52);
+                Real radial = c9*(-3*invP32+15*pqs*invP52);
+                Real cross = -3*c9*invP52;
+                Real ru = radial/u, rv = radial/v, rw = radial/w;
+                Real dA[3], dB[3], dC[3];
+                for (int k = 0; k < 3; k++) {
+                    dA[k] = A[k]*ru + (B[k]*(q*s) + c[k]*(pp*s))*cross;
+                    dB[k] = B[k]*rv + (A[k]*(q*s) + c[k]*(pp*q))*cross;
+                    dC[k] = c[k]*rw + (A[k]*(pp*s) + B[k]*(pp*q))*cross;
+                }
+                if (t.useDamping || t.invDampingRadius > 0) {
+                    Real len[3] = {sqrt(u), sqrt(v), sqrt(w)};
+                    Real rates[3] = {2*t.dampingScale/(sI+pa.z), 2*t.dampingScale/(sI+pb.z), 2*t.dampingScale/(pa.z+pb.z)};
+                    Real f[3], dfdr[3];
+                    for (int k = 0; k < 3; k++) {
+                        if (t.useDamping) {
+                            tt3<Real>(rates[k]*len[k], f[k], dfdr[k]);
+                            dfdr[k] *= rates[k];
+                        }
+                        else {
+                            f[k] = 1;
+                            dfdr[k] = 0;
+                        }
+                        if (t.invDampingRadius > 0) {
+                            // InteraxThreeBodyTerms.h:71-77
+                            Real x = len[k]*t.invDampingRadius;
+                            Real x6 = x*x*x;
+                            x6 *= x6;
+                            Real g = -expm1(-x6);
+                            Real dg = 6*x6*exp(-x6)/len[k];
+                            dfdr[k] = dfdr[k]*g + f[k]*dg;
+                            f[k] *= g;
+                        }
+                    }
+                    Real damping = f[0]*f[1]*f[2];
+                    Real sa = e*dfdr[0]*f[1]*f[2]/len[0];
+                    Real sb = e*f[0]*dfdr[1]*f[2]/len[1];
+                    Real sc = e*f[0]*f[1]*dfdr[2]/len[2];
+                    for (int k = 0; k < 3; k++) {
+                        dA[k] = dA[k]*damping + A[k]*sa;
+                        dB[k] = dB[k]*damping + B[k]*sb;
+                        dC[k] = dC[k]*damping + c[k]*sc;
+                    }
+                    e *= damping;
+                }
+                energy += (double) e;
+                count++;
+                for (int k = 0; k < 3; k++) {
+                    fA[k] += dA[k]+dB[k];
+                    atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(dC[k]-dA[k]));
+                    atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(-(dB[k]+dC[k])));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // flush the neighbour forces and the force on A
+    for (int a = lane; a < m; a += 64) {
+        int j = sIndex[a];
+        for (int k = 0; k < 3; k++)
+            add_fixed(&t.siteForce[(size_t) k*t.numSites+j], (long long) sForce[3*a+k]);
+    }
+    for (int k = 0; k < 3; k++) {
+        long long v = to_fixed(fA[k]);
+        for (int offset = 32; offset > 0; offset >>= 1)
+            v += __shfl_xor(v, offset, 64);
+        if (lane == 0)
+            add_fixed(&t.siteForce[(size_t) k*t.numSites+i], v);
+    }
+    double e = wave_sum(energy);
+    int c = (int) wave_sum((double) count);
+    if (lane == 0) {
+        t.siteEnergy[i] = e;
+        t.siteCount[i] = c;
+    }
+}
+
+__global__ void k_zero_long(long long* p, long long n) {
+    long long i = (long long) blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+// 4. ReferenceInteraxThreeBodyIxn.cpp:129-133
+__global__ void k_apply_site_forces(const long long* siteForce, int numSites, const int* offsets, const int* particles,
+                                    long long* force, int stride) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= numSites)
+        return;
+    double scale = 1.0/(double) (offsets[i+1]-offsets[i]);
+    long long f[3];
+    for (int k = 0; k < 3; k++) {
+        double v = (double) siteForce[(size_t) k*numSites+i]/FIXED_POINT_SCALE;
+        f[k] = (long long) (v*scale*FIXED_POINT_SCALE);
+    }
+    for (int p = offsets[i]; p < offsets[i+1]; p++) {
+        int particle = particles[p];
+        for (int k = 0; k < 3; k++)
+            add_fixed(&force[(size_t) k*stride+particle], f[k]);
+    }
+}
+
+__global__ void k_reduce_sites(const double* siteEnergy, const int* siteCount, int numSites, double* energy, long long* tripleCount,
+                               int addEnergy) {
+    __shared__ double sE[256];
+    __shared__ long long sC[256];
+    int per = (numSites+255)/256;
+    int begin = threadIdx.x*per, end = min(numSites, begin+per);
+    double e = 0;
+    long long c = 0;
+    for (int s = begin; s < end; s++) {
+        e += siteEnergy[s];
+        c += siteCount[s];
+    }
+    sE[threadIdx.x] = e;
+    sC[threadIdx.x] = c;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) {
+            sE[threadIdx.x] += sE[threadIdx.x+stride];
+            sC[threadIdx.x] += sC[threadIdx.x+stride];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (addEnergy)
+            energy[0] += sE[0];
+        *tripleCount = sC[0];
+    }
+}
+
+int grid_of(long long n, int threads) {
+    return (int) std::max(1LL, (n+threads-1)/threads);
+}
+
+}  // namespace
+
+struct ixh_threebody {
+    int device = 0;
+    int numParticles = 0;
+    int numSites = 0;
+    int precision = IXH_MIXED;
+    double cutoff = 1.2;
+    double scaleFactor = 1, dampingScale = 1, centroidDampingRadius = 0;
+    int dampingMethod = 0;
+    int periodic = 0;
+    bool anyAlchemical = false;
+    std::vector<int> offsets, particles, molecule;
+    std::vector<unsigned char> alchemical;
+    std::vector<double> params;   // alpha, frequency, width, alchemical
+    DeviceArray<int> offsetsDev, particlesDev, moleculeDev, start, count, nbrIndex, siteCount;
+    DeviceArray<double> paramsDev, sites, siteEnergy;
+    DeviceArray<double4> nbrDelta;
+    DeviceArray<long long> siteForce, tripleCount;
+    hipStream_t lastStream = nullptr;
+};
+
+namespace {
+
+void check_threebody(const ixh_threebody_desc* d) {
+    if (d == nullptr)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody: null descriptor");
+    if (d->num_sites < 0 || d->num_particles < 0)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody: negative size");
+    if (d->precision != IXH_SINGLE && d->precision != IXH_MIXED && d->precision != IXH_DOUBLE)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody: illegal precision");
+    // InteraxThreeBodyForce.cpp:98-101, 115-118 and InteraxThreeBodyForceImpl.cpp:42-59
+    if (d->damping_method != IXH_DAMPING_NONE && d->damping_method != IXH_DAMPING_TANG_TOENNIES)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxThreeBodyForce: Illegal value for damping method");
+    if (d->centroid_damping_radius < 0)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxThreeBodyForce: The centroid damping radius cannot be negative");
+    for (int i = 0; i < d->num_sites; i++) {
+        if (d->site_offsets[i+1] <= d->site_offsets[i])
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxThreeBodyForce: A site must contain at least one particle");
+        for (int k = d->site_offsets[i]; k < d->site_offsets[i+1]; k++)
+            if (d->site_particles[k] < 0 || d->site_particles[k] >= d->num_particles)
+                throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxThreeBodyForce: Illegal particle index for a site");
+        if (d->damping_method == IXH_DAMPING_TANG_TOENNIES && d->width[i] <= 0)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT,
+                        "InteraxThreeBodyForce: The Tang-Toennies damping requires a positive width for every site");
+    }
+    if (d->cutoff <= 0)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxThreeBodyForce: The cutoff distance must be positive");
+}
+
+void load_threebody_params(ixh_threebody* h, const ixh_threebody_desc* d) {
+    h->params.resize(4*(size_t) h->numSites);
+    for (int i = 0; i < h->numSites; i++) {
+        h->params[4*i] = d->alpha[i];
+        h->params[4*i+1] = d->frequency[i];
+        h->params[4*i+2] = d->width[i];
+        h->params[4*i+3] = h->alchemical[i];
+    }
+    h->scaleFactor = d->scale_factor;
+    h->dampingScale = d->damping_scale;
+    h->centroidDampingRadius = d->centroid_damping_radius;
+    h->paramsDev.upload(h->params.data(), h->params.size(), nullptr);
+    IXH_HIP_CHECK(hipStreamSynchronize(nullptr));
+}
+
+template <typename Real>
+void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, double cutoff2, int maxCount, hipStream_t stream) {
+    typedef typename Vec4Of<Real>::type R4;
+    TripleArgs<Real> t;
+    t.sites = h->sites.ptr;
+    t.molecule = h->moleculeDev.ptr;
+    t.siteParams = h->paramsDev.ptr;
+    t.start = h->start.ptr;
+    t.nbrIndex = h->nbrIndex.ptr;
+    t.nbrDelta = h->nbrDelta.ptr;
+    t.numSites = h->numSites;
+    t.periodic = h->periodic;
+    t.box = box;
+    t.boxR[0] = Real(box.ax);
+    t.boxR[1] = Real(box.bx);
+    t.boxR[2] = Real(box.by);
+    t.boxR[3] = Real(box.cx);
+    t.boxR[4] = Real(box.cy);
+    t.boxR[5] = Real(box.cz);
+    t.boxR[6] = Real(h->periodic ? 1.0/box.ax : 0.0);
+    t.boxR[7] = Real(h->periodic ? 1.0/box.by : 0.0);
+    t.boxR[8] = Real(h->periodic ? 1.0/box.cz : 0.0);
+    t.cutoff2 = cutoff2;
+    t.cutoff2r = Real(cutoff2);
+    t.band = Real(sizeof(Real) == 4 ? 1e-5 : 1e-12);
+    t.scaleFactor = Real(h->scaleFactor);
+    t.lambda = Real(h->anyAlchemical ? b->lambda : 1.0);
+    t.useDamping = (h->dampingMethod == IXH_DAMPING_TANG_TOENNIES);
+    t.dampingScale = Real(h->dampingScale);
+    t.invDampingRadius = Real(h->centroidDampingRadius > 0 ? 1.0/h->centroidDampingRadius : 0.0);
+    t.capacity = std::max(maxCount, 1);
+    t.siteForce = h->siteForce.ptr;
+    t.siteEnergy = h->siteEnergy.ptr;
+    t.siteCount = h->siteCount.ptr;
+    size_t shared = (size_t) t.capacity*(2*sizeof(R4)+3*sizeof(long long)+2*sizeof(int));
+    if (shared > 160*1024)
+        throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_execute: a site has too many neighbours for one workgroup");
+    hipLaunchKernelGGL(k_triples<Real>, dim3(h->numSites), dim3(64), shared, stream, t);
+    IXH_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixh_threebody_create(int device, const ixh_threebody_desc* d, ixh_threebody** out) {
+    return guarded([&] {
+        if (out == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_create: null output pointer");
+        check_threebody(d);
+        DeviceGuard guard(device);
+        ixh_threebody* h = new ixh_threebody();
+        try {
+            h->device = device;
+            h->numParticles = d->num_particles;
+            h->numSites = d->num_sites;
+            h->precision = d->precision;
+            h->cutoff = d->cutoff;
+            h->dampingMethod = d->damping_method;
+            h->periodic = d->periodic;
+            int n = h->numSites;
+            h->offsets.assign(d->site_offsets, d->site_offsets+n+1);
+            h->particles.assign(d->site_particles, d->site_particles+h->offsets[n]);
+            h->molecule.resize(n);
+            h->alchemical.assign(n, 0);
+            for (int i = 0; i < n; i++) {
+                // ReferenceInteraxThreeBodyIxn.cpp:48-49
+                h->molecule[i] = (d->molecule[i] < 0 ? -1-i : d->molecule[i]);
+                h->alchemical[i] = (d->alchemical != nullptr && d->alchemical[i] != 0);
+            }
+            h->anyAlchemical = std::any_of(h->alchemical.begin(), h->alchemical.end(), [](unsigned char c) { return c != 0; });
+            h->offsetsDev.upload(h->offsets.data(), h->offsets.size(), nullptr);
+            h->particlesDev.upload(h->particles.data(), h->particles.size(), nullptr);
+            h->moleculeDev.upload(h->molecule.data(), h->molecule.size(), nullptr);
+            h->sites.reserve(3*(size_t) std::max(n, 1));
+            h->count.reserve(std::max(n, 1));
+            h->siteEnergy.reserve(std::max(n, 1));
+            h->siteCount.reserve(std::max(n, 1));
+            h->siteForce.reserve(3*(size_t) std::max(n, 1));
+            h->tripleCount.reserve(1);
+            load_threebody_params(h, d);
+        }
+        catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int ixh_threebody_destroy(ixh_threebody* h) {
+    return guarded([&] {
+        if (h == nullptr)
+            return;
+        DeviceGuard guard(h->device);
+        (void) hipDeviceSynchronize();
+        delete h;
+    });
+}
+
+int ixh_threebody_set_parameters(ixh_threebody* h, const ixh_threebody_desc* d) {
+    return guarded([&] {
+        if (h == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_set_parameters: null handle");
+        check_threebody(d);
+        if (d->num_sites != h->numSites)
+            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The number of sites has changed");
+        if ((d->centroid_damping_radius > 0) != (h->centroidDampingRadius > 0))
+            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The centroid damping cannot be switched on or off");
+        DeviceGuard guard(h->device);
+        if (h->lastStream != nullptr)
+            IXH_HIP_CHECK(hipStreamSynchronize(h->lastStream));
+        load_threebody_params(h, d);
+    });
+}
+
+int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
+    return guarded([&] {
+        if (h == nullptr || b == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute: null argument");
+        if (b->num_particles != h->numParticles)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute: the buffers hold a different number of particles");
+        int n = h->numSites;
+        if (n == 0)
+            return;
+        DeviceGuard guard(h->device);
+        hipStream_t stream = (hipStream_t) b->stream;
+        h->lastStream = stream;
+        const float4* corr = (const float4*) b->posq_correction;
+        BoxT box = {b->box[0], b->box[3], b->box[4], b->box[6], b->box[7], b->box[8]};
+        hipLaunchKernelGGL(k_sites, dim3(grid_of(n, 128)), dim3(128), 0, stream, b->posq, corr, b->precision, h->offsetsDev.ptr,
+                           h->particlesDev.ptr, n, h->sites.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        double cutoff2 = h->cutoff*h->cutoff;
+        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2,
+                           h->count.ptr, (const int*) nullptr, (int*) nullptr, (double4*) nullptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        std::vector<int> counts(n), starts(n+1, 0);
+        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), h->count.ptr, sizeof(int)*(size_t) n, hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        int maxCount = 0;
+        for (int i = 0; i < n; i++) {
+            starts[i+1] = starts[i]+counts[i];
+            maxCount = std::max(maxCount, counts[i]);
+        }
+        h->start.upload(starts.data(), starts.size(), stream);
+        h->nbrIndex.reserve(std::max(starts[n], 1));
+        h->nbrDelta.reserve(std::max(starts[n], 1));
+        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2,
+                           h->count.ptr, (const int*) h->start.ptr, h->nbrIndex.ptr, h->nbrDelta.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
+        IXH_HIP_CHECK(hipGetLastError());
+        if (h->precision == IXH_DOUBLE)
+            launch_triples<double>(h, b, box, cutoff2, maxCount, stream);
+        else
+            launch_triples<float>(h, b, box, cutoff2, maxCount, stream);
+        if (flags & IXH_INCLUDE_FORCES) {
+            hipLaunchKernelGGL(k_apply_site_forces, dim3(grid_of(n, 128)), dim3(128), 0, stream, h->siteForce.ptr, n, h->offsetsDev.ptr,
+                               h->particlesDev.ptr, b->force, b->force_stride);
+            IXH_HIP_CHECK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
+                           h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
+        IXH_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int ixh_threebody_last_triple_count(ixh_threebody* h, long long* count) {
+    return guarded([&] {
+        if (h == nullptr || count == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_last_triple_count: null argument");
+        DeviceGuard guard(h->device);
+        IXH_HIP_CHECK(hipStreamSynchronize(h->lastStream));
+        IXH_HIP_CHECK(hipMemcpy(count, h->tripleCount.ptr, sizeof(long long), hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,774 @@
+"""Python binding of the interax_hip C ABI (include/interax_hip.h).
+
+It mirrors the public API of the reference plugin's force classes (openmmapi/include/InteraxPairForce.h:70-239,
+InteraxDispersionForce.h:70-182, InteraxExchangeForce.h:60-160, InteraxShortRangeForce.h:59-162,
+InteraxThreeBodyForce.h:75-258): same method names, argument meaning, defaults and error messages, so that a
+System can be described as with the reference and evaluated on the GPU.  `Context` plays the role of an OpenMM
+Context on this platform: it owns the device buffers (ixh_state) and one kernel handle per fused group of pair
+forces and per three-body force, and evaluates force groups on request (the `groups` argument of getState).
+
+Only the native library does any arithmetic; this module moves arrays across the C ABI.  Loading fails loudly if
+libinterax_hip.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import gzip
+import math
+import os
+import xml.etree.ElementTree as ET
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libinterax_hip.so")
+
+IXH_OK = 0
+IXH_SINGLE, IXH_MIXED, IXH_DOUBLE = 0, 1, 2
+IXH_TERM_DISPERSION, IXH_TERM_EXCHANGE, IXH_TERM_SHORT_RANGE = 1, 2, 4
+IXH_INCLUDE_FORCES, IXH_INCLUDE_ENERGY = 1, 2
+PRECISIONS = {"single": IXH_SINGLE, "mixed": IXH_MIXED, "double": IXH_DOUBLE}
+
+
+class InteraxError(RuntimeError):
+    """The error raised where the reference throws OpenMMException."""
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# ctypes declarations of include/interax_hip.h
+# ----------------------------------------------------------------------------------------------------------------
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_ucp = ctypes.POINTER(ctypes.c_ubyte)
+
+
+class Buffers(ctypes.Structure):
+    _fields_ = [("num_particles", ctypes.c_int), ("precision", ctypes.c_int), ("posq", ctypes.c_void_p),
+                ("posq_correction", ctypes.c_void_p), ("force", ctypes.c_void_p), ("force_stride", ctypes.c_int),
+                ("energy", ctypes.c_void_p), ("box", ctypes.c_double*9), ("lambda_", ctypes.c_double),
+                ("stream", ctypes.c_void_p)]
+
+
+class PairDesc(ctypes.Structure):
+    _fields_ = [("num_particles", ctypes.c_int), ("terms", ctypes.c_int), ("method", ctypes.c_int),
+                ("cutoff", ctypes.c_double), ("use_switching_function", ctypes.c_int),
+                ("switching_distance", ctypes.c_double), ("softcore_radius", ctypes.c_double),
+                ("alchemical", _ucp), ("num_exclusions", ctypes.c_int), ("exclusions", _ip),
+                ("precision", ctypes.c_int), ("dispersion", _dp), ("damping_scale", ctypes.c_double),
+                ("use_long_range_correction", ctypes.c_int), ("exchange", _dp),
+                ("exchange_overlap_method", ctypes.c_int), ("short_range", _dp),
+                ("short_range_overlap_method", ctypes.c_int), ("default_box", _dp)]
+
+
+class ThreeBodyDesc(ctypes.Structure):
+    _fields_ = [("num_particles", ctypes.c_int), ("num_sites", ctypes.c_int), ("site_offsets", _ip),
+                ("site_particles", _ip), ("alpha", _dp), ("frequency", _dp), ("width", _dp), ("molecule", _ip),
+                ("alchemical", _ucp), ("cutoff", ctypes.c_double), ("scale_factor", ctypes.c_double),
+                ("damping_method", ctypes.c_int), ("damping_scale", ctypes.c_double),
+                ("centroid_damping_radius", ctypes.c_double), ("periodic", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+# Every exported symbol of include/interax_hip.h: (name, argtypes).
+SIGNATURES = {
+    "ixh_last_error": [],
+    "ixh_version": [_ip, _ip],
+    "ixh_device_count": [_ip],
+    "ixh_state_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)],
+    "ixh_state_destroy": [ctypes.c_void_p],
+    "ixh_state_set_positions": [ctypes.c_void_p, _dp],
+    "ixh_state_set_box": [ctypes.c_void_p, _dp],
+    "ixh_state_set_lambda": [ctypes.c_void_p, ctypes.c_double],
+    "ixh_state_clear": [ctypes.c_void_p],
+    "ixh_state_get_forces": [ctypes.c_void_p, _dp],
+    "ixh_state_get_energy": [ctypes.c_void_p, _dp],
+    "ixh_state_synchronize": [ctypes.c_void_p],
+    "ixh_state_get_buffers": [ctypes.c_void_p, ctypes.POINTER(Buffers)],
+    "ixh_pair_create": [ctypes.c_int, ctypes.POINTER(PairDesc), ctypes.POINTER(ctypes.c_void_p)],
+    "ixh_pair_destroy": [ctypes.c_void_p],
+    "ixh_pair_set_parameters": [ctypes.c_void_p, ctypes.POINTER(PairDesc)],
+    "ixh_pair_execute": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int],
+    "ixh_pair_long_range_coefficients": [ctypes.c_void_p, _dp, _dp],
+    "ixh_pair_long_range_energy": [ctypes.c_void_p, _dp, ctypes.c_double, _dp],
+    "ixh_pair_last_pair_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
+    "ixh_threebody_create": [ctypes.c_int, ctypes.POINTER(ThreeBodyDesc), ctypes.POINTER(ctypes.c_void_p)],
+    "ixh_threebody_destroy": [ctypes.c_void_p],
+    "ixh_threebody_set_parameters": [ctypes.c_void_p, ctypes.POINTER(ThreeBodyDesc)],
+    "ixh_threebody_execute": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int],
+    "ixh_threebody_last_triple_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
+}
+
+_lib = None
+
+
+def load_library(path: str = LIBRARY_PATH) -> ctypes.CDLL:
+    """Load libinterax_hip.so (raises if it has not been built: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise InteraxError(f"{path} has not been built (run __graft_entry__.build() or make -C openmm-interax-plugin_amd)")
+    lib = ctypes.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_char_p if name == "ixh_last_error" else ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != IXH_OK:
+        raise InteraxError(_lib.ixh_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+def _uptr(a: np.ndarray):
+    return a.ctypes.data_as(_ucp)
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# Force classes (the reference's public API)
+# ----------------------------------------------------------------------------------------------------------------
+
+class InteraxPairForce:
+    """Base of the pair forces (InteraxPairForce.h:70-239; defaults InteraxPairForce.cpp:41-43)."""
+    NoCutoff, CutoffNonPeriodic, CutoffPeriodic = 0, 1, 2
+
+    def __init__(self):
+        self._method = self.NoCutoff
+        self._cutoff = 1.0
+        self._switching_distance = -1.0
+        self._softcore_radius = 0.0
+        self._use_switch = False
+        self._alchemical_name = "lambda_sterics"
+        self._exclusions: List[Tuple[int, int]] = []
+        self._group = 0
+
+    # settings
+    def getNonbondedMethod(self): return self._method
+
+    def setNonbondedMethod(self, method):
+        if method < 0 or method > 2:
+            raise InteraxError("InteraxPairForce: Illegal value for nonbonded method")
+        self._method = int(method)
+
+    def getCutoffDistance(self): return self._cutoff
+    def setCutoffDistance(self, d): self._cutoff = float(d)
+    def getUseSwitchingFunction(self): return self._use_switch
+    def setUseSwitchingFunction(self, use): self._use_switch = bool(use)
+    def getSwitchingDistance(self): return self._switching_distance
+    def setSwitchingDistance(self, d): self._switching_distance = float(d)
+    def getNumExclusions(self): return len(self._exclusions)
+
+    def addExclusion(self, p1, p2):
+        self._exclusions.append((int(p1), int(p2)))
+        return len(self._exclusions)-1
+
+    def getExclusionParticles(self, index): return self._exclusions[index]
+    def setExclusionParticles(self, index, p1, p2): self._exclusions[index] = (int(p1), int(p2))
+    def getAlchemicalParameterName(self): return self._alchemical_name
+    def setAlchemicalParameterName(self, name): self._alchemical_name = str(name)
+    def getSoftcoreRadius(self): return self._softcore_radius
+
+    def setSoftcoreRadius(self, r):
+        if r < 0:
+            raise InteraxError("InteraxPairForce: The soft-core radius cannot be negative")
+        self._softcore_radius = float(r)
+
+    def usesPeriodicBoundaryConditions(self): return self._method == self.CutoffPeriodic
+    def getForceGroup(self): return self._group
+    def setForceGroup(self, g): self._group = int(g)
+
+    # common per-particle views
+    def _alchemical_array(self) -> np.ndarray:
+        return np.array([self.getParticleIsAlchemical(i) for i in range(self.getNumParticles())], dtype=np.uint8)
+
+    def _exclusion_array(self) -> np.ndarray:
+        return np.array(self._exclusions, dtype=np.int32).reshape(-1, 2)
+
+
+class InteraxDispersionForce(InteraxPairForce):
+    """InteraxDispersionForce.h:70-182 (defaults InteraxDispersionForce.cpp:39-40)."""
+    NAME = "InteraxDispersionForce"
+    TERM = IXH_TERM_DISPERSION
+
+    def __init__(self):
+        super().__init__()
+        self._particles: List[Tuple[float, float, float, float, float, bool]] = []
+        self._damping_scale = 1.0
+        self._use_lrc = False
+
+    def getNumParticles(self): return len(self._particles)
+
+    def addParticle(self, m1, m2, m3, alpha, width, isAlchemical=False):
+        self._particles.append((float(m1), float(m2), float(m3), float(alpha), float(width), bool(isAlchemical)))
+        return len(self._particles)-1
+
+    def getParticleParameters(self, index): return self._particles[index]
+
+    def setParticleParameters(self, index, m1, m2, m3, alpha, width, isAlchemical):
+        self._particles[index] = (float(m1), float(m2), float(m3), float(alpha), float(width), bool(isAlchemical))
+
+    def getParticleIsAlchemical(self, index): return self._particles[index][5]
+    def getDampingScale(self): return self._damping_scale
+    def setDampingScale(self, s): self._damping_scale = float(s)
+    def getUseLongRangeCorrection(self): return self._use_lrc
+    def setUseLongRangeCorrection(self, use): self._use_lrc = bool(use)
+
+    def parameter_array(self) -> np.ndarray:
+        return np.array([p[:5] for p in self._particles], dtype=np.float64).reshape(-1, 5)
+
+
+class _OverlapForce(InteraxPairForce):
+    Exact, InteraxOrder2 = 0, 2
+
+    def __init__(self):
+        super().__init__()
+        self._overlap_method = self.Exact
+
+    def getOverlapMethod(self): return self._overlap_method
+
+    def setOverlapMethod(self, method):
+        if method not in (self.Exact, self.InteraxOrder2):
+            raise InteraxError(f"{self.NAME}: Illegal value for overlap method")
+        self._overlap_method = int(method)
+
+
+class InteraxExchangeForce(_OverlapForce):
+    """InteraxExchangeForce.h:60-160."""
+    NAME = "InteraxExchangeForce"
+    TERM = IXH_TERM_EXCHANGE
+
+    def __init__(self):
+        super().__init__()
+        self._particles: List[Tuple[float, float, bool]] = []
+
+    def getNumParticles(self): return len(self._particles)
+
+    def addParticle(self, amplitude, width, isAlchemical=False):
+        self._particles.append((float(amplitude), float(width), bool(isAlchemical)))
+        return len(self._particles)-1
+
+    def getParticleParameters(self, index): return self._particles[index]
+
+    def setParticleParameters(self, index, amplitude, width, isAlchemical):
+        self._particles[index] = (float(amplitude), float(width), bool(isAlchemical))
+
+    def getParticleIsAlchemical(self, index): return self._particles[index][2]
+
+    def parameter_array(self) -> np.ndarray:
+        return np.array([p[:2] for p in self._particles], dtype=np.float64).reshape(-1, 2)
+
+
+class InteraxShortRangeForce(_OverlapForce):
+    """InteraxShortRangeForce.h:59-162."""
+    NAME = "InteraxShortRangeForce"
+    TERM = IXH_TERM_SHORT_RANGE
+
+    def __init__(self):
+        super().__init__()
+        self._particles: List[Tuple[float, float, float, bool]] = []
+
+    def getNumParticles(self): return len(self._particles)
+
+    def addParticle(self, k, l, width, isAlchemical=False):
+        self._particles.append((float(k), float(l), float(width), bool(isAlchemical)))
+        return len(self._particles)-1
+
+    def getParticleParameters(self, index): return self._particles[index]
+
+    def setParticleParameters(self, index, k, l, width, isAlchemical):
+        self._particles[index] = (float(k), float(l), float(width), bool(isAlchemical))
+
+    def getParticleIsAlchemical(self, index): return self._particles[index][3]
+
+    def parameter_array(self) -> np.ndarray:
+        return np.array([p[:3] for p in self._particles], dtype=np.float64).reshape(-1, 3)
+
+
+class InteraxThreeBodyForce:
+    """InteraxThreeBodyForce.h:75-258 (defaults InteraxThreeBodyForce.cpp:39-41)."""
+    NAME = "InteraxThreeBodyForce"
+    NoDamping, TangToennies = 0, 1
+
+    def __init__(self):
+        self._sites: List[Tuple[List[int], float, float, float, int, bool]] = []
+        self._cutoff = 1.2
+        self._scale = 1.0
+        self._damping_scale = 1.0
+        self._centroid_radius = 0.0
+        self._damping_method = self.NoDamping
+        self._periodic = False
+        self._alchemical_name = "lambda_sterics"
+        self._group = 0
+
+    def getNumSites(self): return len(self._sites)
+
+    def addSite(self, particles, alpha, frequency, width=0.0, molecule=-1, isAlchemical=False):
+        self._sites.append((list(map(int, particles)), float(alpha), float(frequency), float(width), int(molecule),
+                            bool(isAlchemical)))
+        return len(self._sites)-1
+
+    def getSiteParameters(self, index): return self._sites[index]
+
+    def setSiteParameters(self, index, particles, alpha, frequency, width, molecule, isAlchemical):
+        self._sites[index] = (list(map(int, particles)), float(alpha), float(frequency), float(width), int(molecule),
+                              bool(isAlchemical))
+
+    def getCutoffDistance(self): return self._cutoff
+    def setCutoffDistance(self, d): self._cutoff = float(d)
+    def getScaleFactor(self): return self._scale
+    def setScaleFactor(self, s): self._scale = float(s)
+    def getDampingMethod(self): return self._damping_method
+
+    def setDampingMethod(self, method):
+        if method not in (self.NoDamping, self.TangToennies):
+            raise InteraxError("InteraxThreeBodyForce: Illegal value for damping method")
+        self._damping_method = int(method)
+
+    def getDampingScale(self): return self._damping_scale
+    def setDampingScale(self, s): self._damping_scale = float(s)
+    def getCentroidDampingRadius(self): return self._centroid_radius
+
+    def setCentroidDampingRadius(self, r):
+        if r < 0:
+            raise InteraxError("InteraxThreeBodyForce: The centroid damping radius cannot be negative")
+        self._centroid_radius = float(r)
+
+    def setUsesPeriodicBoundaryConditions(self, periodic): self._periodic = bool(periodic)
+    def usesPeriodicBoundaryConditions(self): return self._periodic
+    def getAlchemicalParameterName(self): return self._alchemical_name
+    def setAlchemicalParameterName(self, name): self._alchemical_name = str(name)
+    def getForceGroup(self): return self._group
+    def setForceGroup(self, g): self._group = int(g)
+
+    def site_arrays(self):
+        """(offsets, particles, alpha, frequency, width, molecule, alchemical) as numpy arrays."""
+        offsets = np.zeros(len(self._sites)+1, dtype=np.int32)
+        parts: List[int] = []
+        for i, s in enumerate(self._sites):
+            parts.extend(s[0])
+            offsets[i+1] = len(parts)
+        col = lambda k, dt: np.array([s[k] for s in self._sites], dtype=dt)
+        return (offsets, np.array(parts, dtype=np.int32), col(1, np.float64), col(2, np.float64), col(3, np.float64),
+                col(4, np.int32), col(5, np.uint8))
+
+
+class System:
+    """The part of OpenMM's System the plugin needs: particle count, default box, forces."""
+
+    def __init__(self, num_particles: int = 0, box: Optional[Sequence[float]] = None):
+        self._n = int(num_particles)
+        self._forces: List[object] = []
+        self._box = np.array(box if box is not None else [2, 0, 0, 0, 2, 0, 0, 0, 2], dtype=np.float64)
+
+    def addParticle(self, mass=1.0):
+        self._n += 1
+        return self._n-1
+
+    def getNumParticles(self): return self._n
+
+    def addForce(self, force):
+        self._forces.append(force)
+        return len(self._forces)-1
+
+    def getForces(self): return list(self._forces)
+    def getNumForces(self): return len(self._forces)
+    def getForce(self, i): return self._forces[i]
+
+    def setDefaultPeriodicBoxVectors(self, a, b, c):
+        self._box = np.array(list(a)+list(b)+list(c), dtype=np.float64)
+
+    def getDefaultPeriodicBoxVectors(self): return self._box.reshape(3, 3).copy()
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# XML (the version 1 format of the reference's serialization proxies, serialization/src/*Proxy.cpp)
+# ----------------------------------------------------------------------------------------------------------------
+
+def _b(node, name):
+    return int(node.get(name)) != 0
+
+
+def load_system_xml(source) -> System:
+    """Read a System written by OpenMM's XmlSerializer with the four Interax forces (others are ignored).
+
+    source: a path (optionally .gz) or XML text."""
+    if isinstance(source, str) and not source.lstrip().startswith("<"):
+        opener = gzip.open if source.endswith(".gz") else open
+        with opener(source, "rt") as f:
+            text = f.read()
+    else:
+        text = source
+    root = ET.fromstring(text)
+    box = root.find("PeriodicBoxVectors")
+    vec = [float(box.find(k).get(c)) for k in "ABC" for c in "xyz"] if box is not None else None
+    system = System(len(root.find("Particles")), vec)
+    for node in root.find("Forces"):
+        kind = node.get("type")
+        if kind not in ("InteraxDispersionForce", "InteraxExchangeForce", "InteraxShortRangeForce",
+                        "InteraxThreeBodyForce"):
+            continue
+        if int(node.get("version")) != 1:
+            raise InteraxError("Unsupported version number")
+        if kind == "InteraxThreeBodyForce":
+            # InteraxThreeBodyForceProxy.cpp deserialize
+            f = InteraxThreeBodyForce()
+            f.setForceGroup(int(node.get("forceGroup", 0)))
+            f.setCutoffDistance(float(node.get("cutoff")))
+            f.setScaleFactor(float(node.get("scaleFactor")))
+            f.setUsesPeriodicBoundaryConditions(_b(node, "usesPeriodic"))
+            f.setAlchemicalParameterName(node.get("alchemicalParameter"))
+            f.setDampingMethod(int(node.get("dampingMethod")))
+            f.setDampingScale(float(node.get("dampingScale")))
+            f.setCentroidDampingRadius(float(node.get("centroidDampingRadius")))
+            for site in node.find("Sites"):
+                f.addSite([int(p.get("index")) for p in site], float(site.get("alpha")), float(site.get("frequency")),
+                          float(site.get("width")), int(site.get("molecule")), _b(site, "alchemical"))
+            system.addForce(f)
+            continue
+        # Interax{Dispersion,Exchange,ShortRange}ForceProxy.cpp deserialize
+        f = {"InteraxDispersionForce": InteraxDispersionForce, "InteraxExchangeForce": InteraxExchangeForce,
+             "InteraxShortRangeForce": InteraxShortRangeForce}[kind]()
+        f.setForceGroup(int(node.get("forceGroup", 0)))
+        f.setNonbondedMethod(int(node.get("method")))
+        f.setCutoffDistance(float(node.get("cutoff")))
+        f.setUseSwitchingFunction(_b(node, "useSwitchingFunction"))
+        f.setSwitchingDistance(float(node.get("switchingDistance")))
+        f.setAlchemicalParameterName(node.get("alchemicalParameter"))
+        f.setSoftcoreRadius(float(node.get("softcoreRadius")))
+        if kind == "InteraxDispersionForce":
+            f.setDampingScale(float(node.get("dampingScale")))
+            f.setUseLongRangeCorrection(_b(node, "useLongRangeCorrection"))
+            for p in node.find("Particles"):
+                f.addParticle(float(p.get("m1")), float(p.get("m2")), float(p.get("m3")), float(p.get("alpha")),
+                              float(p.get("width")), _b(p, "alchemical"))
+        elif kind == "InteraxExchangeForce":
+            f.setOverlapMethod(int(node.get("overlapMethod")))
+            for p in node.find("Particles"):
+                f.addParticle(float(p.get("amplitude")), float(p.get("width")), _b(p, "alchemical"))
+        else:
+            f.setOverlapMethod(int(node.get("overlapMethod")))
+            for p in node.find("Particles"):
+                f.addParticle(float(p.get("k")), float(p.get("l")), float(p.get("width")), _b(p, "alchemical"))
+        for e in node.find("Exclusions"):
+            f.addExclusion(int(e.get("p1")), int(e.get("p2")))
+        system.addForce(f)
+    return system
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# Context
+# ----------------------------------------------------------------------------------------------------------------
+
+def check_pair_force(force: InteraxPairForce, system: System) -> None:
+    """InteraxPairForce.cpp:113-136 (checkPairForce)."""
+    name = force.NAME
+    n = force.getNumParticles()
+    if n != system.getNumParticles():
+        raise InteraxError(f"{name} must have exactly as many particles as the System it belongs to.")
+    for p1, p2 in force._exclusions:
+        if p1 < 0 or p1 >= n or p2 < 0 or p2 >= n:
+            raise InteraxError(f"{name}: Illegal particle index for an exclusion")
+    if force.getNonbondedMethod() == InteraxPairForce.NoCutoff:
+        return
+    if force.getCutoffDistance() <= 0:
+        raise InteraxError(f"{name}: The cutoff distance must be positive")
+    if force.getUseSwitchingFunction() and (force.getSwitchingDistance() < 0 or
+                                            force.getSwitchingDistance() >= force.getCutoffDistance()):
+        raise InteraxError(f"{name}: Switching distance must satisfy 0 <= r_switch < r_cutoff")
+    if force.getNonbondedMethod() == InteraxPairForce.CutoffPeriodic:
+        b = system.getDefaultPeriodicBoxVectors()
+        c = force.getCutoffDistance()
+        if c > 0.5*b[0, 0] or c > 0.5*b[1, 1] or c > 0.5*b[2, 2]:
+            raise InteraxError(f"{name}: The cutoff distance cannot be greater than half the periodic box size.")
+
+
+def _fusion_key(force: InteraxPairForce, precision: int):
+    excl = force._exclusion_array()
+    excl = np.sort(np.sort(excl, axis=1), axis=0) if len(excl) else excl
+    ex_key = tuple(map(tuple, np.unique(excl, axis=0))) if len(excl) else ()
+    return (force.getNonbondedMethod(), force.getCutoffDistance(), force.getUseSwitchingFunction(),
+            force.getSwitchingDistance(), force.getSoftcoreRadius(), force.getAlchemicalParameterName(),
+            ex_key, force._alchemical_array().tobytes(), precision)
+
+
+class _PairGroup:
+    """Pair forces sharing geometry settings: one ixh_pair handle, one kernel launch per evaluation."""
+
+    def __init__(self, device: int, system: System, forces: List[InteraxPairForce], precision: int):
+        self.forces = forces
+        self.device = device
+        self.system = system
+        self.precision = precision
+        self.handle = ctypes.c_void_p()
+        desc, self._keep = self._describe()
+        _check(_lib.ixh_pair_create(device, ctypes.byref(desc), ctypes.byref(self.handle)))
+
+    def _describe(self):
+        f0 = self.forces[0]
+        n = f0.getNumParticles()
+        keep = []
+        d = PairDesc()
+        d.num_particles = n
+        d.terms = 0
+        d.method = f0.getNonbondedMethod()
+        d.cutoff = f0.getCutoffDistance()
+        d.use_switching_function = int(f0.getUseSwitchingFunction())
+        d.switching_distance = f0.getSwitchingDistance()
+        d.softcore_radius = f0.getSoftcoreRadius()
+        alch = f0._alchemical_array()
+        excl = np.ascontiguousarray(f0._exclusion_array().reshape(-1), dtype=np.int32)
+        box = np.ascontiguousarray(self.system.getDefaultPeriodicBoxVectors().reshape(-1))
+        keep += [alch, excl, box]
+        d.alchemical = _uptr(alch)
+        d.num_exclusions = len(excl)//2
+        d.exclusions = _iptr(excl)
+        d.precision = self.precision
+        d.default_box = _dptr(box)
+        for f in self.forces:
+            d.terms |= f.TERM
+            arr = np.ascontiguousarray(f.parameter_array().reshape(-1))
+            keep.append(arr)
+            if isinstance(f, InteraxDispersionForce):
+                d.dispersion = _dptr(arr)
+                d.damping_scale = f.getDampingScale()
+                d.use_long_range_correction = int(f.getUseLongRangeCorrection())
+            elif isinstance(f, InteraxExchangeForce):
+                d.exchange = _dptr(arr)
+                d.exchange_overlap_method = f.getOverlapMethod()
+            else:
+                d.short_range = _dptr(arr)
+                d.short_range_overlap_method = f.getOverlapMethod()
+        return d, keep
+
+    def update(self):
+        desc, keep = self._describe()
+        _check(_lib.ixh_pair_set_parameters(self.handle, ctypes.byref(desc)))
+
+    def close(self):
+        if self.handle:
+            _lib.ixh_pair_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+class _ThreeBody:
+    def __init__(self, device: int, system: System, force: InteraxThreeBodyForce, precision: int):
+        self.force = force
+        self.system = system
+        self.precision = precision
+        self.handle = ctypes.c_void_p()
+        desc, self._keep = self._describe()
+        _check(_lib.ixh_threebody_create(device, ctypes.byref(desc), ctypes.byref(self.handle)))
+
+    def _describe(self):
+        f = self.force
+        arrays = [np.ascontiguousarray(a) for a in f.site_arrays()]
+        offsets, parts, alpha, freq, width, mol, alch = arrays
+        d = ThreeBodyDesc()
+        d.num_particles = self.system.getNumParticles()
+        d.num_sites = f.getNumSites()
+        d.site_offsets = _iptr(offsets)
+        d.site_particles = _iptr(parts)
+        d.alpha = _dptr(alpha)
+        d.frequency = _dptr(freq)
+        d.width = _dptr(width)
+        d.molecule = _iptr(mol)
+        d.alchemical = _uptr(alch)
+        d.cutoff = f.getCutoffDistance()
+        d.scale_factor = f.getScaleFactor()
+        d.damping_method = f.getDampingMethod()
+        d.damping_scale = f.getDampingScale()
+        d.centroid_damping_radius = f.getCentroidDampingRadius()
+        d.periodic = int(f.usesPeriodicBoundaryConditions())
+        d.precision = self.precision
+        return d, arrays
+
+    def update(self):
+        desc, keep = self._describe()
+        _check(_lib.ixh_threebody_set_parameters(self.handle, ctypes.byref(desc)))
+
+    def close(self):
+        if self.handle:
+            _lib.ixh_threebody_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+class State:
+    def __init__(self, energy: float, forces: Optional[np.ndarray]):
+        self._energy = energy
+        self._forces = forces
+
+    def getPotentialEnergy(self) -> float: return self._energy
+    def getForces(self) -> np.ndarray: return self._forces
+
+
+class Context:
+    """A System on one MI355X: device buffers plus one kernel handle per fused pair group and three-body force.
+
+    properties: {"Precision": "single" | "mixed" | "double", "DeviceIndex": "0"} as OpenMM's GPU platforms."""
+
+    def __init__(self, system: System, properties: Optional[Dict[str, str]] = None):
+        load_library()
+        properties = properties or {}
+        self.system = system
+        self.precision = PRECISIONS[properties.get("Precision", "mixed")]
+        self.device = int(properties.get("DeviceIndex", "0"))
+        n = system.getNumParticles()
+        self._parameters: Dict[str, float] = {}
+        self._groups: List[_PairGroup] = []
+        self._threebody: List[_ThreeBody] = []
+        self._state = ctypes.c_void_p()
+        pair_forces = [f for f in system.getForces() if isinstance(f, InteraxPairForce)]
+        for f in pair_forces:
+            check_pair_force(f, system)
+        for f in system.getForces():
+            if isinstance(f, InteraxThreeBodyForce):
+                self._validate_threebody(f, n)
+        # default context parameters (InteraxPairForce.cpp:138-146, InteraxThreeBodyForceImpl.cpp:70-84)
+        for f in system.getForces():
+            if isinstance(f, InteraxPairForce) and any(f.getParticleIsAlchemical(i) for i in range(f.getNumParticles())):
+                self._parameters[f.getAlchemicalParameterName()] = 1.0
+            if isinstance(f, InteraxThreeBodyForce) and any(s[5] for s in f._sites):
+                self._parameters[f.getAlchemicalParameterName()] = 1.0
+        _check(_lib.ixh_state_create(self.device, n, self.precision, ctypes.byref(self._state)))
+        # fuse pair forces with identical geometry settings (one force of each class per group)
+        groups: List[Tuple[object, List[InteraxPairForce]]] = []
+        for f in pair_forces:
+            key = _fusion_key(f, self.precision)
+            for k, members in groups:
+                if k == key and all(type(m) is not type(f) for m in members):
+                    members.append(f)
+                    break
+            else:
+                groups.append((key, [f]))
+        for _, members in groups:
+            self._groups.append(_PairGroup(self.device, system, members, self.precision))
+        for f in system.getForces():
+            if isinstance(f, InteraxThreeBodyForce):
+                self._threebody.append(_ThreeBody(self.device, system, f, self.precision))
+        self._box = system.getDefaultPeriodicBoxVectors().reshape(-1).copy()
+        _check(_lib.ixh_state_set_box(self._state, _dptr(self._box)))
+
+    @staticmethod
+    def _validate_threebody(f: InteraxThreeBodyForce, n: int) -> None:
+        """InteraxThreeBodyForceImpl.cpp:42-59."""
+        for particles, alpha, freq, width, mol, alch in f._sites:
+            if not particles:
+                raise InteraxError("InteraxThreeBodyForce: A site must contain at least one particle")
+            if any(p < 0 or p >= n for p in particles):
+                raise InteraxError("InteraxThreeBodyForce: Illegal particle index for a site")
+            if f.getDampingMethod() == InteraxThreeBodyForce.TangToennies and width <= 0:
+                raise InteraxError("InteraxThreeBodyForce: The Tang-Toennies damping requires a positive width for every site")
+        if f.getCutoffDistance() <= 0:
+            raise InteraxError("InteraxThreeBodyForce: The cutoff distance must be positive")
+
+    def setPositions(self, positions) -> None:
+        p = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
+        if p.size != 3*self.system.getNumParticles():
+            raise InteraxError("Called setPositions() on a Context with the wrong number of positions")
+        _check(_lib.ixh_state_set_positions(self._state, _dptr(p)))
+
+    def setPeriodicBoxVectors(self, a, b, c) -> None:
+        self._box = np.array(list(a)+list(b)+list(c), dtype=np.float64)
+        _check(_lib.ixh_state_set_box(self._state, _dptr(self._box)))
+
+    def setParameter(self, name: str, value: float) -> None:
+        if name not in self._parameters:
+            raise InteraxError(f"Called setParameter() with invalid parameter name: {name}")
+        self._parameters[name] = float(value)
+
+    def getParameter(self, name: str) -> float:
+        return self._parameters[name]
+
+    def _buffers(self, lam: float) -> Buffers:
+        b = Buffers()
+        _check(_lib.ixh_state_get_buffers(self._state, ctypes.byref(b)))
+        b.lambda_ = lam
+        return b
+
+    def getState(self, getEnergy: bool = True, getForces: bool = True, groups: int = 0xFFFFFFFF) -> State:
+        _check(_lib.ixh_state_clear(self._state))
+        flags = IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY
+        host_energy = 0.0
+        for g in self._groups:
+            terms = 0
+            for f in g.forces:
+                if groups & (1 << f.getForceGroup()):
+                    terms |= f.TERM
+            if terms == 0:
+                continue
+            lam = self._parameters.get(g.forces[0].getAlchemicalParameterName(), 1.0)
+            b = self._buffers(lam)
+            _check(_lib.ixh_pair_execute(g.handle, ctypes.byref(b), terms, flags))
+            if terms & IXH_TERM_DISPERSION:
+                e = ctypes.c_double()
+                _check(_lib.ixh_pair_long_range_energy(g.handle, _dptr(self._box), lam, ctypes.byref(e)))
+                host_energy += e.value
+        for t in self._threebody:
+            if groups & (1 << t.force.getForceGroup()):
+                lam = self._parameters.get(t.force.getAlchemicalParameterName(), 1.0)
+                b = self._buffers(lam)
+                _check(_lib.ixh_threebody_execute(t.handle, ctypes.byref(b), flags))
+        energy = ctypes.c_double()
+        _check(_lib.ixh_state_get_energy(self._state, ctypes.byref(energy)))
+        forces = None
+        if getForces:
+            forces = np.zeros((self.system.getNumParticles(), 3))
+            _check(_lib.ixh_state_get_forces(self._state, _dptr(forces)))
+        return State(energy.value+host_energy, forces)
+
+    def updateParametersInContext(self, force) -> None:
+        """Force.updateParametersInContext (InteraxPairForceImpl.h:97-100, InteraxThreeBodyForceImpl.cpp:92-95)."""
+        for g in self._groups:
+            if any(f is force for f in g.forces):
+                if force.getNumParticles() != g.forces[0].getNumParticles() and len(g.forces) == 1:
+                    raise InteraxError("updateParametersInContext: The number of particles has changed")
+                g.update()
+                return
+        for t in self._threebody:
+            if t.force is force:
+                t.update()
+                return
+        raise InteraxError("updateParametersInContext: the force is not part of this Context")
+
+    def pair_counts(self) -> List[int]:
+        out = []
+        for g in self._groups:
+            c = ctypes.c_longlong()
+            _check(_lib.ixh_pair_last_pair_count(g.handle, ctypes.byref(c)))
+            out.append(c.value)
+        return out
+
+    def triple_counts(self) -> List[int]:
+        out = []
+        for t in self._threebody:
+            c = ctypes.c_longlong()
+            _check(_lib.ixh_threebody_last_triple_count(t.handle, ctypes.byref(c)))
+            out.append(c.value)
+        return out
+
+    def close(self) -> None:
+        for g in self._groups:
+            g.close()
+        for t in self._threebody:
+            t.close()
+        if self._state:
+            _lib.ixh_state_destroy(self._state)
+            self._state = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
