@@ -174,6 +174,15 @@ int ixh_pair_long_range_energy(ixh_pair* pair, const double* box, double lambda,
 /* Number of interacting pairs of the last execute() (waits for it). */
 int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
 
+/*
+ * Measurement (no counterpart in the reference): with profiling enabled, every launch of the pair kernel (and of
+ * the triple kernel) is bracketed by HIP events on its stream.  *_kernel_time() waits for the recorded launches and
+ * returns their summed duration in milliseconds and their number since the last reset.
+ */
+int ixh_set_profiling(int enable);
+int ixh_pair_kernel_time(ixh_pair* pair, double* milliseconds, long long* launches);
+int ixh_pair_reset_kernel_time(ixh_pair* pair);
+
 /* InteraxThreeBodyForce (InteraxThreeBodyForce.h:75-258). */
 typedef struct {
     int num_particles;                 /* of the System */
@@ -205,6 +214,8 @@ int ixh_threebody_set_parameters(ixh_threebody* threebody, const ixh_threebody_d
 int ixh_threebody_execute(ixh_threebody* threebody, const ixh_buffers* buffers, int flags);
 /* Number of interacting triples of the last execute() (waits for it). */
 int ixh_threebody_last_triple_count(ixh_threebody* threebody, long long* count);
+int ixh_threebody_kernel_time(ixh_threebody* threebody, double* milliseconds, long long* launches);
+int ixh_threebody_reset_kernel_time(ixh_threebody* threebody);
 
 #ifdef __cplusplus
 }

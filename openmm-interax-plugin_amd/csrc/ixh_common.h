@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "interax_hip.h"
 
@@ -91,6 +92,56 @@ struct DeviceGuard {
             IXH_HIP_CHECK(hipSetDevice(device));
     }
     ~DeviceGuard() { (void) hipSetDevice(previous); }
+};
+
+// Whether launches of the main kernels are bracketed by events (ixh_set_profiling).
+bool profiling_enabled();
+
+// HIP events around the launches of one kernel, summed on request.
+struct KernelTimer {
+    std::vector<hipEvent_t> pool;   // start, stop, start, stop, ...
+    size_t used = 0;
+    double accumulated = 0;
+    long long launches = 0;
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            IXH_HIP_CHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    // Record a start event; returns false when profiling is off.
+    bool start(hipStream_t stream) {
+        if (!profiling_enabled())
+            return false;
+        if (used+2 > 4096)
+            collect();
+        IXH_HIP_CHECK(hipEventRecord(next(), stream));
+        return true;
+    }
+    void stop(hipStream_t stream) {
+        IXH_HIP_CHECK(hipEventRecord(next(), stream));
+        launches++;
+    }
+    void collect() {
+        for (size_t k = 0; k+1 < used; k += 2) {
+            IXH_HIP_CHECK(hipEventSynchronize(pool[k+1]));
+            float ms = 0;
+            IXH_HIP_CHECK(hipEventElapsedTime(&ms, pool[k], pool[k+1]));
+            accumulated += ms;
+        }
+        used = 0;
+    }
+    void reset() {
+        collect();
+        accumulated = 0;
+        launches = 0;
+    }
+    ~KernelTimer() {
+        for (hipEvent_t e : pool)
+            (void) hipEventDestroy(e);
+    }
 };
 
 constexpr double FIXED_POINT_SCALE = 4294967296.0;   // 2^32 units per kJ/mol/nm, as OpenMM's GPU platforms

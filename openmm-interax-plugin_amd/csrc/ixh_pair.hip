@@ -842,6 +842,7 @@ struct ixh_pair {
     DeviceArray<long long> pairCount;
     int* hostFlag = nullptr;
     hipStream_t lastStream = nullptr;
+    KernelTimer timer;
 };
 
 namespace {
@@ -1233,6 +1234,7 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     a.tileCount = p->tileCount.ptr;
     int blocks = std::max(1, std::min((p->numTiles+3)/4, 4096));
     if (p->numTiles > 0) {
+        bool timed = p->timer.start(stream);
         switch (terms) {
 #define IXH_CASE(T) case T: hipLaunchKernelGGL((k_pair_tiles<Real, T>), dim3(blocks), dim3(256), 0, stream, a); break;
             IXH_CASE(1) IXH_CASE(2) IXH_CASE(3) IXH_CASE(4) IXH_CASE(5) IXH_CASE(6) IXH_CASE(7)
@@ -1241,6 +1243,8 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
                 throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: illegal terms");
         }
         IXH_HIP_CHECK(hipGetLastError());
+        if (timed)
+            p->timer.stop(stream);
     }
     hipLaunchKernelGGL(k_reduce_tiles, dim3(1), dim3(256), 0, stream, p->tileEnergy.ptr, p->tileCount.ptr, p->numTiles, b->energy,
                        p->pairCount.ptr, addEnergy);
@@ -1406,6 +1410,28 @@ int ixh_pair_long_range_energy(ixh_pair* p, const double* box, double lambda, do
             e = (p->lrcCoefficient+lam*p->lrcAlchemical)/(box[0]*box[4]*box[8]);
         }
         *energy = e;
+    });
+}
+
+int ixh_pair_kernel_time(ixh_pair* p, double* milliseconds, long long* launches) {
+    return guarded([&] {
+        if (p == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_kernel_time: null handle");
+        DeviceGuard guard(p->device);
+        p->timer.collect();
+        if (milliseconds != nullptr)
+            *milliseconds = p->timer.accumulated;
+        if (launches != nullptr)
+            *launches = p->timer.launches;
+    });
+}
+
+int ixh_pair_reset_kernel_time(ixh_pair* p) {
+    return guarded([&] {
+        if (p == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_reset_kernel_time: null handle");
+        DeviceGuard guard(p->device);
+        p->timer.reset();
     });
 }
 

@@ -4,6 +4,7 @@
 // precision, float4 posq plus a float4 correction in mixed precision, double4 in double precision.
 #include "ixh_common.h"
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -11,6 +12,11 @@
 namespace ixh {
 
 static thread_local std::string lastError;
+static bool profiling = false;
+
+bool profiling_enabled() {
+    return profiling;
+}
 
 void set_last_error(const std::string& message) {
     lastError = message;
@@ -19,6 +25,18 @@ void set_last_error(const std::string& message) {
 }  // namespace ixh
 
 using namespace ixh;
+
+namespace {
+
+__global__ void k_clear(long long* force, long long n, double* energy) {
+    long long i = (long long) blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        force[i] = 0;
+    if (i == 0)
+        energy[0] = 0;
+}
+
+}  // namespace
 
 struct ixh_state {
     int device = 0;
@@ -50,6 +68,10 @@ int ixh_version(int* major, int* minor) {
         if (minor != nullptr)
             *minor = IXH_VERSION_MINOR;
     });
+}
+
+int ixh_set_profiling(int enable) {
+    return guarded([&] { ixh::profiling = (enable != 0); });
 }
 
 int ixh_device_count(int* count) {
@@ -168,8 +190,9 @@ int ixh_state_clear(ixh_state* s) {
         if (s == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_state_clear: invalid argument");
         DeviceGuard guard(s->device);
-        IXH_HIP_CHECK(hipMemsetAsync(s->force.ptr, 0, sizeof(long long)*3*(size_t) s->stride, s->stream));
-        IXH_HIP_CHECK(hipMemsetAsync(s->energy.ptr, 0, sizeof(double), s->stream));
+        long long n = 3LL*s->stride;
+        hipLaunchKernelGGL(k_clear, dim3((unsigned) std::max(1LL, (n+255)/256)), dim3(256), 0, s->stream, s->force.ptr, n, s->energy.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
     });
 }
 

@@ -431,6 +431,7 @@ struct ixh_threebody {
     DeviceArray<double4> nbrDelta;
     DeviceArray<long long> siteForce, tripleCount;
     hipStream_t lastStream = nullptr;
+    KernelTimer timer;
 };
 
 namespace {
@@ -513,8 +514,11 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     size_t shared = (size_t) t.capacity*(2*sizeof(R4)+3*sizeof(long long)+2*sizeof(int));
     if (shared > 160*1024)
         throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_execute: a site has too many neighbours for one workgroup");
+    bool timed = h->timer.start(stream);
     hipLaunchKernelGGL(k_triples<Real>, dim3(h->numSites), dim3(64), shared, stream, t);
     IXH_HIP_CHECK(hipGetLastError());
+    if (timed)
+        h->timer.stop(stream);
 }
 
 }  // namespace
@@ -641,6 +645,28 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
         hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
                            h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
         IXH_HIP_CHECK(hipGetLastError());
+    });
+}
+
+int ixh_threebody_kernel_time(ixh_threebody* h, double* milliseconds, long long* launches) {
+    return guarded([&] {
+        if (h == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_kernel_time: null handle");
+        DeviceGuard guard(h->device);
+        h->timer.collect();
+        if (milliseconds != nullptr)
+            *milliseconds = h->timer.accumulated;
+        if (launches != nullptr)
+            *launches = h->timer.launches;
+    });
+}
+
+int ixh_threebody_reset_kernel_time(ixh_threebody* h) {
+    return guarded([&] {
+        if (h == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_reset_kernel_time: null handle");
+        DeviceGuard guard(h->device);
+        h->timer.reset();
     });
 }
 
