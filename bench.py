@@ -150,7 +150,7 @@ def bench_pair(args):
         "roofline": {"bound": "mfma", "note": "compute-bound: FP32 VALU (no MFMA on this path) against the MI355X FP32 "
                                               "peak; 240 algorithmic flops per interacting pair",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved/PEAK_FP32_TFLOPS,
-                     "traffic": traffic_from_profiles("k_pair_tiles"), "kernel": "k_pair_tiles",
+                     "traffic": traffic_from_profiles("k_pair_queue"), "kernel": "k_pair_queue",
                      "kernel_avg_us": avg_kernel_s*1e6, "launches": launches.value},
     }
 
@@ -187,19 +187,39 @@ def bench_pair(args):
     return result
 
 
+def copy_calibration(args):
+    """A device-to-device copy of a known size: the HBM counters of its blit kernel calibrate FETCH_SIZE and
+    WRITE_SIZE (bytes per counted unit) for the traffic figures of the other kernels."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    size = 256 << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(size)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(size)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(size)) == 0
+    for _ in range(args.steps):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(size), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"workload": "copy", "bytes_per_copy": size, "copies": args.steps}
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=100)
     parser.add_argument("--warmup", type=int, default=5)
-    parser.add_argument("--workload", default="pair", choices=["pair"])
+    parser.add_argument("--workload", default="pair", choices=["pair", "copy"])
     parser.add_argument("--skip-oracle", action="store_true", help="no accuracy check and no CPU baseline")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
-    result = bench_pair(args)
+    if args.workload == "copy":
+        result = copy_calibration(args)
+    else:
+        result = bench_pair(args)
     print(json.dumps(result), flush=True)
 
 

@@ -1,21 +1,28 @@
-// The fused InteraxFF pair forces on gfx950: neighbour list, exclusion masks, the rotating-tile pair kernel and the
-// dispersion long range correction.
+// The fused InteraxFF pair forces on gfx950: neighbour list, candidate masks, the pair kernels and the dispersion
+// long range correction.
 //
 // Reference behaviour (the contract): ReferenceInteraxPairIxn.h:53-99 (pairs, cutoff, soft core, switching,
 // alchemical scaling), ReferenceInteraxKernels.cpp:41-198 (the three force kernels), InteraxPairForce.cpp:113-146
 // (validation), InteraxDispersionForceImpl.cpp:86-149 (long range correction).
 //
 // Layout in HBM.  Particles are sorted along a Morton curve of their fractional coordinates into blocks of 64 (one
-// wavefront); per sorted slot the kernel reads one 16-byte position in the frame of its block and three 16-byte
-// parameter vectors.  A tile is a pair of blocks (I <= J) whose bounding boxes are within cutoff + skin; it is
-// evaluated by one wavefront as 64 rotation steps: at step s lane l pairs its particle I*64+l with J*64+(l+s)%64,
-// reading j from LDS, so every lane owns a distinct i and a distinct j at every step and the j force accumulators
-// follow their particle around the wave with a DPP rotation.  Excluded pairs, padding slots and the lower triangle
-// of diagonal tiles are removed by a 64x64 bit mask per tile (only tiles that need one have one).
+// wavefront); per sorted slot a kernel reads one 16-byte position in the frame of its block and three 16-byte
+// parameter vectors.  A tile pairs the 64 particles of a block I with 64 partner slots: the block itself (the
+// diagonal tile) or a chunk of the partner list of I.  With a cutoff, the partner list of I holds the particles of
+// blocks J > I that lie within cutoff + skin of the bounding box of I (a particle-box test, which keeps about twice
+// as many useful pairs per tile as a box-box test); without a cutoff it is every block J > I.  Each tile has a
+// 64 x 64 candidate mask, built with the list: bit (i, j) is set when the pair is not excluded, both slots hold
+// particles, and (with a cutoff) the pair was within cutoff + skin.
 //
-// Determinism.  Each tile converts its FP32 force sums to 64-bit fixed point before they are added with integer
-// atomics, and writes its FP64 energy to its own slot; the slots are summed in tile order.  Results therefore do
-// not depend on the launch shape, on which wave ran which tile, or on other systems sharing a launch.
+// Kernels.  Without a cutoff every candidate interacts: one wavefront evaluates a tile in 64 rotation steps (lane l
+// pairs i = l with j = (l+s) % 64, j read from LDS, the j force accumulators following their particle around the
+// wave with a DPP rotation).  With a cutoff each lane walks the set bits of its own mask row, tests the distance and
+// evaluates only the pairs inside the cutoff, so lanes do useful work instead of idling on distant pairs; the forces
+// on j are added in LDS as 64-bit fixed point.
+//
+// Determinism.  Forces are summed per tile in FP32 in a fixed order, converted to 64-bit fixed point and added with
+// integer atomics; each tile writes its FP64 energy to its own slot and the slots are summed in tile order.  Results
+// therefore do not depend on the launch shape, on which wave ran which tile, or on other systems sharing a launch.
 #include "ixh_pair_math.h"
 
 #include <algorithm>
@@ -33,6 +40,9 @@ using namespace ixh;
 namespace {
 
 constexpr int BLOCK = 64;
+// Per-slot flags: the original particle index, the alchemical flag in bit 30, -1 for a padding slot.
+constexpr int INDEX_MASK = 0x3fffffff;
+constexpr int ALCHEMICAL_BIT = 0x40000000;
 constexpr double SKIN = 0.1;   // neighbour list padding (nm); rebuilt when a particle has moved skin/2
 
 template <typename Real>
@@ -289,99 +299,135 @@ __device__ double block_distance2(const double* bounds, int I, int J, int period
     return best;
 }
 
-// Count (pass 0) or write (pass 1) the tiles of each row I: J >= I with box distance below the list cutoff.
-__global__ void k_find_tiles(const double* bounds, int numBlocks, int periodic, BoxD box, double listCutoff2, int* rowCount,
-                             const int* rowStart, int2* tiles) {
+// Smallest distance between a point d (relative to the centre of block I) and the box of I, over the periodic images
+// within one lattice vector of the reduced displacement.
+__device__ double point_box_distance2(double dx, double dy, double dz, const double* h, int periodic, const BoxD& box) {
+    if (!periodic) {
+        double gx = fmax(0.0, fabs(dx)-h[0]), gy = fmax(0.0, fabs(dy)-h[1]), gz = fmax(0.0, fabs(dz)-h[2]);
+        return gx*gx+gy*gy+gz*gz;
+    }
+    reference_min_image(dx, dy, dz, box);
+    double best = 1e300;
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) {
+                double ex = dx+i*box.ax+j*box.bx+k*box.cx, ey = dy+j*box.by+k*box.cy, ez = dz+k*box.cz;
+                double gx = fmax(0.0, fabs(ex)-h[0]), gy = fmax(0.0, fabs(ey)-h[1]), gz = fmax(0.0, fabs(ez)-h[2]);
+                best = fmin(best, gx*gx+gy*gy+gz*gz);
+            }
+    return best;
+}
+
+// Count (pass 0) or write (pass 1) the candidate blocks J > I of each block I: box distance below the list cutoff.
+__global__ void k_find_blocks(const double* bounds, int numBlocks, int periodic, BoxD box, double listCutoff2, int* rowCount,
+                              const int* rowStart, int* blocks) {
     int I = blockIdx.x;
     int lane = threadIdx.x;   // 64 threads
     int written = 0;
-    for (int J0 = I; J0 < numBlocks; J0 += 64) {
+    for (int J0 = I+1; J0 < numBlocks; J0 += 64) {
         int J = J0+lane;
         bool hit = (J < numBlocks) && block_distance2(bounds, I, J, periodic, box) < listCutoff2;
         unsigned long long ballot = __ballot(hit);
-        if (tiles != nullptr && hit) {
-            int pos = written + __popcll(ballot & ((1ULL << lane)-1));
-            tiles[rowStart[I]+pos] = make_int2(I, J);
-        }
+        if (blocks != nullptr && hit)
+            blocks[rowStart[I]+written+__popcll(ballot & ((1ULL << lane)-1))] = J;
         written += __popcll(ballot);
     }
-    if (tiles == nullptr && lane == 0)
+    if (blocks == nullptr && lane == 0)
         rowCount[I] = written;
 }
 
-// Index of tile (I, J) in the row-sorted tile list, or -1.
-__device__ __forceinline__ int find_tile(const int2* tiles, const int* rowStart, int I, int J) {
-    int lo = rowStart[I], hi = rowStart[I+1];
+// Count (pass 0) or write (pass 1) the partners of each block I: the particles of its candidate blocks within the
+// list cutoff of the bounding box of I, in ascending slot order.
+__global__ void k_find_partners(const double* bounds, const double* pos3, const int* perm, const int* blockStart,
+                                const int* blocks, int periodic, BoxD box, double listCutoff2, int* count,
+                                const int* partnerStart, int* partners) {
+    int I = blockIdx.x;
+    int lane = threadIdx.x;   // 64 threads
+    const double c[3] = {bounds[6*I], bounds[6*I+1], bounds[6*I+2]};
+    const double h[3] = {bounds[6*I+3], bounds[6*I+4], bounds[6*I+5]};
+    int written = 0;
+    for (int k = blockStart[I]; k < blockStart[I+1]; k++) {
+        int slot = blocks[k]*BLOCK+lane;
+        int atom = perm[slot];
+        bool hit = false;
+        if (atom >= 0)
+            hit = point_box_distance2(pos3[3*atom]-c[0], pos3[3*atom+1]-c[1], pos3[3*atom+2]-c[2], h, periodic, box) < listCutoff2;
+        unsigned long long ballot = __ballot(hit);
+        if (partners != nullptr && hit)
+            partners[partnerStart[I]+written+__popcll(ballot & ((1ULL << lane)-1))] = slot;
+        written += __popcll(ballot);
+    }
+    if (partners == nullptr && lane == 0)
+        count[I] = written;
+}
+
+__device__ __forceinline__ bool excluded(const int* offsets, const int* partners, int i, int j) {
+    int lo = offsets[i], hi = offsets[i+1];
     while (lo < hi) {
         int mid = (lo+hi) >> 1;
-        int v = tiles[mid].y;
-        if (v == J)
-            return mid;
-        if (v < J)
+        int v = partners[mid];
+        if (v == j)
+            return true;
+        if (v < j)
             lo = mid+1;
         else
             hi = mid;
     }
-    return -1;
+    return false;
 }
 
-// Mark the tiles that need a mask: those holding an excluded pair or padding slots.
-__global__ void k_mark_mask_tiles(const int* exclusions, int numExclusions, const int* slotOf, const int2* tiles,
-                                  const int* rowStart, int numTiles, int paddingBlock, int* needMask) {
-    int e = blockIdx.x*blockDim.x+threadIdx.x;
-    if (e < numExclusions) {
-        int p = exclusions[2*e], q = exclusions[2*e+1];
-        if (p != q) {
-            int bp = slotOf[p]/BLOCK, bq = slotOf[q]/BLOCK;
-            int t = find_tile(tiles, rowStart, min(bp, bq), max(bp, bq));
-            if (t >= 0)
-                needMask[t] = 1;
+// The candidate mask of every tile (one wavefront per tile, lane = i): bit j is set when the pair is not excluded,
+// both slots hold particles, i != j, and (with a cutoff) the pair is within the list cutoff.  Masks of diagonal tiles
+// are symmetric.
+struct MaskArgs {
+    const int2* tiles;
+    const int* partners;     // slot lists; tile.y < 0 marks the diagonal tile
+    const int* perm;
+    const double* pos3;
+    const int* exclusionOffsets;
+    const int* exclusionPartners;
+    int useCutoff;
+    int periodic;
+    BoxD box;
+    double listCutoff2;
+};
+
+__global__ void k_build_masks(MaskArgs a, unsigned long long* masks) {
+    __shared__ int sAtom[BLOCK];
+    __shared__ double sPos[BLOCK][3];
+    int t = blockIdx.x;
+    int lane = threadIdx.x;
+    int2 tile = a.tiles[t];
+    int jslot = (tile.y < 0 ? tile.x*BLOCK+lane : a.partners[tile.y+lane]);
+    int ja = (jslot < 0 ? -1 : a.perm[jslot]);
+    sAtom[lane] = ja;
+    if (ja >= 0) {
+        sPos[lane][0] = a.pos3[3*ja];
+        sPos[lane][1] = a.pos3[3*ja+1];
+        sPos[lane][2] = a.pos3[3*ja+2];
+    }
+    __syncthreads();
+    int ia = a.perm[tile.x*BLOCK+lane];
+    unsigned long long m = 0;
+    if (ia >= 0) {
+        double x = a.pos3[3*ia], y = a.pos3[3*ia+1], z = a.pos3[3*ia+2];
+        for (int b = 0; b < BLOCK; b++) {
+            int other = sAtom[b];
+            if (other < 0 || other == ia)
+                continue;
+            if (excluded(a.exclusionOffsets, a.exclusionPartners, ia, other))
+                continue;
+            if (a.useCutoff) {
+                double dx = sPos[b][0]-x, dy = sPos[b][1]-y, dz = sPos[b][2]-z;
+                if (a.periodic)
+                    reference_min_image(dx, dy, dz, a.box);
+                if (dx*dx+dy*dy+dz*dz >= a.listCutoff2)
+                    continue;
+            }
+            m |= 1ULL << b;
         }
     }
-    if (e < numTiles && paddingBlock >= 0) {
-        int2 tile = tiles[e];
-        if (tile.x == paddingBlock || tile.y == paddingBlock)
-            needMask[e] = 1;
-    }
-}
-
-__global__ void k_init_masks(const int2* tiles, const int* maskIndex, int numTiles, int paddingBlock, int validInLast,
-                             unsigned long long* masks) {
-    int t = blockIdx.x;
-    int row = threadIdx.x;   // 64 threads
-    if (t >= numTiles || maskIndex[t] < 0)
-        return;
-    int2 tile = tiles[t];
-    unsigned long long m = 0;
-    if (tile.y == paddingBlock)
-        m |= ~0ULL << validInLast;
-    if (tile.x == paddingBlock && row >= validInLast)
-        m = ~0ULL;
-    masks[(size_t) maskIndex[t]*BLOCK+row] = m;
-}
-
-__global__ void k_set_exclusion_bits(const int* exclusions, int numExclusions, const int* slotOf, const int2* tiles,
-                                     const int* rowStart, const int* maskIndex, unsigned long long* masks) {
-    int e = blockIdx.x*blockDim.x+threadIdx.x;
-    if (e >= numExclusions)
-        return;
-    int p = exclusions[2*e], q = exclusions[2*e+1];
-    if (p == q)
-        return;
-    int sp = slotOf[p], sq = slotOf[q];
-    if (sp > sq) {
-        int tmp = sp;
-        sp = sq;
-        sq = tmp;
-    }
-    int I = sp/BLOCK, J = sq/BLOCK;
-    int t = find_tile(tiles, rowStart, I, J);
-    if (t < 0)
-        return;
-    unsigned long long* m = masks+(size_t) maskIndex[t]*BLOCK;
-    atomicOr(&m[sp%BLOCK], 1ULL << (sq%BLOCK));
-    if (I == J)
-        atomicOr(&m[sq%BLOCK], 1ULL << (sp%BLOCK));
+    masks[(size_t) t*BLOCK+lane] = m;
 }
 
 // Largest squared displacement since the list was built exceeds (skin/2)^2?
@@ -439,20 +485,21 @@ __global__ void k_local_positions(const void* posq, const float4* correction, in
 template <typename Real>
 struct TileArgs {
     typedef typename Vec4T<Real>::type R4;
-    const int2* tiles;
+    const int2* tiles;       // (block I, offset of the partner chunk, or -1 for the diagonal tile)
     int numTiles;
-    const int* maskIndex;
+    const int* partners;
     const unsigned long long* masks;
     const R4* local;
     const double* centers;
     const R4* pA;    // m1, m2, m3, alpha
     const R4* pB;    // dispersion width, exchange amplitude, exchange width, 1/exchange width
     const R4* pC;    // k, l, short-range width, 1/short-range width
-    const int* flags;   // original index | alchemical << 31, -1 for padding
+    const int* flags;   // original index | alchemical bit, -1 for padding
     int n;
     // geometry
     int method;
     int periodic;
+    int pairImage;   // the minimum image of each pair must be taken (small boxes)
     BoxD box;
     Real boxR[9];   // ax, bx, by, cx, cy, cz, 1/ax, 1/by, 1/cz
     Real cutoff2;
@@ -487,8 +534,83 @@ __device__ __noinline__ bool exact_within_cutoff(const void* posq, const float4*
     return __dsqrt_rn(r2) < cutoff;
 }
 
+// Energy and force of one pair in range (ReferenceInteraxPairIxn.h:71-96): soft core, the pair terms, switching,
+// alchemical scaling.  d = r_j - r_i; returns the force on i as fs*d.
 template <typename Real, int TERMS>
-__global__ void __launch_bounds__(256) k_pair_tiles(TileArgs<Real> a) {
+__device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, Real r2, const PairParticle<Real>& P, const PairParticle<Real>& Q,
+                                              bool alchI, int flagJ, int origI, Real& e, Real& fs) {
+    const bool alchJ = ((flagJ >> 30) & 1) != 0;
+    const bool alchPair = (alchI != alchJ);
+    const Real invR = fast_rsqrt(r2);
+    const Real r = r2*invR;
+    Real rEff = r, invREff = invR;
+    if (a.useSoftcore && alchPair) {
+        Real r6 = r2*r2*r2;
+        rEff = pow(r6+a.softcoreTerm, Real(1.0/6.0));
+        invREff = Real(1)/rEff;
+    }
+    Real dEdR;
+    const bool iFirst = origI < (flagJ & INDEX_MASK);
+    pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
+    if (rEff != r) {
+        Real ratio = r*invREff;
+        Real ratio2 = ratio*ratio;
+        dEdR *= ratio2*ratio2*ratio;
+    }
+    if (a.useSwitch && r > a.switchDistance) {
+        Real x = (r-a.switchDistance)*a.invSwitchWidth;
+        Real sw = 1+x*x*x*(Real(-10)+x*(Real(15)-x*Real(6)));
+        Real dsw = x*x*(Real(-30)+x*(Real(60)-x*Real(30)))*a.invSwitchWidth;
+        dEdR = dEdR*sw + e*dsw;
+        e *= sw;
+    }
+    const Real scale = (alchPair ? a.lambda : Real(1));
+    e *= scale;
+    fs = scale*dEdR*invR;
+}
+
+template <typename Real>
+__device__ __forceinline__ void unpack(PairParticle<Real>& P, const typename Vec4T<Real>::type& A, const typename Vec4T<Real>::type& B,
+                                       const typename Vec4T<Real>::type& C) {
+    P.m1 = A.x; P.m2 = A.y; P.m3 = A.z; P.alpha = A.w;
+    P.wd = B.x; P.amp = B.y; P.wx = B.z; P.iwx = B.w;
+    P.k = C.x; P.l = C.y; P.ws = C.z; P.iws = C.w;
+}
+
+// Load the partner chunk of a tile into LDS, in the frame of block I.
+template <typename Real>
+__device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 tile, int lane, typename Vec4T<Real>::type* sPos,
+                                               typename Vec4T<Real>::type* sA, typename Vec4T<Real>::type* sB,
+                                               typename Vec4T<Real>::type* sC, int* sFlags) {
+    typedef typename Vec4T<Real>::type R4;
+    const int I = tile.x;
+    const int slot = (tile.y < 0 ? I*BLOCK+lane : a.partners[tile.y+lane]);
+    if (slot < 0) {
+        R4 zero = {0, 0, 0, 0};
+        sPos[lane] = zero;
+        sFlags[lane] = -1;
+        return;
+    }
+    const int J = slot/BLOCK;
+    R4 pj = a.local[slot];
+    if (J != I) {
+        double sx = a.centers[3*J]-a.centers[3*I], sy = a.centers[3*J+1]-a.centers[3*I+1], sz = a.centers[3*J+2]-a.centers[3*I+2];
+        if (a.periodic)
+            reference_min_image(sx, sy, sz, a.box);
+        pj.x += Real(sx);
+        pj.y += Real(sy);
+        pj.z += Real(sz);
+    }
+    sPos[lane] = pj;
+    sA[lane] = a.pA[slot];
+    sB[lane] = a.pB[slot];
+    sC[lane] = a.pC[slot];
+    sFlags[lane] = a.flags[slot];
+}
+
+// Tiles without a cutoff: every candidate interacts.  64 rotation steps, lane l paired with j = (l+s) % 64.
+template <typename Real, int TERMS>
+__global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
     __shared__ R4 sPos[4][BLOCK], sA[4][BLOCK], sB[4][BLOCK], sC[4][BLOCK];
     __shared__ int sFlags[4][BLOCK];
@@ -500,17 +622,17 @@ __global__ void __launch_bounds__(256) k_pair_tiles(TileArgs<Real> a) {
     const long long begin = T*wave/numWaves, end = T*(wave+1)/numWaves;
 
     int curI = -1;
-    R4 pi = {0, 0, 0, 0}, iA = {0, 0, 0, 0}, iB = {0, 0, 0, 0}, iC = {0, 0, 0, 0};
+    R4 pi = {0, 0, 0, 0};
     int flagI = -1;
     long long fixI[3] = {0, 0, 0};
     PairParticle<Real> P, Q;
 
     for (long long t = begin; t < end; t++) {
         const int2 tile = a.tiles[t];
-        const int I = tile.x, J = tile.y;
+        const int I = tile.x;
         if (I != curI) {
             if (curI >= 0 && flagI >= 0) {
-                int oi = flagI & 0x7fffffff;
+                int oi = flagI & INDEX_MASK;
                 add_fixed(&a.force[oi], fixI[0]);
                 add_fixed(&a.force[a.stride+oi], fixI[1]);
                 add_fixed(&a.force[2*a.stride+oi], fixI[2]);
@@ -519,98 +641,34 @@ __global__ void __launch_bounds__(256) k_pair_tiles(TileArgs<Real> a) {
             curI = I;
             int slot = I*BLOCK+lane;
             pi = a.local[slot];
-            iA = a.pA[slot];
-            iB = a.pB[slot];
-            iC = a.pC[slot];
+            unpack<Real>(P, a.pA[slot], a.pB[slot], a.pC[slot]);
             flagI = a.flags[slot];
-            P.m1 = iA.x; P.m2 = iA.y; P.m3 = iA.z; P.alpha = iA.w;
-            P.wd = iB.x; P.amp = iB.y; P.wx = iB.z; P.iwx = iB.w;
-            P.k = iC.x; P.l = iC.y; P.ws = iC.z; P.iws = iC.w;
         }
-        // shift of block J into the frame of block I
-        double sx = a.centers[3*J]-a.centers[3*I], sy = a.centers[3*J+1]-a.centers[3*I+1], sz = a.centers[3*J+2]-a.centers[3*I+2];
-        if (a.periodic)
-            reference_min_image(sx, sy, sz, a.box);
-        {
-            int slot = J*BLOCK+lane;
-            R4 pj = a.local[slot];
-            pj.x += Real(sx);
-            pj.y += Real(sy);
-            pj.z += Real(sz);
-            sPos[w][lane] = pj;
-            sA[w][lane] = a.pA[slot];
-            sB[w][lane] = a.pB[slot];
-            sC[w][lane] = a.pC[slot];
-            sFlags[w][lane] = a.flags[slot];
-        }
+        stage_partners<Real>(a, tile, lane, sPos[w], sA[w], sB[w], sC[w], sFlags[w]);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int mi = a.maskIndex[t];
-        const unsigned long long mrow = (mi >= 0 ? a.masks[(size_t) mi*BLOCK+lane] : 0ULL);
-        const bool diag = (I == J);
+        const unsigned long long mrow = a.masks[(size_t) t*BLOCK+lane];
+        const bool diag = (tile.y < 0);
         const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
-        const bool alchI = (flagI < 0 ? false : (flagI >> 31) != 0);
-        const int origI = flagI & 0x7fffffff;
+        const bool alchI = (flagI >= 0 && ((flagI >> 30) & 1) != 0);
+        const int origI = flagI & INDEX_MASK;
         Real fi[3] = {0, 0, 0};
         Real fj[3] = {0, 0, 0};
         double energy = 0;
         int count = 0;
         for (int s = sBegin; s < sEnd; s++) {
             const int jl = (lane+s) & 63;
-            bool skip = ((mrow >> jl) & 1ULL) != 0;
+            bool interact = ((mrow >> jl) & 1ULL) != 0;
             if (diag && s == 32 && lane >= 32)
-                skip = true;
-            const R4 pj = sPos[w][jl];
-            Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
-            if (a.periodic)
-                min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
-                                a.boxR[7], a.boxR[8]);
-            const Real r2 = dx*dx+dy*dy+dz*dz;
-            bool interact = !skip;
-            if (a.method != IXH_NO_CUTOFF && interact) {
-                Real margin = a.band*a.cutoff2;
-                if (r2 >= a.cutoff2+margin)
-                    interact = false;
-                else if (r2 > a.cutoff2-margin) {
-                    int fl = sFlags[w][jl];
-                    interact = exact_within_cutoff(a.posq, a.correction, a.precision, a.periodic, a.box, a.cutoff, origI,
-                                                   fl & 0x7fffffff);
-                }
-            }
+                interact = false;
             if (interact) {
+                const R4 pj = sPos[w][jl];
+                const Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
+                const Real r2 = dx*dx+dy*dy+dz*dz;
                 const int flagJ = sFlags[w][jl];
-                const R4 jA = sA[w][jl], jB = sB[w][jl], jC = sC[w][jl];
-                Q.m1 = jA.x; Q.m2 = jA.y; Q.m3 = jA.z; Q.alpha = jA.w;
-                Q.wd = jB.x; Q.amp = jB.y; Q.wx = jB.z; Q.iwx = jB.w;
-                Q.k = jC.x; Q.l = jC.y; Q.ws = jC.z; Q.iws = jC.w;
-                const bool alchJ = (flagJ >> 31) != 0;
-                const bool alchPair = (alchI != alchJ);
-                const Real invR = fast_rsqrt(r2);
-                const Real r = r2*invR;
-                Real rEff = r, invREff = invR;
-                if (a.useSoftcore && alchPair) {
-                    // ReferenceInteraxPairIxn.h:72-77
-                    Real r6 = r2*r2*r2;
-                    rEff = pow(r6+a.softcoreTerm, Real(1.0/6.0));
-                    invREff = Real(1)/rEff;
-                }
-                Real e, dEdR;
-                const bool iFirst = origI < (flagJ & 0x7fffffff);
-                pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
-                if (rEff != r) {
-                    Real ratio = r*invREff;
-                    Real ratio2 = ratio*ratio;
-                    dEdR *= ratio2*ratio2*ratio;
-                }
-                if (a.useSwitch && r > a.switchDistance) {
-                    Real x = (r-a.switchDistance)*a.invSwitchWidth;
-                    Real sw = 1+x*x*x*(Real(-10)+x*(Real(15)-x*Real(6)));
-                    Real dsw = x*x*(Real(-30)+x*(Real(60)-x*Real(30)))*a.invSwitchWidth;
-                    dEdR = dEdR*sw + e*dsw;
-                    e *= sw;
-                }
-                const Real scale = (alchPair ? a.lambda : Real(1));
-                energy += (double) (scale*e);
-                const Real fs = scale*dEdR*invR;
+                unpack<Real>(Q, sA[w][jl], sB[w][jl], sC[w][jl]);
+                Real e, fs;
+                evaluate_pair<Real, TERMS>(a, r2, P, Q, alchI, flagJ, origI, e, fs);
+                energy += (double) e;
                 fi[0] += dx*fs;
                 fi[1] += dy*fs;
                 fi[2] += dz*fs;
@@ -628,7 +686,7 @@ __global__ void __launch_bounds__(256) k_pair_tiles(TileArgs<Real> a) {
             const int jl = (lane+sEnd) & 63;
             const int flagJ = sFlags[w][jl];
             if (flagJ >= 0) {
-                int oj = flagJ & 0x7fffffff;
+                int oj = flagJ & INDEX_MASK;
                 add_fixed(&a.force[oj], to_fixed(fj[0]));
                 add_fixed(&a.force[a.stride+oj], to_fixed(fj[1]));
                 add_fixed(&a.force[2*a.stride+oj], to_fixed(fj[2]));
@@ -646,10 +704,162 @@ __global__ void __launch_bounds__(256) k_pair_tiles(TileArgs<Real> a) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     if (curI >= 0 && flagI >= 0) {
-        int oi = flagI & 0x7fffffff;
+        int oi = flagI & INDEX_MASK;
         add_fixed(&a.force[oi], fixI[0]);
         add_fixed(&a.force[a.stride+oi], fixI[1]);
         add_fixed(&a.force[2*a.stride+oi], fixI[2]);
+    }
+}
+
+// Tiles with a cutoff.  A lockstep filter pass over the 64 rotation steps pushes the pairs inside the cutoff into a
+// per-wave ring queue in LDS (ballot + rank, so the order is deterministic); every full batch of 64 queued pairs is
+// then evaluated with all lanes busy, whatever the spread of neighbour counts between the particles of the tile.
+// The forces on i and j are added in LDS as 64-bit fixed point and flushed to the force buffer per tile (j) and per
+// block (i).
+constexpr int QUEUE = 256;
+
+template <typename Real>
+struct QueueLds {
+    typedef typename Vec4T<Real>::type R4;
+    R4 iPos[BLOCK], iA[BLOCK], iB[BLOCK], iC[BLOCK];
+    R4 jPos[BLOCK], jA[BLOCK], jB[BLOCK], jC[BLOCK];
+    int iFlags[BLOCK], jFlags[BLOCK];
+    unsigned long long fI[3][BLOCK], fJ[3][BLOCK];
+    unsigned short queue[QUEUE];
+};
+
+template <typename Real, int TERMS>
+__device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, QueueLds<Real>& q, int head, int valid, int lane,
+                                               double& energy, int& pairs) {
+    typedef typename Vec4T<Real>::type R4;
+    if (lane >= valid)
+        return;
+    const unsigned e = q.queue[(head+lane) & (QUEUE-1)];
+    const int il = (int) (e >> 6), jl = (int) (e & 63);
+    const R4 pi = q.iPos[il], pj = q.jPos[jl];
+    Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
+    if (a.pairImage)
+        min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6], a.boxR[7],
+                        a.boxR[8]);
+    const Real r2 = dx*dx+dy*dy+dz*dz;
+    PairParticle<Real> P, Q;
+    unpack<Real>(P, q.iA[il], q.iB[il], q.iC[il]);
+    unpack<Real>(Q, q.jA[jl], q.jB[jl], q.jC[jl]);
+    const int flagI = q.iFlags[il], flagJ = q.jFlags[jl];
+    Real en, fs;
+    evaluate_pair<Real, TERMS>(a, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
+    energy += (double) en;
+    pairs++;
+    const Real fx = dx*fs, fy = dy*fs, fz = dz*fs;
+    atomicAdd(&q.fI[0][il], (unsigned long long) to_fixed(fx));
+    atomicAdd(&q.fI[1][il], (unsigned long long) to_fixed(fy));
+    atomicAdd(&q.fI[2][il], (unsigned long long) to_fixed(fz));
+    atomicAdd(&q.fJ[0][jl], (unsigned long long) to_fixed(-fx));
+    atomicAdd(&q.fJ[1][jl], (unsigned long long) to_fixed(-fy));
+    atomicAdd(&q.fJ[2][jl], (unsigned long long) to_fixed(-fz));
+}
+
+template <typename Real, int TERMS>
+__global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
+    typedef typename Vec4T<Real>::type R4;
+    __shared__ QueueLds<Real> lds[4];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    QueueLds<Real>& q = lds[w];
+    const long long wave = (long long) blockIdx.x*4+w;
+    const long long numWaves = (long long) gridDim.x*4;
+    const long long T = a.numTiles;
+    const long long begin = T*wave/numWaves, end = T*(wave+1)/numWaves;
+    const Real margin = a.band*a.cutoff2;
+    const Real inner = a.cutoff2-margin, outer = a.cutoff2+margin;
+
+    int curI = -1;
+    R4 pi = {0, 0, 0, 0};
+    int flagI = -1;
+    for (long long t = begin; t < end; t++) {
+        const int2 tile = a.tiles[t];
+        const int I = tile.x;
+        if (I != curI) {
+            if (curI >= 0 && flagI >= 0) {
+                int oi = flagI & INDEX_MASK;
+                add_fixed(&a.force[oi], (long long) q.fI[0][lane]);
+                add_fixed(&a.force[a.stride+oi], (long long) q.fI[1][lane]);
+                add_fixed(&a.force[2*a.stride+oi], (long long) q.fI[2][lane]);
+            }
+            curI = I;
+            const int slot = I*BLOCK+lane;
+            pi = a.local[slot];
+            flagI = a.flags[slot];
+            q.iPos[lane] = pi;
+            q.iA[lane] = a.pA[slot];
+            q.iB[lane] = a.pB[slot];
+            q.iC[lane] = a.pC[slot];
+            q.iFlags[lane] = flagI;
+            q.fI[0][lane] = q.fI[1][lane] = q.fI[2][lane] = 0;
+        }
+        stage_partners<Real>(a, tile, lane, q.jPos, q.jA, q.jB, q.jC, q.jFlags);
+        q.fJ[0][lane] = q.fJ[1][lane] = q.fJ[2][lane] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const unsigned long long mrow = a.masks[(size_t) t*BLOCK+lane];
+        const bool diag = (tile.y < 0);
+        const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
+        int head = 0, count = 0;
+        double energy = 0;
+        int pairs = 0;
+        for (int s = sBegin; s < sEnd; s++) {
+            const int jl = (lane+s) & 63;
+            bool in = ((mrow >> jl) & 1ULL) != 0 && !(diag && s == 32 && lane >= 32);
+            if (in) {
+                const R4 pj = q.jPos[jl];
+                Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
+                if (a.pairImage)
+                    min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
+                                    a.boxR[7], a.boxR[8]);
+                const Real r2 = dx*dx+dy*dy+dz*dz;
+                if (r2 >= inner)
+                    in = (r2 < outer) && exact_within_cutoff(a.posq, a.correction, a.precision, a.periodic, a.box, a.cutoff,
+                                                             flagI & INDEX_MASK, q.jFlags[jl] & INDEX_MASK);
+            }
+            const unsigned long long ballot = __ballot(in);
+            if (in) {
+                const int rank = __popcll(ballot & ((1ULL << lane)-1));
+                q.queue[(head+count+rank) & (QUEUE-1)] = (unsigned short) ((lane << 6) | jl);
+            }
+            count += __popcll(ballot);
+            if (count >= BLOCK) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                evaluate_batch<Real, TERMS>(a, q, head, BLOCK, lane, energy, pairs);
+                head = (head+BLOCK) & (QUEUE-1);
+                count -= BLOCK;
+            }
+        }
+        if (count > 0) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            evaluate_batch<Real, TERMS>(a, q, head, count, lane, energy, pairs);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        {
+            const int flagJ = q.jFlags[lane];
+            if (flagJ >= 0) {
+                int oj = flagJ & INDEX_MASK;
+                add_fixed(&a.force[oj], (long long) q.fJ[0][lane]);
+                add_fixed(&a.force[a.stride+oj], (long long) q.fJ[1][lane]);
+                add_fixed(&a.force[2*a.stride+oj], (long long) q.fJ[2][lane]);
+            }
+        }
+        double tileEnergy = wave_sum(energy);
+        int tileCount = (int) wave_sum((double) pairs);
+        if (lane == 0) {
+            a.tileEnergy[t] = tileEnergy;
+            a.tileCount[t] = tileCount;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (curI >= 0 && flagI >= 0) {
+        int oi = flagI & INDEX_MASK;
+        add_fixed(&a.force[oi], (long long) q.fI[0][lane]);
+        add_fixed(&a.force[a.stride+oi], (long long) q.fI[1][lane]);
+        add_fixed(&a.force[2*a.stride+oi], (long long) q.fI[2][lane]);
     }
 }
 
@@ -828,10 +1038,11 @@ struct ixh_pair {
     int numBlocks = 0, padded = 0;
     bool built = false;
     int numTiles = 0;
-    std::vector<int> hostRowStart;
+    int pairImage = 0;
     // parameters in particle order and in slot order
     DeviceArray<char> paramA, paramB, paramC, sortedA, sortedB, sortedC, local;
-    DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionsDev, rowStart, rowCount, maskIndex, needMask, tileCount;
+    DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionOffsets, exclusionPartners, blockStart, blockCount, blockList,
+        partnerCount, partnerStart, partners, tileCount;
     DeviceArray<unsigned> keys, keysOut;
     DeviceArray<int> values;
     DeviceArray<double> pos3, buildPos, bounds, centers, tileEnergy, bbox;
@@ -842,6 +1053,7 @@ struct ixh_pair {
     DeviceArray<long long> pairCount;
     int* hostFlag = nullptr;
     hipStream_t lastStream = nullptr;
+    hipEvent_t done = nullptr;   // recorded at the end of every execute: calls on other streams wait for it
     KernelTimer timer;
 };
 
@@ -850,8 +1062,8 @@ namespace {
 void check_desc(const ixh_pair_desc* d) {
     if (d == nullptr)
         throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: null descriptor");
-    if (d->num_particles < 0)
-        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: negative number of particles");
+    if (d->num_particles < 0 || d->num_particles > INDEX_MASK)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: illegal number of particles");
     if (d->terms <= 0 || d->terms > 7)
         throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: no pair terms selected");
     if (d->precision != IXH_SINGLE && d->precision != IXH_MIXED && d->precision != IXH_DOUBLE)
@@ -962,7 +1174,7 @@ void load_parameters(ixh_pair* p, const ixh_pair_desc* d, hipStream_t stream) {
             C[4*i+2] = p->shortRange[3*i+2];
             C[4*i+3] = (p->shortRange[3*i+2] != 0 ? 1.0/p->shortRange[3*i+2] : 0.0);
         }
-        flags[i] = i | (p->alchemical[i] ? (int) 0x80000000u : 0);
+        flags[i] = i | (p->alchemical[i] ? ALCHEMICAL_BIT : 0);
     }
     if (p->precision == IXH_DOUBLE) {
         p->paramA.upload((const char*) A.data(), sizeof(double)*A.size(), stream);
@@ -1035,34 +1247,33 @@ void compute_long_range(ixh_pair* p, hipStream_t stream) {
     p->lrcAlchemical = normalization*alch;
 }
 
-// Build the slot order, the tiles and the masks from the current positions.
+// Build the slot order, the partner lists, the tiles and the candidate masks from the current positions.
 void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     int n = p->n;
     bool periodic = (p->method == IXH_CUTOFF_PERIODIC);
+    bool cutoff = (p->method != IXH_NO_CUTOFF);
     BoxD box = make_box(b->box);
     const float4* corr = (const float4*) b->posq_correction;
+    int Nb = p->numBlocks;
     p->pos3.reserve(3*(size_t) n);
     p->perm.reserve(p->padded);
     p->slotOf.reserve(n);
-    if (p->method == IXH_NO_CUTOFF) {
-        // all pairs: the particle order is kept
+    std::vector<int2> tiles;
+    if (!cutoff) {
+        // all pairs: the particle order is kept, every block J > I is a partner chunk
         std::vector<int> perm(p->padded);
         for (int s = 0; s < p->padded; s++)
             perm[s] = (s < n ? s : -1);
         p->perm.upload(perm.data(), perm.size(), stream);
-        std::vector<int> slotOf(n);
-        std::iota(slotOf.begin(), slotOf.end(), 0);
-        p->slotOf.upload(slotOf.data(), slotOf.size(), stream);
-        std::vector<int2> tiles;
-        p->hostRowStart.assign(p->numBlocks+1, 0);
-        for (int I = 0; I < p->numBlocks; I++) {
-            p->hostRowStart[I] = (int) tiles.size();
-            for (int J = I; J < p->numBlocks; J++)
-                tiles.push_back(make_int2(I, J));
+        std::vector<int> partners(p->padded);
+        std::iota(partners.begin(), partners.end(), 0);
+        p->partners.upload(partners.data(), partners.size(), stream);
+        for (int I = 0; I < Nb; I++) {
+            tiles.push_back(make_int2(I, -1));
+            for (int J = I+1; J < Nb; J++)
+                tiles.push_back(make_int2(I, J*BLOCK));
         }
-        p->hostRowStart[p->numBlocks] = (int) tiles.size();
-        p->numTiles = (int) tiles.size();
-        p->tiles.upload(tiles.data(), tiles.size(), stream);
+        p->pairImage = 0;
     }
     else {
         KeyArgs k;
@@ -1114,71 +1325,86 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
                            (float4*) p->sortedA.ptr, (float4*) p->sortedB.ptr, (float4*) p->sortedC.ptr, p->sortedFlags.ptr);
     IXH_HIP_CHECK(hipGetLastError());
 
-    if (p->method != IXH_NO_CUTOFF) {
-        // tiles from block bounding boxes
-        p->bounds.reserve(6*(size_t) p->numBlocks);
+    double listCutoff = p->cutoff+SKIN;
+    if (cutoff) {
+        // candidate blocks from bounding boxes, then candidate particles against the box of each block
+        p->bounds.reserve(6*(size_t) Nb);
         BlockBoundsArgs bb;
         bb.pos3 = p->pos3.ptr;
         bb.perm = p->perm.ptr;
-        bb.numBlocks = p->numBlocks;
+        bb.numBlocks = Nb;
         bb.periodic = periodic;
         bb.box = box;
-        hipLaunchKernelGGL(k_block_bounds, dim3(grid_for(p->numBlocks, 64)), dim3(64), 0, stream, bb, p->bounds.ptr);
+        hipLaunchKernelGGL(k_block_bounds, dim3(grid_for(Nb, 64)), dim3(64), 0, stream, bb, p->bounds.ptr);
         IXH_HIP_CHECK(hipGetLastError());
-        double listCutoff = p->cutoff+SKIN;
-        p->rowCount.reserve(p->numBlocks);
-        hipLaunchKernelGGL(k_find_tiles, dim3(p->numBlocks), dim3(64), 0, stream, p->bounds.ptr, p->numBlocks, (int) periodic, box,
-                           listCutoff*listCutoff, p->rowCount.ptr, (const int*) nullptr, (int2*) nullptr);
+        p->blockCount.reserve(Nb);
+        hipLaunchKernelGGL(k_find_blocks, dim3(Nb), dim3(64), 0, stream, p->bounds.ptr, Nb, (int) periodic, box, listCutoff*listCutoff,
+                           p->blockCount.ptr, (const int*) nullptr, (int*) nullptr);
         IXH_HIP_CHECK(hipGetLastError());
-        std::vector<int> counts(p->numBlocks);
-        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), p->rowCount.ptr, sizeof(int)*counts.size(), hipMemcpyDeviceToHost, stream));
+        std::vector<int> counts(Nb), start(Nb+1, 0);
+        std::vector<double> bounds(6*(size_t) Nb);
+        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), p->blockCount.ptr, sizeof(int)*Nb, hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipMemcpyAsync(bounds.data(), p->bounds.ptr, sizeof(double)*bounds.size(), hipMemcpyDeviceToHost, stream));
         IXH_HIP_CHECK(hipStreamSynchronize(stream));
-        p->hostRowStart.assign(p->numBlocks+1, 0);
-        for (int I = 0; I < p->numBlocks; I++)
-            p->hostRowStart[I+1] = p->hostRowStart[I]+counts[I];
-        p->numTiles = p->hostRowStart[p->numBlocks];
-        p->rowStart.upload(p->hostRowStart.data(), p->hostRowStart.size(), stream);
-        p->tiles.reserve(p->numTiles);
-        hipLaunchKernelGGL(k_find_tiles, dim3(p->numBlocks), dim3(64), 0, stream, p->bounds.ptr, p->numBlocks, (int) periodic, box,
-                           listCutoff*listCutoff, p->rowCount.ptr, (const int*) p->rowStart.ptr, p->tiles.ptr);
+        for (int I = 0; I < Nb; I++)
+            start[I+1] = start[I]+counts[I];
+        p->blockStart.upload(start.data(), start.size(), stream);
+        p->blockList.reserve(std::max(start[Nb], 1));
+        hipLaunchKernelGGL(k_find_blocks, dim3(Nb), dim3(64), 0, stream, p->bounds.ptr, Nb, (int) periodic, box, listCutoff*listCutoff,
+                           p->blockCount.ptr, (const int*) p->blockStart.ptr, p->blockList.ptr);
         IXH_HIP_CHECK(hipGetLastError());
+        p->partnerCount.reserve(Nb);
+        hipLaunchKernelGGL(k_find_partners, dim3(Nb), dim3(64), 0, stream, p->bounds.ptr, p->pos3.ptr, p->perm.ptr, p->blockStart.ptr,
+                           p->blockList.ptr, (int) periodic, box, listCutoff*listCutoff, p->partnerCount.ptr, (const int*) nullptr,
+                           (int*) nullptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), p->partnerCount.ptr, sizeof(int)*Nb, hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        std::vector<int> pstart(Nb+1, 0);
+        for (int I = 0; I < Nb; I++)
+            pstart[I+1] = pstart[I]+(counts[I]+BLOCK-1)/BLOCK*BLOCK;
+        p->partnerStart.upload(pstart.data(), pstart.size(), stream);
+        p->partners.reserve(std::max(pstart[Nb], 1));
+        IXH_HIP_CHECK(hipMemsetAsync(p->partners.ptr, 0xff, sizeof(int)*(size_t) std::max(pstart[Nb], 1), stream));
+        hipLaunchKernelGGL(k_find_partners, dim3(Nb), dim3(64), 0, stream, p->bounds.ptr, p->pos3.ptr, p->perm.ptr, p->blockStart.ptr,
+                           p->blockList.ptr, (int) periodic, box, listCutoff*listCutoff, p->partnerCount.ptr,
+                           (const int*) p->partnerStart.ptr, p->partners.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        for (int I = 0; I < Nb; I++) {
+            tiles.push_back(make_int2(I, -1));
+            for (int off = pstart[I]; off < pstart[I+1]; off += BLOCK)
+                tiles.push_back(make_int2(I, off));
+        }
+        // Pairs need their own minimum image unless every pair in range is the nearest image of its block shift:
+        // cutoff + the block extents (which grow by at most the skin before a rebuild) below half the box.
+        double extent = 0;
+        for (int I = 0; I < Nb; I++)
+            extent = std::max(extent, 2*std::sqrt(bounds[6*I+3]*bounds[6*I+3]+bounds[6*I+4]*bounds[6*I+4]+bounds[6*I+5]*bounds[6*I+5]));
+        double width = std::min(b->box[0], std::min(b->box[4], b->box[8]));
+        p->pairImage = (periodic && p->cutoff+2*(extent+SKIN) >= 0.5*width) ? 1 : 0;
     }
-    else
-        p->rowStart.upload(p->hostRowStart.data(), p->hostRowStart.size(), stream);
-
-    // masks
+    p->numTiles = (int) tiles.size();
+    p->tiles.upload(tiles.data(), tiles.size(), stream);
     int T = p->numTiles;
-    int paddingBlock = (n%BLOCK != 0 ? p->numBlocks-1 : -1);
-    int validInLast = n-(p->numBlocks-1)*BLOCK;
-    int numExclusions = (int) (p->exclusions.size()/2);
-    p->needMask.reserve(T);
-    p->maskIndex.reserve(T);
-    IXH_HIP_CHECK(hipMemsetAsync(p->needMask.ptr, 0, sizeof(int)*(size_t) std::max(T, 1), stream));
-    int markCount = std::max(numExclusions, T);
-    hipLaunchKernelGGL(k_mark_mask_tiles, dim3(grid_for(markCount, 256)), dim3(256), 0, stream, p->exclusionsDev.ptr, numExclusions,
-                       p->slotOf.ptr, p->tiles.ptr, p->rowStart.ptr, T, paddingBlock, p->needMask.ptr);
-    IXH_HIP_CHECK(hipGetLastError());
-    std::vector<int> need(T);
-    IXH_HIP_CHECK(hipMemcpyAsync(need.data(), p->needMask.ptr, sizeof(int)*(size_t) T, hipMemcpyDeviceToHost, stream));
-    IXH_HIP_CHECK(hipStreamSynchronize(stream));
-    std::vector<int> index(T);
-    int numMasks = 0;
-    for (int t = 0; t < T; t++)
-        index[t] = (need[t] ? numMasks++ : -1);
-    p->maskIndex.upload(index.data(), index.size(), stream);
-    p->masks.reserve((size_t) std::max(numMasks, 1)*BLOCK);
-    if (numMasks > 0) {
-        hipLaunchKernelGGL(k_init_masks, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->maskIndex.ptr, T, paddingBlock, validInLast,
-                           p->masks.ptr);
-        IXH_HIP_CHECK(hipGetLastError());
-        if (numExclusions > 0)
-            hipLaunchKernelGGL(k_set_exclusion_bits, dim3(grid_for(numExclusions, 256)), dim3(256), 0, stream, p->exclusionsDev.ptr,
-                               numExclusions, p->slotOf.ptr, p->tiles.ptr, p->rowStart.ptr, p->maskIndex.ptr, p->masks.ptr);
+    p->masks.reserve((size_t) std::max(T, 1)*BLOCK);
+    MaskArgs m;
+    m.tiles = p->tiles.ptr;
+    m.partners = p->partners.ptr;
+    m.perm = p->perm.ptr;
+    m.pos3 = p->pos3.ptr;
+    m.exclusionOffsets = p->exclusionOffsets.ptr;
+    m.exclusionPartners = p->exclusionPartners.ptr;
+    m.useCutoff = cutoff;
+    m.periodic = periodic;
+    m.box = box;
+    m.listCutoff2 = listCutoff*listCutoff;
+    if (T > 0) {
+        hipLaunchKernelGGL(k_build_masks, dim3(T), dim3(BLOCK), 0, stream, m, p->masks.ptr);
         IXH_HIP_CHECK(hipGetLastError());
     }
     p->tileEnergy.reserve(std::max(T, 1));
     p->tileCount.reserve(std::max(T, 1));
-    if (p->method != IXH_NO_CUTOFF) {
+    if (cutoff) {
         p->buildPos.reserve(3*(size_t) n);
         IXH_HIP_CHECK(hipMemcpyAsync(p->buildPos.ptr, p->pos3.ptr, sizeof(double)*3*(size_t) n, hipMemcpyDeviceToDevice, stream));
     }
@@ -1192,7 +1418,7 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     TileArgs<Real> a;
     a.tiles = p->tiles.ptr;
     a.numTiles = p->numTiles;
-    a.maskIndex = p->maskIndex.ptr;
+    a.partners = p->partners.ptr;
     a.masks = p->masks.ptr;
     a.local = (const R4*) p->local.ptr;
     a.centers = p->centers.ptr;
@@ -1203,6 +1429,7 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     a.n = p->n;
     a.method = p->method;
     a.periodic = (p->method == IXH_CUTOFF_PERIODIC);
+    a.pairImage = p->pairImage;
     a.box = make_box(b->box);
     a.boxR[0] = Real(b->box[0]);
     a.boxR[1] = Real(b->box[3]);
@@ -1235,8 +1462,14 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     int blocks = std::max(1, std::min((p->numTiles+3)/4, 4096));
     if (p->numTiles > 0) {
         bool timed = p->timer.start(stream);
+        bool compact = (p->method != IXH_NO_CUTOFF);
         switch (terms) {
-#define IXH_CASE(T) case T: hipLaunchKernelGGL((k_pair_tiles<Real, T>), dim3(blocks), dim3(256), 0, stream, a); break;
+#define IXH_CASE(T) case T: \
+            if (compact) \
+                hipLaunchKernelGGL((k_pair_queue<Real, T>), dim3(blocks), dim3(256), 0, stream, a); \
+            else \
+                hipLaunchKernelGGL((k_pair_rotate<Real, T>), dim3(blocks), dim3(256), 0, stream, a); \
+            break;
             IXH_CASE(1) IXH_CASE(2) IXH_CASE(3) IXH_CASE(4) IXH_CASE(5) IXH_CASE(6) IXH_CASE(7)
 #undef IXH_CASE
             default:
@@ -1281,10 +1514,28 @@ int ixh_pair_create(int device, const ixh_pair_desc* d, ixh_pair** out) {
                     p->alchemical[i] = (d->alchemical[i] != 0);
             p->anyAlchemical = std::any_of(p->alchemical.begin(), p->alchemical.end(), [](unsigned char c) { return c != 0; });
             p->exclusions.assign(d->exclusions, d->exclusions+2*(size_t) d->num_exclusions);
+            {
+                // per-particle sorted exclusion lists (the sets of ReferenceInteraxPairIxn.cpp:43-49)
+                std::vector<std::vector<int>> lists(p->n);
+                for (int e = 0; e < d->num_exclusions; e++) {
+                    int p1 = d->exclusions[2*e], p2 = d->exclusions[2*e+1];
+                    lists[p1].push_back(p2);
+                    lists[p2].push_back(p1);
+                }
+                std::vector<int> offsets(p->n+1, 0), partners;
+                for (int i = 0; i < p->n; i++) {
+                    std::sort(lists[i].begin(), lists[i].end());
+                    lists[i].erase(std::unique(lists[i].begin(), lists[i].end()), lists[i].end());
+                    partners.insert(partners.end(), lists[i].begin(), lists[i].end());
+                    offsets[i+1] = (int) partners.size();
+                }
+                p->exclusionOffsets.upload(offsets.data(), offsets.size(), nullptr);
+                p->exclusionPartners.upload(partners.data(), std::max<size_t>(partners.size(), 0), nullptr);
+                p->exclusionPartners.reserve(std::max<size_t>(partners.size(), 1));
+            }
             p->numBlocks = std::max(1, (p->n+BLOCK-1)/BLOCK);
             p->padded = p->numBlocks*BLOCK;
             hipStream_t stream = nullptr;
-            p->exclusionsDev.upload(p->exclusions.data(), p->exclusions.size(), stream);
             p->flagDev.reserve(1);
             p->pairCount.reserve(1);
             IXH_HIP_CHECK(hipHostMalloc((void**) &p->hostFlag, sizeof(int), hipHostMallocDefault));
@@ -1309,6 +1560,8 @@ int ixh_pair_destroy(ixh_pair* p) {
         (void) hipDeviceSynchronize();
         if (p->hostFlag != nullptr)
             (void) hipHostFree(p->hostFlag);
+        if (p->done != nullptr)
+            (void) hipEventDestroy(p->done);
         delete p;
     });
 }
@@ -1354,6 +1607,11 @@ int ixh_pair_execute(ixh_pair* p, const ixh_buffers* b, int terms, int flags) {
         }
         DeviceGuard guard(p->device);
         hipStream_t stream = (hipStream_t) b->stream;
+        // the handle's buffers are reused by every call: a call on another stream waits for the previous one
+        if (p->done == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+        else if (stream != p->lastStream)
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->done, 0));
         p->lastStream = stream;
         const float4* corr = (const float4*) b->posq_correction;
         bool rebuild = !p->built;
@@ -1384,6 +1642,7 @@ int ixh_pair_execute(ixh_pair* p, const ixh_buffers* b, int terms, int flags) {
             IXH_HIP_CHECK(hipGetLastError());
             launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, stream);
         }
+        IXH_HIP_CHECK(hipEventRecord(p->done, stream));
     });
 }
 

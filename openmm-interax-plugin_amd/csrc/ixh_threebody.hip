@@ -431,6 +431,7 @@ struct ixh_threebody {
     DeviceArray<double4> nbrDelta;
     DeviceArray<long long> siteForce, tripleCount;
     hipStream_t lastStream = nullptr;
+    hipEvent_t done = nullptr;   // recorded at the end of every execute: calls on other streams wait for it
     KernelTimer timer;
 };
 
@@ -576,6 +577,8 @@ int ixh_threebody_destroy(ixh_threebody* h) {
             return;
         DeviceGuard guard(h->device);
         (void) hipDeviceSynchronize();
+        if (h->done != nullptr)
+            (void) hipEventDestroy(h->done);
         delete h;
     });
 }
@@ -607,6 +610,10 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
             return;
         DeviceGuard guard(h->device);
         hipStream_t stream = (hipStream_t) b->stream;
+        if (h->done == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
+        else if (stream != h->lastStream)
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, h->done, 0));
         h->lastStream = stream;
         const float4* corr = (const float4*) b->posq_correction;
         BoxT box = {b->box[0], b->box[3], b->box[4], b->box[6], b->box[7], b->box[8]};
@@ -645,6 +652,7 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
         hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
                            h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
         IXH_HIP_CHECK(hipGetLastError());
+        IXH_HIP_CHECK(hipEventRecord(h->done, stream));
     });
 }
 

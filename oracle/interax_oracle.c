@@ -387,6 +387,7 @@ typedef struct {
     double lambda;
     double* forces;
     double energy;
+    double abs_energy;     /* sum of |pair energy| before switching: the scale of rounding errors */
     long long count;
     int count_only;
 } pair_loop_state;
@@ -448,6 +449,7 @@ static void visit_pair(int i, int j, void* data) {
         return;
     if (rEff != r)
         dEdR *= pow(r/rEff, 5);
+    s->abs_energy += fabs((alchemicalPair ? s->lambda : 1.0)*e);
     int useSwitch = (f->method != ORC_NO_CUTOFF && f->use_switch);
     if (useSwitch && r > f->switching_distance) {
         double width = f->cutoff-f->switching_distance;
@@ -476,7 +478,7 @@ static int any_alchemical(const unsigned char* flags, int n) {
 }
 
 static int run_pair_loop(const orc_pair_force* f, const double* positions, const double* box, double lambda,
-                         double* forces, double* energy, long long* count, int count_only) {
+                         double* forces, double* energy, long long* count, int count_only, double* abs_energy) {
     if (f == NULL || positions == NULL || f->num_particles < 0 || f->kind < 0 || f->kind > 2)
         return -1;
     if (f->method == ORC_CUTOFF_PERIODIC && box == NULL)
@@ -517,17 +519,24 @@ static int run_pair_loop(const orc_pair_force* f, const double* positions, const
     }
     if (energy != NULL)
         *energy = s.energy;
+    if (abs_energy != NULL)
+        *abs_energy = s.abs_energy;
     return 0;
 }
 
 int orc_pair_compute(const orc_pair_force* f, const double* positions, const double* box, double lambda,
                      double* forces, double* energy) {
-    return run_pair_loop(f, positions, box, lambda, forces, energy, NULL, 0);
+    return run_pair_loop(f, positions, box, lambda, forces, energy, NULL, 0, NULL);
+}
+
+int orc_pair_compute_scale(const orc_pair_force* f, const double* positions, const double* box, double lambda,
+                           double* forces, double* energy, double* abs_energy) {
+    return run_pair_loop(f, positions, box, lambda, forces, energy, NULL, 0, abs_energy);
 }
 
 long long orc_pair_count(const orc_pair_force* f, const double* positions, const double* box) {
     long long count = 0;
-    if (run_pair_loop(f, positions, box, 1.0, NULL, NULL, &count, 1) != 0)
+    if (run_pair_loop(f, positions, box, 1.0, NULL, NULL, &count, 1, NULL) != 0)
         return -1;
     return count;
 }
@@ -764,7 +773,7 @@ static void tb_visit(int i, int j, void* data) {
 }
 
 static int run_threebody(const orc_threebody_force* f, const double* positions, const double* box, double lambda,
-                         double* forces, double* energy_out, long long* count_out) {
+                         double* forces, double* energy_out, long long* count_out, double* abs_out) {
     if (f == NULL || positions == NULL || (f->periodic && box == NULL))
         return -1;
     int n = f->num_sites;
@@ -803,7 +812,7 @@ static int run_threebody(const orc_threebody_force* f, const double* positions, 
         qsort(ns.lists[i], ns.count[i], sizeof(int), cmp_int);
     /* ReferenceInteraxThreeBodyIxn.cpp:107-125: i < j < k, all three pairs neighbors, in lexicographic order */
     double* siteForces = (double*) calloc(3*(size_t) (n > 0 ? n : 1), sizeof(double));
-    double energy = 0;
+    double energy = 0, absEnergy = 0;
     long long count = 0;
     int useDamping = (f->damping_method == 1);
     for (int i = 0; i < n; i++) {
@@ -831,8 +840,10 @@ static int run_threebody(const orc_threebody_force* f, const double* positions, 
                 double dij[3], dik[3];
                 delta_vec(sites+3*i, sites+3*j, box, f->periodic, dij);
                 delta_vec(sites+3*i, sites+3*k, box, f->periodic, dik);
-                energy += orc_triple(dij, dik, djk, c9, useDamping ? rates : NULL, siteForces+3*i, siteForces+3*j,
-                                     siteForces+3*k, f->centroid_damping_radius);
+                double e = orc_triple(dij, dik, djk, c9, useDamping ? rates : NULL, siteForces+3*i, siteForces+3*j,
+                                      siteForces+3*k, f->centroid_damping_radius);
+                energy += e;
+                absEnergy += fabs(e);
             }
         }
     }
@@ -856,17 +867,24 @@ static int run_threebody(const orc_threebody_force* f, const double* positions, 
         *energy_out = energy;
     if (count_out != NULL)
         *count_out = count;
+    if (abs_out != NULL)
+        *abs_out = absEnergy;
     return 0;
 }
 
 int orc_threebody_compute(const orc_threebody_force* f, const double* positions, const double* box, double lambda,
                           double* forces, double* energy) {
-    return run_threebody(f, positions, box, lambda, forces, energy, NULL);
+    return run_threebody(f, positions, box, lambda, forces, energy, NULL, NULL);
+}
+
+int orc_threebody_compute_scale(const orc_threebody_force* f, const double* positions, const double* box, double lambda,
+                                double* forces, double* energy, double* abs_energy) {
+    return run_threebody(f, positions, box, lambda, forces, energy, NULL, abs_energy);
 }
 
 long long orc_threebody_count(const orc_threebody_force* f, const double* positions, const double* box) {
     long long count = 0;
-    if (run_threebody(f, positions, box, 1.0, NULL, NULL, &count) != 0)
+    if (run_threebody(f, positions, box, 1.0, NULL, NULL, &count, NULL) != 0)
         return -1;
     return count;
 }

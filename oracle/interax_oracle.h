@@ -66,6 +66,10 @@ typedef struct {
  */
 int orc_pair_compute(const orc_pair_force* force, const double* positions, const double* box, double lambda,
                      double* forces, double* energy);
+/* The same, also returning the sum of |pair energies| before switching, the scale of rounding errors that
+   TestInteraxPairForce.h:90-127 compares energies against. */
+int orc_pair_compute_scale(const orc_pair_force* force, const double* positions, const double* box, double lambda,
+                           double* forces, double* energy, double* abs_energy);
 /* Number of interacting pairs (not excluded, gate nonzero, r < cutoff) the same call would evaluate. */
 long long orc_pair_count(const orc_pair_force* force, const double* positions, const double* box);
 /* Long range correction coefficients (InteraxDispersionForceImpl.cpp:86-149). */
@@ -100,6 +104,9 @@ double orc_c9(double a1, double w1, double a2, double w2, double a3, double w3);
 /* ReferenceInteraxThreeBodyIxn.cpp:84-135.  Returns 0 on success. */
 int orc_threebody_compute(const orc_threebody_force* force, const double* positions, const double* box, double lambda,
                           double* forces, double* energy);
+/* The same, also returning the sum of |triple energies| (TestInteraxThreeBodyForce.h:150-165 compares against it). */
+int orc_threebody_compute_scale(const orc_threebody_force* force, const double* positions, const double* box, double lambda,
+                                double* forces, double* energy, double* abs_energy);
 /* Number of interacting triples. */
 long long orc_threebody_count(const orc_threebody_force* force, const double* positions, const double* box);
 

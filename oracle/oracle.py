@@ -64,6 +64,8 @@ def lib() -> ctypes.CDLL:
         L.orc_short_range_pair.argtypes = [ctypes.c_double, _dp, _dp, ctypes.POINTER(TaylorScheme), _dp, _dp]
         L.orc_pair_compute.argtypes = [ctypes.POINTER(PairForce), _dp, _dp, ctypes.c_double, _dp, _dp]
         L.orc_pair_compute.restype = ctypes.c_int
+        L.orc_pair_compute_scale.argtypes = [ctypes.POINTER(PairForce), _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_pair_compute_scale.restype = ctypes.c_int
         L.orc_pair_count.argtypes = [ctypes.POINTER(PairForce), _dp, _dp]
         L.orc_pair_count.restype = ctypes.c_longlong
         L.orc_long_range_correction.argtypes = [ctypes.POINTER(PairForce), _dp, _dp]
@@ -75,6 +77,8 @@ def lib() -> ctypes.CDLL:
         L.orc_c9.restype = ctypes.c_double
         L.orc_threebody_compute.argtypes = [ctypes.POINTER(ThreeBodyForce), _dp, _dp, ctypes.c_double, _dp, _dp]
         L.orc_threebody_compute.restype = ctypes.c_int
+        L.orc_threebody_compute_scale.argtypes = [ctypes.POINTER(ThreeBodyForce), _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_threebody_compute_scale.restype = ctypes.c_int
         L.orc_threebody_count.argtypes = [ctypes.POINTER(ThreeBodyForce), _dp, _dp]
         L.orc_threebody_count.restype = ctypes.c_longlong
         _lib = L
@@ -176,15 +180,19 @@ def _pair_struct(force, double_precision=True):
     return s, (params, alch, excl)
 
 
-def pair_force(force, positions, box=None, lam=1.0, double_precision=True):
-    """Energy and forces (N x 3) of one pair force on the Reference platform."""
+def pair_force(force, positions, box=None, lam=1.0, double_precision=True, with_scale=False):
+    """Energy and forces (N x 3) of one pair force on the Reference platform; with_scale also returns the sum of
+    |pair energies| before switching."""
     s, keep = _pair_struct(force, double_precision)
     pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
     bx = np.ascontiguousarray(np.asarray(box if box is not None else np.eye(3)*1e6, dtype=np.float64).reshape(-1))
     forces = np.zeros(pos.size)
     e = ctypes.c_double()
-    if lib().orc_pair_compute(ctypes.byref(s), _d(pos), _d(bx), lam, _d(forces), ctypes.byref(e)) != 0:
+    scale = ctypes.c_double()
+    if lib().orc_pair_compute_scale(ctypes.byref(s), _d(pos), _d(bx), lam, _d(forces), ctypes.byref(e), ctypes.byref(scale)) != 0:
         raise ValueError("oracle: invalid pair force")
+    if with_scale:
+        return e.value, forces.reshape(-1, 3), scale.value
     return e.value, forces.reshape(-1, 3)
 
 
@@ -222,14 +230,18 @@ def _threebody_struct(force):
     return s, (offsets, parts, alpha, freq, width, mol, alch)
 
 
-def threebody_force(force, positions, box=None, lam=1.0):
+def threebody_force(force, positions, box=None, lam=1.0, with_scale=False):
+    """Energy and forces of a three-body force; with_scale also returns the sum of |triple energies|."""
     s, keep = _threebody_struct(force)
     pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
     bx = np.ascontiguousarray(np.asarray(box if box is not None else np.eye(3)*1e6, dtype=np.float64).reshape(-1))
     forces = np.zeros(pos.size)
     e = ctypes.c_double()
-    if lib().orc_threebody_compute(ctypes.byref(s), _d(pos), _d(bx), lam, _d(forces), ctypes.byref(e)) != 0:
+    scale = ctypes.c_double()
+    if lib().orc_threebody_compute_scale(ctypes.byref(s), _d(pos), _d(bx), lam, _d(forces), ctypes.byref(e), ctypes.byref(scale)) != 0:
         raise ValueError("oracle: invalid three-body force")
+    if with_scale:
+        return e.value, forces.reshape(-1, 3), scale.value
     return e.value, forces.reshape(-1, 3)
 
 
@@ -240,11 +252,13 @@ def threebody_count(force, positions, box=None):
     return lib().orc_threebody_count(ctypes.byref(s), _d(pos), _d(bx))
 
 
-def evaluate(system, positions, box=None, groups=0xFFFFFFFF, parameters=None, double_precision=True):
-    """Energy and forces of the force groups of a System, as Context.getState(groups) on the Reference platform."""
+def evaluate(system, positions, box=None, groups=0xFFFFFFFF, parameters=None, double_precision=True, with_scale=False):
+    """Energy and forces of the force groups of a System, as Context.getState(groups) on the Reference platform.
+    with_scale also returns the sum of |pair energies| of the pair forces (the rounding-error scale)."""
     parameters = parameters or {}
     n = system.getNumParticles()
     energy = 0.0
+    scale = 0.0
     forces = np.zeros((n, 3))
     if box is None:
         box = system.getDefaultPeriodicBoxVectors()
@@ -253,9 +267,13 @@ def evaluate(system, positions, box=None, groups=0xFFFFFFFF, parameters=None, do
             continue
         lam = parameters.get(f.getAlchemicalParameterName(), 1.0)
         if f.NAME == "InteraxThreeBodyForce":
-            e, fr = threebody_force(f, positions, box, lam)
+            e, fr, sc = threebody_force(f, positions, box, lam, with_scale=True)
+            scale += sc
         else:
-            e, fr = pair_force(f, positions, box, lam, double_precision)
+            e, fr, sc = pair_force(f, positions, box, lam, double_precision, with_scale=True)
+            scale += sc
         energy += e
         forces += fr
+    if with_scale:
+        return energy, forces, scale
     return energy, forces

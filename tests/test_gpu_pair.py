@@ -69,9 +69,11 @@ def compare(context, system, positions, precision, lam=1.0, groups=0xFFFFFFFF, b
     params = {}
     for f in system.getForces():
         params[f.getAlchemicalParameterName()] = lam
-    e_ref, f_ref = oracle.evaluate(system, positions, box, groups=groups, parameters=params)
+    e_ref, f_ref, abs_sum = oracle.evaluate(system, positions, box, groups=groups, parameters=params, with_scale=True)
     etol, ftol = TOL[precision]
-    escale = abs(e_ref) if scale_energy is None else scale_energy
+    # relative to the sum of |pair energies| before switching, the scale of rounding errors, as
+    # TestInteraxPairForce.h:137-143 compares (or to a given larger scale)
+    escale = abs_sum if scale_energy is None else max(abs_sum, scale_energy)
     assert abs(state.getPotentialEnergy()-e_ref) <= etol*max(escale, 1e-300)+1e-12, (state.getPotentialEnergy(), e_ref)
     fmax = np.abs(f_ref).max() if f_ref.size else 0.0
     err = np.abs(state.getForces()-f_ref).max() if f_ref.size else 0.0

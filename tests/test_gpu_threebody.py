@@ -35,10 +35,10 @@ def single_particle_sites(n, box=None):
 
 def compare(context, system, force, pos, precision, lam=1.0, box=None):
     state = context.getState()
-    e_ref, f_ref = oracle.threebody_force(force, pos, box, lam)
+    e_ref, f_ref, abs_sum = oracle.threebody_force(force, pos, box, lam, with_scale=True)
     etol, ftol = TOL[precision]
-    # the scale of the energy: the sum of |triple energies| bounds the cancellation between triples
-    abs_scale = max(abs(e_ref), 1e-300)
+    # the scale of the energy: the sum of |triple energies| (TestInteraxThreeBodyForce.h:150-165)
+    abs_scale = max(abs_sum, 1e-300)
     assert abs(state.getPotentialEnergy()-e_ref) <= etol*abs_scale+1e-14, (state.getPotentialEnergy(), e_ref)
     fmax = np.abs(f_ref).max()
     assert np.abs(state.getForces()-f_ref).max() <= ftol*fmax+RESOLUTION
