@@ -236,6 +236,14 @@ __global__ void k_gather_params(const int* perm, int padded, const typename Vec4
     sFlags[s] = flags[i];
 }
 
+__global__ void k_gather_int(const int* perm, int padded, const int* src, int* dst) {
+    int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= padded)
+        return;
+    int i = perm[s];
+    dst[s] = (i < 0 ? 0 : src[i]);
+}
+
 // Bounding box of each block, in the frame of its first particle (periodic images taken about it).
 struct BlockBoundsArgs {
     const double* pos3;
@@ -495,6 +503,12 @@ struct TileArgs {
     const R4* pB;    // dispersion width, exchange amplitude, exchange width, 1/exchange width
     const R4* pC;    // k, l, short-range width, 1/short-range width
     const int* flags;   // original index | alchemical bit, -1 for padding
+    // type tables (FP32 only): pA then holds wd, amp, K/2, L/2 and cls the class indices of each slot
+    const int* cls;
+    const float* tables;
+    int tableFloats;
+    int numDisp, numExch, numSr, exchOffset, srOffset;
+    float beta2;
     int n;
     // geometry
     int method;
@@ -517,9 +531,77 @@ struct TileArgs {
     int precision;
     long long* force;
     int stride;
-    double* tileEnergy;
-    int* tileCount;
+    double* waveEnergy;   // per wave of the launch: energy and pair count of its tiles, in tile order
+    int* waveCount;
 };
+
+// Copy the type tables into LDS (all threads of the workgroup) and describe them.
+template <typename Real>
+__device__ __forceinline__ PairTables load_tables(const TileArgs<Real>& a, float* lds) {
+    for (int i = threadIdx.x; i < a.tableFloats; i += blockDim.x)
+        lds[i] = a.tables[i];
+    __syncthreads();
+    PairTables tb;
+    tb.disp = (const float4*) lds;
+    tb.exch = (const OverlapRow*) (lds+a.exchOffset);
+    tb.sr = (const OverlapRow*) (lds+a.srOffset);
+    tb.numDisp = a.numDisp;
+    tb.numExch = a.numExch;
+    tb.numSr = a.numSr;
+    tb.beta2 = a.beta2;
+    tb.shared = a.settings.shared_overlap;
+    return tb;
+}
+
+// The particles of one side (i or j) of a tile in LDS.  With type tables only positions, the packed per-particle
+// values and the class indices are kept.
+template <typename Real, bool TABLES>
+struct SideLds {
+    typedef typename Vec4T<Real>::type R4;
+    R4 pos[BLOCK];
+    R4 A[BLOCK];
+    R4 B[TABLES ? 1 : BLOCK];
+    R4 C[TABLES ? 1 : BLOCK];
+    int cls[TABLES ? BLOCK : 1];
+    int flags[BLOCK];
+};
+
+template <typename Real, bool TABLES>
+struct ParticleType {
+    typedef PairParticle<Real> type;
+};
+template <>
+struct ParticleType<float, true> {
+    typedef TableParticle type;
+};
+
+template <typename Real, bool TABLES>
+__device__ __forceinline__ void stage_particle(SideLds<Real, TABLES>& s, int lane, const TileArgs<Real>& a, int slot) {
+    s.A[lane] = a.pA[slot];
+    if constexpr (TABLES)
+        s.cls[lane] = a.cls[slot];
+    else {
+        s.B[lane] = a.pB[slot];
+        s.C[lane] = a.pC[slot];
+    }
+    s.flags[lane] = a.flags[slot];
+}
+
+__device__ __forceinline__ void fetch_particle(const SideLds<float, true>& s, int l, TableParticle& P) {
+    const float4 A = s.A[l];
+    P.wd = A.x;
+    P.amp = A.y;
+    P.kh = A.z;
+    P.lh = A.w;
+    P.cls = s.cls[l];
+}
+template <typename Real>
+__device__ __forceinline__ void fetch_particle(const SideLds<Real, false>& s, int l, PairParticle<Real>& P) {
+    const typename Vec4T<Real>::type A = s.A[l], B = s.B[l], C = s.C[l];
+    P.m1 = A.x; P.m2 = A.y; P.m3 = A.z; P.alpha = A.w;
+    P.wd = B.x; P.amp = B.y; P.wx = B.z; P.iwx = B.w;
+    P.k = C.x; P.l = C.y; P.ws = C.z; P.iws = C.w;
+}
 
 // Exact membership test of the reference (ReferenceInteraxPairIxn.h:64-70) in FP64.
 __device__ __noinline__ bool exact_within_cutoff(const void* posq, const float4* correction, int precision, int periodic,
@@ -536,8 +618,8 @@ __device__ __noinline__ bool exact_within_cutoff(const void* posq, const float4*
 
 // Energy and force of one pair in range (ReferenceInteraxPairIxn.h:71-96): soft core, the pair terms, switching,
 // alchemical scaling.  d = r_j - r_i; returns the force on i as fs*d.
-template <typename Real, int TERMS>
-__device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, Real r2, const PairParticle<Real>& P, const PairParticle<Real>& Q,
+template <typename Real, int TERMS, typename Particle>
+__device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const PairTables& tb, Real r2, const Particle& P, const Particle& Q,
                                               bool alchI, int flagJ, int origI, Real& e, Real& fs) {
     const bool alchJ = ((flagJ >> 30) & 1) != 0;
     const bool alchPair = (alchI != alchJ);
@@ -551,7 +633,10 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, Real r2, 
     }
     Real dEdR;
     const bool iFirst = origI < (flagJ & INDEX_MASK);
-    pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
+    if constexpr (std::is_same<Particle, TableParticle>::value)
+        pair_energy_table<TERMS>(rEff, invREff, P, Q, iFirst, tb, e, dEdR);
+    else
+        pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
     if (rEff != r) {
         Real ratio = r*invREff;
         Real ratio2 = ratio*ratio;
@@ -569,26 +654,33 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, Real r2, 
     fs = scale*dEdR*invR;
 }
 
+// The parameters of one slot, from global memory.
+__device__ __forceinline__ void load_particle(const TileArgs<float>& a, int slot, TableParticle& P) {
+    const float4 A = a.pA[slot];
+    P.wd = A.x;
+    P.amp = A.y;
+    P.kh = A.z;
+    P.lh = A.w;
+    P.cls = a.cls[slot];
+}
 template <typename Real>
-__device__ __forceinline__ void unpack(PairParticle<Real>& P, const typename Vec4T<Real>::type& A, const typename Vec4T<Real>::type& B,
-                                       const typename Vec4T<Real>::type& C) {
+__device__ __forceinline__ void load_particle(const TileArgs<Real>& a, int slot, PairParticle<Real>& P) {
+    const typename Vec4T<Real>::type A = a.pA[slot], B = a.pB[slot], C = a.pC[slot];
     P.m1 = A.x; P.m2 = A.y; P.m3 = A.z; P.alpha = A.w;
     P.wd = B.x; P.amp = B.y; P.wx = B.z; P.iwx = B.w;
     P.k = C.x; P.l = C.y; P.ws = C.z; P.iws = C.w;
 }
 
 // Load the partner chunk of a tile into LDS, in the frame of block I.
-template <typename Real>
-__device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 tile, int lane, typename Vec4T<Real>::type* sPos,
-                                               typename Vec4T<Real>::type* sA, typename Vec4T<Real>::type* sB,
-                                               typename Vec4T<Real>::type* sC, int* sFlags) {
+template <typename Real, bool TABLES>
+__device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 tile, int lane, SideLds<Real, TABLES>& s) {
     typedef typename Vec4T<Real>::type R4;
     const int I = tile.x;
     const int slot = (tile.y < 0 ? I*BLOCK+lane : a.partners[tile.y+lane]);
     if (slot < 0) {
         R4 zero = {0, 0, 0, 0};
-        sPos[lane] = zero;
-        sFlags[lane] = -1;
+        s.pos[lane] = zero;
+        s.flags[lane] = -1;
         return;
     }
     const int J = slot/BLOCK;
@@ -601,21 +693,33 @@ __device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 til
         pj.y += Real(sy);
         pj.z += Real(sz);
     }
-    sPos[lane] = pj;
-    sA[lane] = a.pA[slot];
-    sB[lane] = a.pB[slot];
-    sC[lane] = a.pC[slot];
-    sFlags[lane] = a.flags[slot];
+    s.pos[lane] = pj;
+    stage_particle<Real, TABLES>(s, lane, a, slot);
+}
+
+// Energy and pair count of a wave, written once at its end: the launch's totals are then summed in wave order.
+__device__ __forceinline__ void store_wave_totals(double* waveEnergy, int* waveCount, long long wave, int lane, double energy, int pairs) {
+    const double e = wave_sum(energy);
+    const int c = (int) wave_sum((double) pairs);
+    if (lane == 0) {
+        waveEnergy[wave] = e;
+        waveCount[wave] = c;
+    }
 }
 
 // Tiles without a cutoff: every candidate interacts.  64 rotation steps, lane l paired with j = (l+s) % 64.
-template <typename Real, int TERMS>
+template <typename Real, int TERMS, bool TABLES>
 __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
-    __shared__ R4 sPos[4][BLOCK], sA[4][BLOCK], sB[4][BLOCK], sC[4][BLOCK];
-    __shared__ int sFlags[4][BLOCK];
+    typedef typename ParticleType<Real, TABLES>::type Particle;
+    __shared__ SideLds<Real, TABLES> side[4];
+    extern __shared__ float tableLds[];   // the type tables, sized at launch
+    PairTables tb = {};
+    if constexpr (TABLES)
+        tb = load_tables<Real>(a, tableLds);
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
+    SideLds<Real, TABLES>& s = side[w];
     const long long wave = (long long) blockIdx.x*4+w;
     const long long numWaves = (long long) gridDim.x*4;
     const long long T = a.numTiles;
@@ -625,7 +729,9 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
     R4 pi = {0, 0, 0, 0};
     int flagI = -1;
     long long fixI[3] = {0, 0, 0};
-    PairParticle<Real> P, Q;
+    Particle P, Q;
+    double energy = 0;
+    int count = 0;
 
     for (long long t = begin; t < end; t++) {
         const int2 tile = a.tiles[t];
@@ -641,10 +747,10 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
             curI = I;
             int slot = I*BLOCK+lane;
             pi = a.local[slot];
-            unpack<Real>(P, a.pA[slot], a.pB[slot], a.pC[slot]);
             flagI = a.flags[slot];
+            load_particle(a, slot, P);
         }
-        stage_partners<Real>(a, tile, lane, sPos[w], sA[w], sB[w], sC[w], sFlags[w]);
+        stage_partners<Real, TABLES>(a, tile, lane, s);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const unsigned long long mrow = a.masks[(size_t) t*BLOCK+lane];
         const bool diag = (tile.y < 0);
@@ -653,21 +759,19 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
         const int origI = flagI & INDEX_MASK;
         Real fi[3] = {0, 0, 0};
         Real fj[3] = {0, 0, 0};
-        double energy = 0;
-        int count = 0;
-        for (int s = sBegin; s < sEnd; s++) {
-            const int jl = (lane+s) & 63;
+        for (int st = sBegin; st < sEnd; st++) {
+            const int jl = (lane+st) & 63;
             bool interact = ((mrow >> jl) & 1ULL) != 0;
-            if (diag && s == 32 && lane >= 32)
+            if (diag && st == 32 && lane >= 32)
                 interact = false;
             if (interact) {
-                const R4 pj = sPos[w][jl];
+                const R4 pj = s.pos[jl];
                 const Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
                 const Real r2 = dx*dx+dy*dy+dz*dz;
-                const int flagJ = sFlags[w][jl];
-                unpack<Real>(Q, sA[w][jl], sB[w][jl], sC[w][jl]);
+                const int flagJ = s.flags[jl];
+                fetch_particle(s, jl, Q);
                 Real e, fs;
-                evaluate_pair<Real, TERMS>(a, r2, P, Q, alchI, flagJ, origI, e, fs);
+                evaluate_pair<Real, TERMS>(a, tb, r2, P, Q, alchI, flagJ, origI, e, fs);
                 energy += (double) e;
                 fi[0] += dx*fs;
                 fi[1] += dy*fs;
@@ -684,7 +788,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
         // lane l now holds the accumulator of j = (l + sEnd) % 64
         {
             const int jl = (lane+sEnd) & 63;
-            const int flagJ = sFlags[w][jl];
+            const int flagJ = s.flags[jl];
             if (flagJ >= 0) {
                 int oj = flagJ & INDEX_MASK;
                 add_fixed(&a.force[oj], to_fixed(fj[0]));
@@ -695,12 +799,6 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
         fixI[0] += to_fixed(fi[0]);
         fixI[1] += to_fixed(fi[1]);
         fixI[2] += to_fixed(fi[2]);
-        double tileEnergy = wave_sum(energy);
-        int tileCount = (int) wave_sum((double) count);
-        if (lane == 0) {
-            a.tileEnergy[t] = tileEnergy;
-            a.tileCount[t] = tileCount;
-        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     if (curI >= 0 && flagI >= 0) {
@@ -709,63 +807,66 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
         add_fixed(&a.force[a.stride+oi], fixI[1]);
         add_fixed(&a.force[2*a.stride+oi], fixI[2]);
     }
+    store_wave_totals(a.waveEnergy, a.waveCount, wave, lane, energy, count);
 }
 
 // Tiles with a cutoff.  A lockstep filter pass over the 64 rotation steps pushes the pairs inside the cutoff into a
 // per-wave ring queue in LDS (ballot + rank, so the order is deterministic); every full batch of 64 queued pairs is
 // then evaluated with all lanes busy, whatever the spread of neighbour counts between the particles of the tile.
-// The forces on i and j are added in LDS as 64-bit fixed point and flushed to the force buffer per tile (j) and per
-// block (i).
-constexpr int QUEUE = 256;
+// The forces on i and j are added in LDS as 64-bit fixed point (the j side gets the exact negation of the i side's
+// value) and flushed to the force buffer per tile (j) and per block (i).
+constexpr int QUEUE = 128;   // pending pairs never exceed 63 + 64
 
-template <typename Real>
+template <typename Real, bool TABLES>
 struct QueueLds {
-    typedef typename Vec4T<Real>::type R4;
-    R4 iPos[BLOCK], iA[BLOCK], iB[BLOCK], iC[BLOCK];
-    R4 jPos[BLOCK], jA[BLOCK], jB[BLOCK], jC[BLOCK];
-    int iFlags[BLOCK], jFlags[BLOCK];
+    SideLds<Real, TABLES> i, j;
     unsigned long long fI[3][BLOCK], fJ[3][BLOCK];
     unsigned short queue[QUEUE];
 };
 
-template <typename Real, int TERMS>
-__device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, QueueLds<Real>& q, int head, int valid, int lane,
-                                               double& energy, int& pairs) {
+template <typename Real, int TERMS, bool TABLES>
+__device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const PairTables& tb, QueueLds<Real, TABLES>& q, int head,
+                                               int valid, int lane, double& energy, int& pairs) {
     typedef typename Vec4T<Real>::type R4;
+    typedef typename ParticleType<Real, TABLES>::type Particle;
     if (lane >= valid)
         return;
     const unsigned e = q.queue[(head+lane) & (QUEUE-1)];
     const int il = (int) (e >> 6), jl = (int) (e & 63);
-    const R4 pi = q.iPos[il], pj = q.jPos[jl];
+    const R4 pi = q.i.pos[il], pj = q.j.pos[jl];
     Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
     if (a.pairImage)
         min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6], a.boxR[7],
                         a.boxR[8]);
     const Real r2 = dx*dx+dy*dy+dz*dz;
-    PairParticle<Real> P, Q;
-    unpack<Real>(P, q.iA[il], q.iB[il], q.iC[il]);
-    unpack<Real>(Q, q.jA[jl], q.jB[jl], q.jC[jl]);
-    const int flagI = q.iFlags[il], flagJ = q.jFlags[jl];
+    Particle P, Q;
+    fetch_particle(q.i, il, P);
+    fetch_particle(q.j, jl, Q);
+    const int flagI = q.i.flags[il], flagJ = q.j.flags[jl];
     Real en, fs;
-    evaluate_pair<Real, TERMS>(a, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
+    evaluate_pair<Real, TERMS>(a, tb, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
     energy += (double) en;
     pairs++;
-    const Real fx = dx*fs, fy = dy*fs, fz = dz*fs;
-    atomicAdd(&q.fI[0][il], (unsigned long long) to_fixed(fx));
-    atomicAdd(&q.fI[1][il], (unsigned long long) to_fixed(fy));
-    atomicAdd(&q.fI[2][il], (unsigned long long) to_fixed(fz));
-    atomicAdd(&q.fJ[0][jl], (unsigned long long) to_fixed(-fx));
-    atomicAdd(&q.fJ[1][jl], (unsigned long long) to_fixed(-fy));
-    atomicAdd(&q.fJ[2][jl], (unsigned long long) to_fixed(-fz));
+    const long long fx = to_fixed(dx*fs), fy = to_fixed(dy*fs), fz = to_fixed(dz*fs);
+    atomicAdd(&q.fI[0][il], (unsigned long long) fx);
+    atomicAdd(&q.fI[1][il], (unsigned long long) fy);
+    atomicAdd(&q.fI[2][il], (unsigned long long) fz);
+    atomicAdd(&q.fJ[0][jl], (unsigned long long) -fx);
+    atomicAdd(&q.fJ[1][jl], (unsigned long long) -fy);
+    atomicAdd(&q.fJ[2][jl], (unsigned long long) -fz);
 }
 
-template <typename Real, int TERMS>
+template <typename Real, int TERMS, bool TABLES>
 __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
-    __shared__ QueueLds<Real> lds[4];
+    __shared__ QueueLds<Real, TABLES> lds[4];
+    extern __shared__ float tableLds[];   // the type tables, sized at launch
+    PairTables tb = {};
+    if constexpr (TABLES)
+        tb = load_tables<Real>(a, tableLds);
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    QueueLds<Real>& q = lds[w];
+    QueueLds<Real, TABLES>& q = lds[w];
     const long long wave = (long long) blockIdx.x*4+w;
     const long long numWaves = (long long) gridDim.x*4;
     const long long T = a.numTiles;
@@ -776,6 +877,8 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
     int curI = -1;
     R4 pi = {0, 0, 0, 0};
     int flagI = -1;
+    double energy = 0;
+    int pairs = 0;
     for (long long t = begin; t < end; t++) {
         const int2 tile = a.tiles[t];
         const int I = tile.x;
@@ -790,27 +893,22 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
             const int slot = I*BLOCK+lane;
             pi = a.local[slot];
             flagI = a.flags[slot];
-            q.iPos[lane] = pi;
-            q.iA[lane] = a.pA[slot];
-            q.iB[lane] = a.pB[slot];
-            q.iC[lane] = a.pC[slot];
-            q.iFlags[lane] = flagI;
+            q.i.pos[lane] = pi;
+            stage_particle<Real, TABLES>(q.i, lane, a, slot);
             q.fI[0][lane] = q.fI[1][lane] = q.fI[2][lane] = 0;
         }
-        stage_partners<Real>(a, tile, lane, q.jPos, q.jA, q.jB, q.jC, q.jFlags);
+        stage_partners<Real, TABLES>(a, tile, lane, q.j);
         q.fJ[0][lane] = q.fJ[1][lane] = q.fJ[2][lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const unsigned long long mrow = a.masks[(size_t) t*BLOCK+lane];
         const bool diag = (tile.y < 0);
         const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
         int head = 0, count = 0;
-        double energy = 0;
-        int pairs = 0;
         for (int s = sBegin; s < sEnd; s++) {
             const int jl = (lane+s) & 63;
             bool in = ((mrow >> jl) & 1ULL) != 0 && !(diag && s == 32 && lane >= 32);
             if (in) {
-                const R4 pj = q.jPos[jl];
+                const R4 pj = q.j.pos[jl];
                 Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
                 if (a.pairImage)
                     min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
@@ -818,7 +916,7 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
                 const Real r2 = dx*dx+dy*dy+dz*dz;
                 if (r2 >= inner)
                     in = (r2 < outer) && exact_within_cutoff(a.posq, a.correction, a.precision, a.periodic, a.box, a.cutoff,
-                                                             flagI & INDEX_MASK, q.jFlags[jl] & INDEX_MASK);
+                                                             flagI & INDEX_MASK, q.j.flags[jl] & INDEX_MASK);
             }
             const unsigned long long ballot = __ballot(in);
             if (in) {
@@ -828,30 +926,24 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
             count += __popcll(ballot);
             if (count >= BLOCK) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                evaluate_batch<Real, TERMS>(a, q, head, BLOCK, lane, energy, pairs);
+                evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy, pairs);
                 head = (head+BLOCK) & (QUEUE-1);
                 count -= BLOCK;
             }
         }
         if (count > 0) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            evaluate_batch<Real, TERMS>(a, q, head, count, lane, energy, pairs);
+            evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, count, lane, energy, pairs);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         {
-            const int flagJ = q.jFlags[lane];
+            const int flagJ = q.j.flags[lane];
             if (flagJ >= 0) {
                 int oj = flagJ & INDEX_MASK;
                 add_fixed(&a.force[oj], (long long) q.fJ[0][lane]);
                 add_fixed(&a.force[a.stride+oj], (long long) q.fJ[1][lane]);
                 add_fixed(&a.force[2*a.stride+oj], (long long) q.fJ[2][lane]);
             }
-        }
-        double tileEnergy = wave_sum(energy);
-        int tileCount = (int) wave_sum((double) pairs);
-        if (lane == 0) {
-            a.tileEnergy[t] = tileEnergy;
-            a.tileCount[t] = tileCount;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -861,26 +953,27 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
         add_fixed(&a.force[a.stride+oi], (long long) q.fI[1][lane]);
         add_fixed(&a.force[2*a.stride+oi], (long long) q.fI[2][lane]);
     }
+    store_wave_totals(a.waveEnergy, a.waveCount, wave, lane, energy, pairs);
 }
 
-// Sum the tile energies in tile order (one workgroup, fixed reduction tree) and add them to the energy buffer.
-__global__ void k_reduce_tiles(const double* tileEnergy, const int* tileCount, int numTiles, double* energy, long long* pairCount,
-                               int addEnergy) {
-    __shared__ double sE[256];
-    __shared__ long long sC[256];
+// Sum the per-wave energies and pair counts of a launch in wave order (one workgroup, contiguous chunks per thread
+// and a fixed reduction tree, so the result does not depend on timing) and add the energy to the energy buffer.
+__global__ void __launch_bounds__(1024) k_reduce_waves(const double* waveEnergy, const int* waveCount, int numWaves, double* energy,
+                                                       long long* pairCount, int addEnergy) {
+    __shared__ double sE[1024];
+    __shared__ long long sC[1024];
     double e = 0;
     long long c = 0;
-    // contiguous chunks per thread, so the sum is a fixed function of the tile order
-    int per = (numTiles+255)/256;
-    int begin = threadIdx.x*per, end = min(numTiles, begin+per);
-    for (int t = begin; t < end; t++) {
-        e += tileEnergy[t];
-        c += tileCount[t];
+    const int per = (numWaves+1023)/1024;
+    const int begin = threadIdx.x*per, end = min(numWaves, begin+per);
+    for (int k = begin; k < end; k++) {
+        e += waveEnergy[k];
+        c += waveCount[k];
     }
     sE[threadIdx.x] = e;
     sC[threadIdx.x] = c;
     __syncthreads();
-    for (int stride = 128; stride > 0; stride >>= 1) {
+    for (int stride = 512; stride > 0; stride >>= 1) {
         if (threadIdx.x < stride) {
             sE[threadIdx.x] += sE[threadIdx.x+stride];
             sC[threadIdx.x] += sC[threadIdx.x+stride];
@@ -1042,10 +1135,15 @@ struct ixh_pair {
     // parameters in particle order and in slot order
     DeviceArray<char> paramA, paramB, paramC, sortedA, sortedB, sortedC, local;
     DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionOffsets, exclusionPartners, blockStart, blockCount, blockList,
-        partnerCount, partnerStart, partners, tileCount;
+        partnerCount, partnerStart, partners, waveCount;
+    // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
+    bool useTables = false;
+    int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
+    DeviceArray<int> classes, sortedClasses;
+    DeviceArray<float> tables;
     DeviceArray<unsigned> keys, keysOut;
     DeviceArray<int> values;
-    DeviceArray<double> pos3, buildPos, bounds, centers, tileEnergy, bbox;
+    DeviceArray<double> pos3, buildPos, bounds, centers, waveEnergy, bbox;
     DeviceArray<int2> tiles;
     DeviceArray<unsigned long long> masks;
     DeviceArray<char> sortTemp;
@@ -1133,6 +1231,133 @@ OverlapScheme make_scheme(int method, bool doublePrecision) {
     return s;
 }
 
+// One row of an overlap table, in FP64 rounded once to FP32.  Same branch rule as slater_overlap: the Taylor
+// branch where |delta| < maxDelta, or |t| < maxT and |t| x < maxTX, written as r |t|/l < maxTX.
+OverlapRow overlap_row(double a, double b, const OverlapScheme& s) {
+    OverlapRow w;
+    std::memset(&w, 0, sizeof(w));
+    if (a == 0 || b == 0)
+        return w;
+    double l = (s.about_smaller ? std::min(a, b) : a);
+    double delta = a+b-l-l;
+    double t = delta/l;
+    if (std::fabs(delta) < s.max_delta) {
+        w.kTaylor = 0;
+        w.txLimit = 1;
+    }
+    else if (std::fabs(t) < s.max_t) {
+        w.kTaylor = (float) (std::fabs(t)/l);
+        w.txLimit = (float) s.max_tx;
+    }
+    else {
+        w.kTaylor = 0;
+        w.txLimit = 0;
+    }
+    for (int k = 0; k <= s.order+2; k++)
+        w.c[k] = (float) OverlapTaylor::coefficient(s.order, k, t);
+    w.invL = (float) (1/l);
+    w.pref = (float) (INV_PI/(l*l*l));
+    w.ka = (float) (LOG2E/a);
+    w.kb = (float) (LOG2E/b);
+    w.lIsA = (l == a ? 1.0f : 0.0f);
+    double D = b*b-a*a;
+    if (D != 0) {
+        w.cc = (float) (4*a*a*b*b/(D*D*D)*INV_8PI);
+        w.cd = (float) (INV_8PI/(D*D));
+    }
+    w.a = (float) a;
+    w.b = (float) b;
+    w.ia = (float) (1/a);
+    w.ib = (float) (1/b);
+    w.norm = (float) (EIGHT_PI*(a+b)*(a+b)*(a+b));
+    w.gate = 1;
+    return w;
+}
+
+// Parameter classes and type tables.  Returns false, leaving the per-pair path in use, when the particles fall into
+// more classes than the LDS tables hold.  packed receives wd, amp, K/2, L/2 per particle.
+bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls, std::vector<float>& tables) {
+    const int n = p->n;
+    std::map<std::tuple<double, double, double, double>, int> dispMap;
+    std::map<double, int> exchMap, srMap;
+    std::vector<std::tuple<double, double, double, double>> dispKeys;
+    std::vector<double> exchWidths, srWidths;
+    cls.assign(n, 0);
+    packed.assign(4*(size_t) n, 0.0f);
+    auto classOf = [](std::map<double, int>& m, std::vector<double>& keys, double w) {
+        auto it = m.find(w);
+        if (it != m.end())
+            return it->second;
+        int c = (int) keys.size();
+        m.emplace(w, c);
+        keys.push_back(w);
+        return c;
+    };
+    for (int i = 0; i < n; i++) {
+        int d = 0, x = 0, s = 0;
+        if (p->terms & IXH_TERM_DISPERSION) {
+            const double* q = &p->dispersion[5*(size_t) i];
+            auto key = std::make_tuple(q[0], q[1], q[2], q[3]);
+            auto it = dispMap.find(key);
+            if (it == dispMap.end()) {
+                d = (int) dispKeys.size();
+                dispMap.emplace(key, d);
+                dispKeys.push_back(key);
+            }
+            else
+                d = it->second;
+            packed[4*i] = (float) q[4];
+        }
+        if (p->terms & IXH_TERM_EXCHANGE) {
+            x = classOf(exchMap, exchWidths, p->exchange[2*i+1]);
+            packed[4*i+1] = (float) p->exchange[2*i];
+        }
+        if (p->terms & IXH_TERM_SHORT_RANGE) {
+            s = classOf(srMap, srWidths, p->shortRange[3*i+2]);
+            packed[4*i+2] = (float) (0.5*p->shortRange[3*i]);
+            packed[4*i+3] = (float) (0.5*p->shortRange[3*i+1]);
+        }
+        if (dispKeys.size() > (size_t) MAX_DISPERSION_CLASSES || exchWidths.size() > (size_t) MAX_WIDTH_CLASSES ||
+            srWidths.size() > (size_t) MAX_WIDTH_CLASSES)
+            return false;
+        cls[i] = d | (x << 8) | (s << 16);
+    }
+    const bool shared = p->settings.shared_overlap != 0;
+    p->numDisp = std::max<int>(1, (int) dispKeys.size());
+    p->numExch = std::max<int>(1, (int) exchWidths.size());
+    p->numSr = (shared ? p->numExch : std::max<int>(1, (int) srWidths.size()));
+    p->exchOffset = 4*p->numDisp*p->numDisp;
+    int exchFloats = OVERLAP_ROW_FLOATS*p->numExch*p->numExch;
+    p->srOffset = (shared ? p->exchOffset : p->exchOffset+exchFloats);
+    p->tableFloats = p->srOffset+(shared ? exchFloats : OVERLAP_ROW_FLOATS*p->numSr*p->numSr);
+    tables.assign(p->tableFloats, 0.0f);
+    // dispersion rows: C6, C8, C10 combined as in InteraxPairTerms.h:271-279, and the gate alpha_i alpha_j != 0
+    for (int ci = 0; ci < (int) dispKeys.size(); ci++)
+        for (int cj = 0; cj < (int) dispKeys.size(); cj++) {
+            double m1i, m2i, m3i, ai, m1j, m2j, m3j, aj;
+            std::tie(m1i, m2i, m3i, ai) = dispKeys[ci];
+            std::tie(m1j, m2j, m3j, aj) = dispKeys[cj];
+            double pre = ai*aj/std::max(ai*m1j+aj*m1i, 1e-30);
+            float* row = &tables[4*(ci*p->numDisp+cj)];
+            row[0] = (float) (pre*m1i*m1j);
+            row[1] = (float) (1.5*pre*(m1i*m2j+m2i*m1j));
+            row[2] = (float) (pre*(2*(m1i*m3j+m3i*m1j)+4.2*m2i*m2j));
+            row[3] = (ai*aj != 0 ? 1.0f : 0.0f);
+        }
+    for (int ci = 0; ci < (int) exchWidths.size(); ci++)
+        for (int cj = 0; cj < (int) exchWidths.size(); cj++) {
+            OverlapRow w = overlap_row(exchWidths[ci], exchWidths[cj], p->settings.exchange_scheme);
+            std::memcpy(&tables[p->exchOffset+OVERLAP_ROW_FLOATS*(ci*p->numExch+cj)], &w, sizeof(w));
+        }
+    if (!shared)
+        for (int ci = 0; ci < (int) srWidths.size(); ci++)
+            for (int cj = 0; cj < (int) srWidths.size(); cj++) {
+                OverlapRow w = overlap_row(srWidths[ci], srWidths[cj], p->settings.short_range_scheme);
+                std::memcpy(&tables[p->srOffset+OVERLAP_ROW_FLOATS*(ci*p->numSr+cj)], &w, sizeof(w));
+            }
+    return true;
+}
+
 // Copy the per-particle parameters of a descriptor into the handle and onto the device.
 void load_parameters(ixh_pair* p, const ixh_pair_desc* d, hipStream_t stream) {
     int n = p->n;
@@ -1183,6 +1408,17 @@ void load_parameters(ixh_pair* p, const ixh_pair_desc* d, hipStream_t stream) {
     }
     else {
         std::vector<float> Af(A.begin(), A.end()), Bf(B.begin(), B.end()), Cf(C.begin(), C.end());
+        std::vector<float> packed, tables;
+        std::vector<int> cls;
+        p->useTables = build_tables(p, packed, cls, tables);
+        if (p->useTables) {
+            // with type tables the first parameter vector holds wd, amp, K/2, L/2
+            Af.swap(packed);
+            if (cls.empty())
+                cls.push_back(0);
+            p->classes.upload(cls.data(), cls.size(), stream);
+            p->tables.upload(tables.data(), tables.size(), stream);
+        }
         p->paramA.upload((const char*) Af.data(), sizeof(float)*Af.size(), stream);
         p->paramB.upload((const char*) Bf.data(), sizeof(float)*Bf.size(), stream);
         p->paramC.upload((const char*) Cf.data(), sizeof(float)*Cf.size(), stream);
@@ -1324,6 +1560,12 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
                            (const float4*) p->paramA.ptr, (const float4*) p->paramB.ptr, (const float4*) p->paramC.ptr, p->flags.ptr,
                            (float4*) p->sortedA.ptr, (float4*) p->sortedB.ptr, (float4*) p->sortedC.ptr, p->sortedFlags.ptr);
     IXH_HIP_CHECK(hipGetLastError());
+    if (p->useTables) {
+        p->sortedClasses.reserve(p->padded);
+        hipLaunchKernelGGL(k_gather_int, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
+                           (const int*) p->classes.ptr, p->sortedClasses.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
 
     double listCutoff = p->cutoff+SKIN;
     if (cutoff) {
@@ -1402,8 +1644,6 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         hipLaunchKernelGGL(k_build_masks, dim3(T), dim3(BLOCK), 0, stream, m, p->masks.ptr);
         IXH_HIP_CHECK(hipGetLastError());
     }
-    p->tileEnergy.reserve(std::max(T, 1));
-    p->tileCount.reserve(std::max(T, 1));
     if (cutoff) {
         p->buildPos.reserve(3*(size_t) n);
         IXH_HIP_CHECK(hipMemcpyAsync(p->buildPos.ptr, p->pos3.ptr, sizeof(double)*3*(size_t) n, hipMemcpyDeviceToDevice, stream));
@@ -1457,30 +1697,56 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     a.precision = b->precision;
     a.force = b->force;
     a.stride = b->force_stride;
-    a.tileEnergy = p->tileEnergy.ptr;
-    a.tileCount = p->tileCount.ptr;
-    int blocks = std::max(1, std::min((p->numTiles+3)/4, 4096));
+    const bool tables = (sizeof(Real) == 4 && p->useTables);
+    a.cls = p->sortedClasses.ptr;
+    a.tables = p->tables.ptr;
+    a.tableFloats = p->tableFloats;
+    a.numDisp = p->numDisp;
+    a.numExch = p->numExch;
+    a.numSr = p->numSr;
+    a.exchOffset = p->exchOffset;
+    a.srOffset = p->srOffset;
+    a.beta2 = (float) (2*p->dampingScale);
+    const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
+    const int blocks = std::max(1, std::min((p->numTiles+3)/4, 4096));
+    const int numWaves = 4*blocks;
+    p->waveEnergy.reserve(numWaves);
+    p->waveCount.reserve(numWaves);
+    a.waveEnergy = p->waveEnergy.ptr;
+    a.waveCount = p->waveCount.ptr;
     if (p->numTiles > 0) {
         bool timed = p->timer.start(stream);
         bool compact = (p->method != IXH_NO_CUTOFF);
         switch (terms) {
-#define IXH_CASE(T) case T: \
+#define IXH_LAUNCH(T, TAB) \
             if (compact) \
-                hipLaunchKernelGGL((k_pair_queue<Real, T>), dim3(blocks), dim3(256), 0, stream, a); \
+                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB>), dim3(blocks), dim3(256), tableBytes, stream, a); \
             else \
-                hipLaunchKernelGGL((k_pair_rotate<Real, T>), dim3(blocks), dim3(256), 0, stream, a); \
+                hipLaunchKernelGGL((k_pair_rotate<Real, T, TAB>), dim3(blocks), dim3(256), tableBytes, stream, a);
+#define IXH_CASE(T) case T: \
+            if constexpr (sizeof(Real) == 4) { \
+                if (tables) { \
+                    IXH_LAUNCH(T, true) \
+                    break; \
+                } \
+            } \
+            IXH_LAUNCH(T, false) \
             break;
             IXH_CASE(1) IXH_CASE(2) IXH_CASE(3) IXH_CASE(4) IXH_CASE(5) IXH_CASE(6) IXH_CASE(7)
 #undef IXH_CASE
+#undef IXH_LAUNCH
             default:
                 throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: illegal terms");
         }
         IXH_HIP_CHECK(hipGetLastError());
         if (timed)
             p->timer.stop(stream);
+        hipLaunchKernelGGL(k_reduce_waves, dim3(1), dim3(1024), 0, stream, p->waveEnergy.ptr, p->waveCount.ptr, numWaves, b->energy,
+                           p->pairCount.ptr, addEnergy);
     }
-    hipLaunchKernelGGL(k_reduce_tiles, dim3(1), dim3(256), 0, stream, p->tileEnergy.ptr, p->tileCount.ptr, p->numTiles, b->energy,
-                       p->pairCount.ptr, addEnergy);
+    else
+        hipLaunchKernelGGL(k_reduce_waves, dim3(1), dim3(1024), 0, stream, p->waveEnergy.ptr, p->waveCount.ptr, 0, b->energy,
+                           p->pairCount.ptr, addEnergy);
     IXH_HIP_CHECK(hipGetLastError());
 }
 

@@ -5,7 +5,8 @@
 // The mathematics is that of openmmapi/include/internal/InteraxPairTerms.h:167-337 (docs/SPEC.md:37-122); the
 // evaluation is organised for the VALU of gfx950: reciprocals, square roots and exponentials use the hardware
 // approximations (v_rcp_f32, v_rsq_f32, v_exp_f32) in FP32, both exponentials of a Slater overlap are shared
-// between its closed form and its Taylor branch, and inverse widths are precomputed per particle.
+// between its closed form and its Taylor branch, polynomials are evaluated in Horner form with compile-time
+// coefficients, and everything that depends only on a pair of particle types can come from a table (PairTables).
 #ifndef IXH_PAIR_MATH_H
 #define IXH_PAIR_MATH_H
 
@@ -22,6 +23,7 @@ constexpr double EIGHT_PI = 25.132741228718345908;
 // (the same table as InteraxPairTerms.h:96-104; it is a property of the integral).  Row n has terms k <= n+2.
 struct OverlapTaylor {
     static constexpr int ORDER = 6;
+    static constexpr int DEGREE = ORDER+2;
     __host__ __device__ static constexpr double v(int n, int k) {
         constexpr double table[7][9] = {
             {1.0/64, 1.0/64, 1.0/192, 0, 0, 0, 0, 0, 0},
@@ -33,9 +35,12 @@ struct OverlapTaylor {
             {7.0/1024, 7.0/1024, -191.0/12288, -73.0/4096, 369.0/20480, -233.0/46080, 389.0/645120, -41.0/1290240, 1.0/1658880}};
         return table[n][k];
     }
-    // coefficient of x^k of d/dx [exp(-x) sum V_k x^k] / exp(-x)
-    __host__ __device__ static constexpr double d(int n, int k) {
-        return (k+1 < 9 ? (k+1)*v(n, k+1) : 0.0) - v(n, k);
+    // c_k(t) = sum_{n <= order} V[n][k] t^n: the coefficient of x^k once t is fixed
+    __host__ static double coefficient(int order, int k, double t) {
+        double c = 0;
+        for (int n = order; n >= 0; n--)
+            c = c*t + v(n, k);
+        return c;
     }
 };
 
@@ -62,6 +67,36 @@ struct SchemeValues<double> {
     __device__ static double max_tx(const OverlapScheme& s) { return s.max_tx; }
 };
 
+// P(x) = sum_k c_k(t) x^k of the Taylor branch and its x derivative, for expansion order ORDER.  The coefficients
+// c_k(t) are formed first (Horner in t, with the t^(k-2) factor of the rows k >= 3 pulled out), then P and P' by a
+// joint Horner recurrence in x: about 60 FMAs at order 6 instead of the 98 of a row-by-row expansion.
+template <typename Real, int ORDER>
+__device__ __forceinline__ void taylor_polynomial(Real t, Real x, Real& P, Real& dP) {
+    constexpr int K = ORDER+2;
+    Real tp[ORDER+1];
+    tp[0] = 1;
+#pragma unroll
+    for (int n = 1; n <= ORDER; n++)
+        tp[n] = tp[n-1]*t;
+    Real c[K+1];
+#pragma unroll
+    for (int k = 0; k <= K; k++) {
+        const int n0 = (k > 2 ? k-2 : 0);
+        Real acc = Real(OverlapTaylor::v(ORDER, k));
+#pragma unroll
+        for (int n = ORDER-1; n >= n0; n--)
+            acc = acc*t + Real(OverlapTaylor::v(n, k));
+        c[k] = (n0 > 0 ? acc*tp[n0] : acc);
+    }
+    P = c[K];
+    dP = 0;
+#pragma unroll
+    for (int k = K-1; k >= 0; k--) {
+        dP = dP*x + P;
+        P = P*x + c[k];
+    }
+}
+
 // Slater overlap S(r; a, b) and dS/dr.  ia, ib are 1/a, 1/b.  For the InteraxOrder2 scheme a must be the width of
 // the particle with the lower index.
 template <typename Real>
@@ -74,7 +109,6 @@ __device__ __forceinline__ void slater_overlap(Real r, Real invR, Real a, Real i
     // Taylor expansion about l (the smaller width, or a).
     bool aSmaller = (a <= b);
     bool aboutA = (scheme.about_smaller ? aSmaller : true);
-    Real l = (aboutA ? a : b);
     Real invL = (aboutA ? ia : ib);
     Real expL = (aboutA ? expA : expB);
     Real delta = (aboutA ? b-a : a-b);
@@ -86,23 +120,14 @@ __device__ __forceinline__ void slater_overlap(Real r, Real invR, Real a, Real i
 
     Real taylorS = 0, taylorD = 0;
     if (__builtin_expect(useTaylor, 1)) {
-        Real sumV = 0, sumD = 0;
-#pragma unroll
-        for (int n = OverlapTaylor::ORDER; n >= 0; n--) {
-            if (n > scheme.order)
-                continue;
-            Real value = 0, dv = 0;
-#pragma unroll
-            for (int k = n+2; k >= 0; k--) {
-                value = value*x + Real(OverlapTaylor::v(n, k));
-                dv = dv*x + Real(OverlapTaylor::d(n, k));
-            }
-            sumV = sumV*t + value;
-            sumD = sumD*t + dv;
-        }
+        Real P, dP;
+        if (scheme.order == 2)
+            taylor_polynomial<Real, 2>(t, x, P, dP);
+        else
+            taylor_polynomial<Real, 6>(t, x, P, dP);
         Real prefactor = expL*invL*invL*invL*Real(INV_PI);
-        taylorS = prefactor*sumV;
-        taylorD = prefactor*invL*sumD;
+        taylorS = prefactor*P;
+        taylorD = prefactor*invL*(dP-P);
     }
     Real closedS = 0, closedD = 0;
     if (!useTaylor) {
@@ -121,40 +146,74 @@ __device__ __forceinline__ void slater_overlap(Real r, Real invR, Real a, Real i
     deriv = (useTaylor ? taylorD : closedD);
 }
 
+// Number of terms of the Tang-Toennies series tail sum_{k>10} x^k/k! evaluated below x = 5.5: the reference sums to
+// k = 40 (InteraxPairTerms.h:240-244); in FP32 the terms beyond k = 26 change the tail by less than 2e-9 relative.
+template <typename Real>
+struct TangToenniesTail {
+    static constexpr int TERMS = 30;
+};
+template <>
+struct TangToenniesTail<float> {
+    static constexpr int TERMS = 16;
+};
+
+// 10!/(10+m)!, the coefficient of x^m in tail/(x^10/10!)
+__host__ __device__ constexpr double tail_coefficient(int m) {
+    double c = 1;
+    for (int j = 1; j <= m; j++)
+        c /= (10+j);
+    return c;
+}
+
+template <typename Real, int M, int K>
+struct TailPolynomial {   // sum_{m=K}^{M} c_m x^(m-K)
+    static constexpr double C = tail_coefficient(K);
+    __device__ static __forceinline__ Real eval(Real x) { return Real(C) + x*TailPolynomial<Real, M, K+1>::eval(x); }
+};
+template <typename Real, int M>
+struct TailPolynomial<Real, M, M> {
+    static constexpr double C = tail_coefficient(M);
+    __device__ static __forceinline__ Real eval(Real) { return Real(C); }
+};
+
 // Tang-Toennies f_6, f_8, f_10 and df/dx (InteraxPairTerms.h:223-253).  Below x = (n+1)/2 the closed form
 // 1 - exp(-x) sum_{k<=n} x^k/k! cancels and f_n = exp(-x) sum_{k>n} x^k/k! is used, a sum of positive terms.
 template <typename Real>
 __device__ __forceinline__ void tang_toennies(Real x, Real f[3], Real df[3]) {
-    Real expX = fast_exp2(-x*Real(LOG2E));
-    Real terms[11];
-    terms[0] = 1;
-#pragma unroll
-    for (int k = 1; k <= 10; k++)
-        terms[k] = terms[k-1]*x*Real(1.0/k);
-    Real s6 = (((((terms[6]+terms[5])+terms[4])+terms[3])+terms[2])+terms[1])+terms[0];
-    Real s8 = s6+terms[7]+terms[8];
-    Real s10 = s8+terms[9]+terms[10];
+    const Real expX = fast_exp2(-x*Real(LOG2E));
+    // s6 = sum_{k<=6} x^k/k! in Horner form; t_n = x^n/n!
+    Real s6 = Real(1.0/720);
+    s6 = s6*x + Real(1.0/120);
+    s6 = s6*x + Real(1.0/24);
+    s6 = s6*x + Real(1.0/6);
+    s6 = s6*x + Real(0.5);
+    s6 = s6*x + Real(1);
+    s6 = s6*x + Real(1);
+    const Real x2 = x*x;
+    const Real x3 = x2*x;
+    const Real t6 = x3*x3*Real(1.0/720);
+    const Real t7 = t6*x*Real(1.0/7);
+    const Real t8 = t7*x*Real(1.0/8);
+    const Real t9 = t8*x*Real(1.0/9);
+    const Real t10 = t9*x*Real(1.0/10);
+    const Real s8 = s6+(t7+t8);
+    const Real s10 = s8+(t9+t10);
     f[0] = 1-expX*s6;
     f[1] = 1-expX*s8;
     f[2] = 1-expX*s10;
     if (x < Real(5.5)) {
-        Real term = terms[10], tail10 = 0;
-#pragma unroll 6
-        for (int k = 11; k <= 40; k++) {
-            term *= x*Real(1.0/k);
-            tail10 += term;
-        }
-        Real tail8 = tail10+terms[9]+terms[10];
-        Real tail6 = tail8+terms[7]+terms[8];
+        const Real tail10 = t10*x*TailPolynomial<Real, TangToenniesTail<Real>::TERMS, 1>::eval(x);
+        const Real tail8 = tail10+(t9+t10);
+        const Real tail6 = tail8+(t7+t8);
         f[2] = expX*tail10;
         if (x < Real(4.5))
             f[1] = expX*tail8;
         if (x < Real(3.5))
             f[0] = expX*tail6;
     }
-    df[0] = expX*terms[6];
-    df[1] = expX*terms[8];
-    df[2] = expX*terms[10];
+    df[0] = expX*t6;
+    df[1] = expX*t8;
+    df[2] = expX*t10;
 }
 
 // Per-particle parameters of a fused pair force, in the arithmetic type.
@@ -237,6 +296,127 @@ __device__ __forceinline__ void pair_energy(Real r, Real invR, const PairParticl
             Real ll = Real(0.5)*(pi.l+pj.l);
             energy += (kk+ll*st)*st;
             dEdR += (kk+Real(2)*ll*st)*norm*dss;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Type tables.  Force fields assign parameters per atom type, so the pair quantities that depend only on the two
+// types -- the combined C6/C8/C10 and every r-independent constant of a Slater overlap -- are computed once on
+// the host in FP64 and read from LDS.  Used in FP32 when the widths and dispersion parameters fall into few
+// classes; the per-pair path above handles everything else.
+// ------------------------------------------------------------------------------------------------------------
+
+constexpr int MAX_DISPERSION_CLASSES = 16;
+constexpr int MAX_WIDTH_CLASSES = 8;
+
+// One ordered pair of Slater widths (a, b): everything of S(r; a, b) that does not depend on r.  For the
+// InteraxOrder2 scheme a is the width of the lower-index particle; the Exact scheme is symmetric in a and b.
+struct alignas(16) OverlapRow {
+    float c[9];            // Taylor coefficients c_k(t) of x^k, x = r/l
+    float invL, pref;        // 1/l and 1/(pi l^3)
+    float kTaylor, txLimit;  // the Taylor branch is taken where r*kTaylor < txLimit
+    float ka, kb;            // log2(e)/a and log2(e)/b
+    float lIsA;              // 1 if the expansion is about a
+    float cc, cd;            // 4 a^2 b^2/(8 pi D^3) and 1/(8 pi D^2), D = b^2 - a^2
+    float a, b, ia, ib;
+    float norm;              // 8 pi (a+b)^3, the short-range normalisation
+    float gate;              // 0 if a or b is zero: the pair has no such term
+};
+static_assert(sizeof(OverlapRow) == 24*sizeof(float), "OverlapRow layout");
+constexpr int OVERLAP_ROW_FLOATS = 24;
+
+// Per-particle values the tables do not cover (FP32): dispersion width, exchange amplitude, K/2 and L/2.
+struct TableParticle {
+    float wd, amp, kh, lh;
+    int cls;                 // dispersion class | exchange width class << 8 | short-range width class << 16
+};
+
+// Pointers into LDS and the sizes of the tables of a launch.
+struct PairTables {
+    const float4* disp;      // numDisp^2 rows: c6, c8, c10, gate
+    const OverlapRow* exch;  // numExch^2 rows
+    const OverlapRow* sr;    // numSr^2 rows (== exch when the overlap is shared)
+    int numDisp, numExch, numSr;
+    float beta2;             // 2 * damping scale
+    int shared;
+};
+
+// S and dS/dr from a table row.
+__device__ __forceinline__ void overlap_from_row(const OverlapRow& w, float r, float invR, float& S, float& dS) {
+    const float eA = fast_exp2(-r*w.ka);
+    const float eB = fast_exp2(-r*w.kb);
+    if (r*w.kTaylor < w.txLimit) {
+        const float x = r*w.invL;
+        float P = w.c[8], dP = 0;
+#pragma unroll
+        for (int k = 7; k >= 0; k--) {
+            dP = dP*x + P;
+            P = P*x + w.c[k];
+        }
+        const float pre = (w.lIsA != 0.0f ? eA : eB)*w.pref;
+        S = pre*P;
+        dS = pre*w.invL*(dP-P);
+    }
+    else {
+        const float sum = w.a*eA + w.b*eB;
+        const float f = w.cc*(eA-eB) + r*w.cd*sum;
+        const float dfdr = w.cc*(eB*w.ib-eA*w.ia) + w.cd*(sum-r*(eA+eB));
+        S = f*invR;
+        dS = (r*dfdr-f)*invR*invR;
+    }
+}
+
+template <int TERMS>
+__device__ __forceinline__ void pair_energy_table(float r, float invR, const TableParticle& pi, const TableParticle& pj, bool iFirst,
+                                                  const PairTables& tb, float& energy, float& dEdR) {
+    energy = 0;
+    dEdR = 0;
+    const int di = pi.cls & 0xff, dj = pj.cls & 0xff;
+    if (TERMS & IXH_TERM_DISPERSION) {
+        const float4 row = tb.disp[di*tb.numDisp+dj];
+        if (row.w != 0.0f) {
+            const float dxdr = tb.beta2*fast_rcp(pi.wd+pj.wd);
+            float f[3], df[3];
+            tang_toennies<float>(dxdr*r, f, df);
+            const float invR2 = invR*invR;
+            const float e6 = row.x*invR2*invR2*invR2;
+            const float e8 = row.y*invR2*invR2*invR2*invR2;
+            const float e10 = row.z*invR2*invR2*invR2*invR2*invR2;
+            energy -= f[0]*e6 + f[1]*e8 + f[2]*e10;
+            dEdR -= e6*(df[0]*dxdr - 6.0f*f[0]*invR) + e8*(df[1]*dxdr - 8.0f*f[1]*invR) +
+                    e10*(df[2]*dxdr - 10.0f*f[2]*invR);
+        }
+    }
+    constexpr bool EXCH = (TERMS & IXH_TERM_EXCHANGE) != 0;
+    constexpr bool SR = (TERMS & IXH_TERM_SHORT_RANGE) != 0;
+    float sx = 0, dsx = 0;
+    if (EXCH) {
+        const int xi = (pi.cls >> 8) & 0xff, xj = (pj.cls >> 8) & 0xff;
+        const OverlapRow& w = tb.exch[iFirst ? xi*tb.numExch+xj : xj*tb.numExch+xi];
+        if (w.gate != 0.0f) {
+            overlap_from_row(w, r, invR, sx, dsx);
+            const float amp = pi.amp*pj.amp;
+            energy += amp*sx;
+            dEdR += amp*dsx;
+        }
+    }
+    if (SR) {
+        const int si = (pi.cls >> 16) & 0xff, sj = (pj.cls >> 16) & 0xff;
+        const OverlapRow& w = tb.sr[iFirst ? si*tb.numSr+sj : sj*tb.numSr+si];
+        if (w.gate != 0.0f) {
+            float ss, dss;
+            if (EXCH && tb.shared) {
+                ss = sx;
+                dss = dsx;
+            }
+            else
+                overlap_from_row(w, r, invR, ss, dss);
+            const float st = w.norm*ss;
+            const float kk = pi.kh+pj.kh;
+            const float ll = pi.lh+pj.lh;
+            energy += (kk+ll*st)*st;
+            dEdR += (kk+2.0f*ll*st)*w.norm*dss;
         }
     }
 }
