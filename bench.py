@@ -133,6 +133,8 @@ def bench_pair(args):
     pairs = ctypes.c_longlong()
     ih._check(lib.ixh_pair_last_pair_count(group.handle, ctypes.byref(pairs)))
     pairs = pairs.value
+    tiles, candidates = ctypes.c_longlong(), ctypes.c_longlong()
+    ih._check(lib.ixh_pair_list_statistics(group.handle, ctypes.byref(tiles), ctypes.byref(candidates)))
     value = pairs*args.steps/elapsed
     avg_kernel_s = kernel_ms.value/1e3/max(1, launches.value)
     achieved = FLOPS_PER_PAIR*pairs/avg_kernel_s/1e12
@@ -145,7 +147,8 @@ def bench_pair(args):
         "config": {"workload": "plain pair kernel at scale: 32768 water-like molecules (98304 atoms), 9.94 nm periodic box, "
                                "cutoff 1.0 nm, long range correction, dispersion+exchange+short range fused, coordinates "
                                "perturbed by N(0, 0.005 nm) per step",
-                   "atoms": n, "pairs_per_step": pairs, "precision": "mixed (FP64 positions, FP32 arithmetic, "
+                   "atoms": n, "pairs_per_step": pairs, "tiles": tiles.value, "candidate_pairs": candidates.value,
+                   "precision": "mixed (FP64 positions, FP32 arithmetic, "
                                                                      "64-bit fixed-point forces, FP64 energy)"},
         "roofline": {"bound": "mfma", "note": "compute-bound: FP32 VALU (no MFMA on this path) against the MI355X FP32 "
                                               "peak; 240 algorithmic flops per interacting pair",

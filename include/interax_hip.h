@@ -173,6 +173,9 @@ int ixh_pair_long_range_coefficients(ixh_pair* pair, double* coefficient, double
 int ixh_pair_long_range_energy(ixh_pair* pair, const double* box, double lambda, double* energy);
 /* Number of interacting pairs of the last execute() (waits for it). */
 int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
+/* Size of the current neighbour list (no counterpart in the reference): the number of 64 x 64 tiles and of candidate
+   pairs in them (within the cutoff plus the list skin, exclusions removed).  Waits for the last execute(). */
+int ixh_pair_list_statistics(ixh_pair* pair, long long* tiles, long long* candidates);
 
 /*
  * Measurement (no counterpart in the reference): with profiling enabled, every launch of the pair kernel (and of

@@ -400,7 +400,7 @@ struct MaskArgs {
     double listCutoff2;
 };
 
-__global__ void k_build_masks(MaskArgs a, unsigned long long* masks) {
+__global__ void k_build_masks(MaskArgs a, unsigned long long* masks, int* tileSlots) {
     __shared__ int sAtom[BLOCK];
     __shared__ double sPos[BLOCK][3];
     int t = blockIdx.x;
@@ -408,6 +408,7 @@ __global__ void k_build_masks(MaskArgs a, unsigned long long* masks) {
     int2 tile = a.tiles[t];
     int jslot = (tile.y < 0 ? tile.x*BLOCK+lane : a.partners[tile.y+lane]);
     int ja = (jslot < 0 ? -1 : a.perm[jslot]);
+    tileSlots[(size_t) t*BLOCK+lane] = jslot;
     sAtom[lane] = ja;
     if (ja >= 0) {
         sPos[lane][0] = a.pos3[3*ja];
@@ -436,6 +437,35 @@ __global__ void k_build_masks(MaskArgs a, unsigned long long* masks) {
         }
     }
     masks[(size_t) t*BLOCK+lane] = m;
+}
+
+// Candidate list of a tile in rotation order: for s = 0..63 the pairs (i, (i+s) % 64) whose mask bit is set, so that
+// consecutive entries have distinct i and distinct j.  Within the diagonal tile s runs over 1..32 (s = 32 for
+// i < 32 only), which meets every unordered pair once.  Entries are i << 6 | j; each tile's list is padded with
+// 0xffff to a multiple of 4 entries.  With offsets == nullptr only the counts (before padding) are written.
+__global__ void k_candidate_lists(const int2* tiles, const unsigned long long* masks, int* counts, const int* offsets,
+                                  unsigned short* lists) {
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const bool diag = (tiles[t].y < 0);
+    const unsigned long long m = masks[(size_t) t*BLOCK+lane];
+    const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
+    const int start = (offsets != nullptr ? offsets[t] : 0);
+    int count = 0;
+    for (int s = sBegin; s < sEnd; s++) {
+        const int j = (lane+s) & 63;
+        const bool in = ((m >> j) & 1ULL) != 0 && !(diag && s == 32 && lane >= 32);
+        const unsigned long long ballot = __ballot(in);
+        if (offsets != nullptr && in)
+            lists[start+count+__popcll(ballot & ((1ULL << lane)-1))] = (unsigned short) ((lane << 6) | j);
+        count += __popcll(ballot);
+    }
+    if (offsets == nullptr) {
+        if (lane == 0)
+            counts[t] = count;
+    }
+    else if (lane < ((count+3) & ~3)-count)
+        lists[start+count+lane] = 0xffff;
 }
 
 // Largest squared displacement since the list was built exceeds (skin/2)^2?
@@ -496,7 +526,10 @@ struct TileArgs {
     const int2* tiles;       // (block I, offset of the partner chunk, or -1 for the diagonal tile)
     int numTiles;
     const int* partners;
+    const int* tileSlots;    // the partner slot of every lane of every tile (-1 for padding)
     const unsigned long long* masks;
+    const int* listOffsets;  // candidate list of tile t: lists[listOffsets[t] .. listOffsets[t+1]), padded to 4 entries
+    const unsigned short* lists;
     const R4* local;
     const double* centers;
     const R4* pA;    // m1, m2, m3, alpha
@@ -516,6 +549,7 @@ struct TileArgs {
     int pairImage;   // the minimum image of each pair must be taken (small boxes)
     BoxD box;
     Real boxR[9];   // ax, bx, by, cx, cy, cz, 1/ax, 1/by, 1/cz
+    double boxInv[3];   // 1/ax, 1/by, 1/cz
     Real cutoff2;
     double cutoff;
     Real band;      // |r^2 - rc^2| < band*rc^2 is decided exactly in FP64
@@ -626,7 +660,8 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const Pai
     const Real invR = fast_rsqrt(r2);
     const Real r = r2*invR;
     Real rEff = r, invREff = invR;
-    if (a.useSoftcore && alchPair) {
+    const bool softcore = a.useSoftcore && alchPair;
+    if (softcore) {
         Real r6 = r2*r2*r2;
         rEff = pow(r6+a.softcoreTerm, Real(1.0/6.0));
         invREff = Real(1)/rEff;
@@ -637,7 +672,7 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const Pai
         pair_energy_table<TERMS>(rEff, invREff, P, Q, iFirst, tb, e, dEdR);
     else
         pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
-    if (rEff != r) {
+    if (softcore && rEff != r) {
         Real ratio = r*invREff;
         Real ratio2 = ratio*ratio;
         dEdR *= ratio2*ratio2*ratio;
@@ -815,7 +850,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
 // then evaluated with all lanes busy, whatever the spread of neighbour counts between the particles of the tile.
 // The forces on i and j are added in LDS as 64-bit fixed point (the j side gets the exact negation of the i side's
 // value) and flushed to the force buffer per tile (j) and per block (i).
-constexpr int QUEUE = 128;   // pending pairs never exceed 63 + 64
+constexpr int QUEUE = 512;   // pending pairs never exceed 63 + 256
 
 template <typename Real, bool TABLES>
 struct QueueLds {
@@ -856,6 +891,69 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
     atomicAdd(&q.fJ[2][jl], (unsigned long long) -fz);
 }
 
+// One lane's partner particle of a tile, loaded from global memory one tile ahead of its use.
+template <typename Real, bool TABLES>
+struct PartnerFetch {
+    typedef typename Vec4T<Real>::type R4;
+    R4 pos, A, B, C;
+    int cls, flags, block;
+    double cx, cy, cz;   // origin of the partner's block frame
+};
+
+template <typename Real, bool TABLES>
+__device__ __forceinline__ void fetch_partner(const TileArgs<Real>& a, int slot, PartnerFetch<Real, TABLES>& f) {
+    if (slot < 0) {
+        f.flags = -1;
+        f.block = -1;
+        return;
+    }
+    f.block = slot/BLOCK;
+    f.pos = a.local[slot];
+    f.A = a.pA[slot];
+    if constexpr (TABLES)
+        f.cls = a.cls[slot];
+    else {
+        f.B = a.pB[slot];
+        f.C = a.pC[slot];
+    }
+    f.flags = a.flags[slot];
+    f.cx = a.centers[3*f.block];
+    f.cy = a.centers[3*f.block+1];
+    f.cz = a.centers[3*f.block+2];
+}
+
+// Write a prefetched partner into LDS, moved into the frame of block I (origin cI).
+template <typename Real, bool TABLES>
+__device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const PartnerFetch<Real, TABLES>& f, int I, double cIx, double cIy,
+                                              double cIz, int lane, SideLds<Real, TABLES>& s) {
+    typedef typename Vec4T<Real>::type R4;
+    if (f.flags < 0) {
+        R4 zero = {0, 0, 0, 0};
+        s.pos[lane] = zero;
+        s.flags[lane] = -1;
+        return;
+    }
+    R4 pj = f.pos;
+    if (f.block != I) {
+        double sx = f.cx-cIx, sy = f.cy-cIy, sz = f.cz-cIz;
+        if (a.periodic)
+            min_image<double>(sx, sy, sz, a.box.ax, a.box.bx, a.box.by, a.box.cx, a.box.cy, a.box.cz, a.boxInv[0], a.boxInv[1],
+                              a.boxInv[2]);
+        pj.x += Real(sx);
+        pj.y += Real(sy);
+        pj.z += Real(sz);
+    }
+    s.pos[lane] = pj;
+    s.A[lane] = f.A;
+    if constexpr (TABLES)
+        s.cls[lane] = f.cls;
+    else {
+        s.B[lane] = f.B;
+        s.C[lane] = f.C;
+    }
+    s.flags[lane] = f.flags;
+}
+
 template <typename Real, int TERMS, bool TABLES>
 __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
@@ -877,10 +975,14 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
     int curI = -1;
     R4 pi = {0, 0, 0, 0};
     int flagI = -1;
+    double cIx = 0, cIy = 0, cIz = 0;
     double energy = 0;
     int pairs = 0;
+
     for (long long t = begin; t < end; t++) {
         const int2 tile = a.tiles[t];
+        PartnerFetch<Real, TABLES> f;
+        fetch_partner<Real, TABLES>(a, a.tileSlots[(size_t) t*BLOCK+lane], f);
         const int I = tile.x;
         if (I != curI) {
             if (curI >= 0 && flagI >= 0) {
@@ -893,43 +995,57 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
             const int slot = I*BLOCK+lane;
             pi = a.local[slot];
             flagI = a.flags[slot];
+            cIx = a.centers[3*I];
+            cIy = a.centers[3*I+1];
+            cIz = a.centers[3*I+2];
             q.i.pos[lane] = pi;
             stage_particle<Real, TABLES>(q.i, lane, a, slot);
             q.fI[0][lane] = q.fI[1][lane] = q.fI[2][lane] = 0;
         }
-        stage_partners<Real, TABLES>(a, tile, lane, q.j);
+        store_partner<Real, TABLES>(a, f, I, cIx, cIy, cIz, lane, q.j);
         q.fJ[0][lane] = q.fJ[1][lane] = q.fJ[2][lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const unsigned long long mrow = a.masks[(size_t) t*BLOCK+lane];
-        const bool diag = (tile.y < 0);
-        const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
+        // The tile's candidate list, 256 entries (4 per lane) per step with the next step's entries in flight: each
+        // entry in range is pushed into the queue, and full batches of 64 queued pairs are evaluated.
+        const int cBegin = a.listOffsets[t], cEnd = a.listOffsets[t+1];
         int head = 0, count = 0;
-        for (int s = sBegin; s < sEnd; s++) {
-            const int jl = (lane+s) & 63;
-            bool in = ((mrow >> jl) & 1ULL) != 0 && !(diag && s == 32 && lane >= 32);
-            if (in) {
-                const R4 pj = q.j.pos[jl];
-                Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
-                if (a.pairImage)
-                    min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
-                                    a.boxR[7], a.boxR[8]);
-                const Real r2 = dx*dx+dy*dy+dz*dz;
-                if (r2 >= inner)
-                    in = (r2 < outer) && exact_within_cutoff(a.posq, a.correction, a.precision, a.periodic, a.box, a.cutoff,
-                                                             flagI & INDEX_MASK, q.j.flags[jl] & INDEX_MASK);
+        uint2 chunk = make_uint2(0xffffffffu, 0xffffffffu), next = chunk;
+        if (cBegin+4*lane < cEnd)
+            chunk = *(const uint2*) (a.lists+cBegin+4*lane);
+        for (int base = cBegin; base < cEnd; base += 4*BLOCK) {
+            const int nextBase = base+4*BLOCK;
+            next = make_uint2(0xffffffffu, 0xffffffffu);
+            if (nextBase+4*lane < cEnd)
+                next = *(const uint2*) (a.lists+nextBase+4*lane);
+#pragma unroll
+            for (int sub = 0; sub < 4; sub++) {
+                const unsigned word = (sub < 2 ? chunk.x : chunk.y);
+                const unsigned e = ((sub & 1) ? (word >> 16) : word) & 0xffffu;
+                bool in = (e != 0xffffu);
+                if (in) {
+                    const int il = (int) (e >> 6), jl = (int) (e & 63);
+                    const R4 pa = q.i.pos[il], pb = q.j.pos[jl];
+                    Real dx = pb.x-pa.x, dy = pb.y-pa.y, dz = pb.z-pa.z;
+                    if (a.pairImage)
+                        min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
+                                        a.boxR[7], a.boxR[8]);
+                    const Real r2 = dx*dx+dy*dy+dz*dz;
+                    if (r2 >= inner)
+                        in = (r2 < outer) && exact_within_cutoff(a.posq, a.correction, a.precision, a.periodic, a.box, a.cutoff,
+                                                                 q.i.flags[il] & INDEX_MASK, q.j.flags[jl] & INDEX_MASK);
+                }
+                const unsigned long long ballot = __ballot(in);
+                if (in)
+                    q.queue[(head+count+__popcll(ballot & ((1ULL << lane)-1))) & (QUEUE-1)] = (unsigned short) e;
+                count += __popcll(ballot);
             }
-            const unsigned long long ballot = __ballot(in);
-            if (in) {
-                const int rank = __popcll(ballot & ((1ULL << lane)-1));
-                q.queue[(head+count+rank) & (QUEUE-1)] = (unsigned short) ((lane << 6) | jl);
-            }
-            count += __popcll(ballot);
-            if (count >= BLOCK) {
+            while (count >= BLOCK) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy, pairs);
                 head = (head+BLOCK) & (QUEUE-1);
                 count -= BLOCK;
             }
+            chunk = next;
         }
         if (count > 0) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1135,7 +1251,9 @@ struct ixh_pair {
     // parameters in particle order and in slot order
     DeviceArray<char> paramA, paramB, paramC, sortedA, sortedB, sortedC, local;
     DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionOffsets, exclusionPartners, blockStart, blockCount, blockList,
-        partnerCount, partnerStart, partners, waveCount;
+        partnerCount, partnerStart, partners, waveCount, tileSlots, tileCandidates, listOffsets;
+    DeviceArray<unsigned short> lists;
+    long long numCandidates = 0;
     // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
     bool useTables = false;
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
@@ -1231,47 +1349,67 @@ OverlapScheme make_scheme(int method, bool doublePrecision) {
     return s;
 }
 
-// One row of an overlap table, in FP64 rounded once to FP32.  Same branch rule as slater_overlap: the Taylor
-// branch where |delta| < maxDelta, or |t| < maxT and |t| x < maxTX, written as r |t|/l < maxTX.
-OverlapRow overlap_row(double a, double b, const OverlapScheme& s) {
-    OverlapRow w;
-    std::memset(&w, 0, sizeof(w));
+// The two rows of an ordered width pair (a, b) of an overlap table, in FP64 rounded once to FP32: the Taylor
+// representation, then the closed form.  The branch rule is that of slater_overlap: Taylor where |delta| <
+// maxDelta, or |t| < maxT and |t| x < maxTX, written here as r |t|/l < maxTX.
+void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]) {
+    std::memset(rows, 0, 2*sizeof(OverlapRow));
     if (a == 0 || b == 0)
-        return w;
-    double l = (s.about_smaller ? std::min(a, b) : a);
-    double delta = a+b-l-l;
-    double t = delta/l;
+        return;   // gate 0: no such term for the pair
+    const double l = (s.about_smaller ? std::min(a, b) : a);
+    const double delta = a+b-l-l;
+    const double t = delta/l;
+    float kTaylor, txLimit;
+    bool taylorUsed = true, closedUsed = true;
     if (std::fabs(delta) < s.max_delta) {
-        w.kTaylor = 0;
-        w.txLimit = 1;
+        kTaylor = 0;
+        txLimit = 1;
+        closedUsed = false;
     }
     else if (std::fabs(t) < s.max_t) {
-        w.kTaylor = (float) (std::fabs(t)/l);
-        w.txLimit = (float) s.max_tx;
+        kTaylor = (float) (std::fabs(t)/l);
+        txLimit = (float) s.max_tx;
     }
     else {
-        w.kTaylor = 0;
-        w.txLimit = 0;
+        kTaylor = 0;
+        txLimit = 0;
+        taylorUsed = false;
     }
-    for (int k = 0; k <= s.order+2; k++)
-        w.c[k] = (float) OverlapTaylor::coefficient(s.order, k, t);
-    w.invL = (float) (1/l);
-    w.pref = (float) (INV_PI/(l*l*l));
-    w.ka = (float) (LOG2E/a);
-    w.kb = (float) (LOG2E/b);
-    w.lIsA = (l == a ? 1.0f : 0.0f);
-    double D = b*b-a*a;
+    OverlapRow taylor, closed;
+    std::memset(&taylor, 0, sizeof(taylor));
+    std::memset(&closed, 0, sizeof(closed));
+    // exp(-r/l)/(pi l^3) sum_k c_k(t) (r/l)^k
+    taylor.k1 = taylor.k2 = (float) (LOG2E/l);
+    taylor.n1 = taylor.n2 = (float) (1/l);
+    double scale = INV_PI/(l*l*l);
+    for (int k = 0; k <= s.order+2; k++, scale /= l)
+        taylor.p[k] = (float) (scale*OverlapTaylor::coefficient(s.order, k, t));
+    const double D = b*b-a*a;
     if (D != 0) {
-        w.cc = (float) (4*a*a*b*b/(D*D*D)*INV_8PI);
-        w.cd = (float) (INV_8PI/(D*D));
+        const double m1 = 4*a*a*b*b/(D*D*D)*INV_8PI;
+        closed.k1 = (float) (LOG2E/a);
+        closed.n1 = (float) (1/a);
+        closed.k2 = (float) (LOG2E/b);
+        closed.n2 = (float) (1/b);
+        closed.m1 = (float) m1;
+        closed.p[0] = (float) (a*INV_8PI/(D*D));
+        closed.q1 = (float) -m1;
+        closed.q0 = (float) (b*INV_8PI/(D*D));
     }
-    w.a = (float) a;
-    w.b = (float) b;
-    w.ia = (float) (1/a);
-    w.ib = (float) (1/b);
-    w.norm = (float) (EIGHT_PI*(a+b)*(a+b)*(a+b));
-    w.gate = 1;
-    return w;
+    else
+        closedUsed = false;   // equal widths are always expanded
+    if (!taylorUsed)
+        taylor = closed;
+    if (!closedUsed)
+        closed = taylor;
+    rows[0] = taylor;
+    rows[1] = closed;
+    for (int k = 0; k < 2; k++) {
+        rows[k].kTaylor = kTaylor;
+        rows[k].txLimit = txLimit;
+        rows[k].norm = (float) (EIGHT_PI*(a+b)*(a+b)*(a+b));
+        rows[k].gate = 1;
+    }
 }
 
 // Parameter classes and type tables.  Returns false, leaving the per-pair path in use, when the particles fall into
@@ -1327,9 +1465,9 @@ bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls
     p->numExch = std::max<int>(1, (int) exchWidths.size());
     p->numSr = (shared ? p->numExch : std::max<int>(1, (int) srWidths.size()));
     p->exchOffset = 4*p->numDisp*p->numDisp;
-    int exchFloats = OVERLAP_ROW_FLOATS*p->numExch*p->numExch;
+    int exchFloats = OVERLAP_PAIR_FLOATS*p->numExch*p->numExch;
     p->srOffset = (shared ? p->exchOffset : p->exchOffset+exchFloats);
-    p->tableFloats = p->srOffset+(shared ? exchFloats : OVERLAP_ROW_FLOATS*p->numSr*p->numSr);
+    p->tableFloats = p->srOffset+(shared ? exchFloats : OVERLAP_PAIR_FLOATS*p->numSr*p->numSr);
     tables.assign(p->tableFloats, 0.0f);
     // dispersion rows: C6, C8, C10 combined as in InteraxPairTerms.h:271-279, and the gate alpha_i alpha_j != 0
     for (int ci = 0; ci < (int) dispKeys.size(); ci++)
@@ -1346,14 +1484,16 @@ bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls
         }
     for (int ci = 0; ci < (int) exchWidths.size(); ci++)
         for (int cj = 0; cj < (int) exchWidths.size(); cj++) {
-            OverlapRow w = overlap_row(exchWidths[ci], exchWidths[cj], p->settings.exchange_scheme);
-            std::memcpy(&tables[p->exchOffset+OVERLAP_ROW_FLOATS*(ci*p->numExch+cj)], &w, sizeof(w));
+            OverlapRow w[2];
+            overlap_rows(exchWidths[ci], exchWidths[cj], p->settings.exchange_scheme, w);
+            std::memcpy(&tables[p->exchOffset+OVERLAP_PAIR_FLOATS*(ci*p->numExch+cj)], w, sizeof(w));
         }
     if (!shared)
         for (int ci = 0; ci < (int) srWidths.size(); ci++)
             for (int cj = 0; cj < (int) srWidths.size(); cj++) {
-                OverlapRow w = overlap_row(srWidths[ci], srWidths[cj], p->settings.short_range_scheme);
-                std::memcpy(&tables[p->srOffset+OVERLAP_ROW_FLOATS*(ci*p->numSr+cj)], &w, sizeof(w));
+                OverlapRow w[2];
+                overlap_rows(srWidths[ci], srWidths[cj], p->settings.short_range_scheme, w);
+                std::memcpy(&tables[p->srOffset+OVERLAP_PAIR_FLOATS*(ci*p->numSr+cj)], w, sizeof(w));
             }
     return true;
 }
@@ -1629,6 +1769,7 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     p->tiles.upload(tiles.data(), tiles.size(), stream);
     int T = p->numTiles;
     p->masks.reserve((size_t) std::max(T, 1)*BLOCK);
+    p->tileSlots.reserve((size_t) std::max(T, 1)*BLOCK);
     MaskArgs m;
     m.tiles = p->tiles.ptr;
     m.partners = p->partners.ptr;
@@ -1641,7 +1782,31 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     m.box = box;
     m.listCutoff2 = listCutoff*listCutoff;
     if (T > 0) {
-        hipLaunchKernelGGL(k_build_masks, dim3(T), dim3(BLOCK), 0, stream, m, p->masks.ptr);
+        hipLaunchKernelGGL(k_build_masks, dim3(T), dim3(BLOCK), 0, stream, m, p->masks.ptr, p->tileSlots.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
+    p->numCandidates = 0;
+    if (cutoff && T > 0) {
+        // candidate lists: count, lay out (each tile padded to a multiple of 4 entries), write
+        p->tileCandidates.reserve(T);
+        hipLaunchKernelGGL(k_candidate_lists, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->masks.ptr, p->tileCandidates.ptr,
+                           (const int*) nullptr, (unsigned short*) nullptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        std::vector<int> counts(T), offsets(T+1, 0);
+        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), p->tileCandidates.ptr, sizeof(int)*T, hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        long long total = 0;
+        for (int t = 0; t < T; t++) {
+            p->numCandidates += counts[t];
+            total += (counts[t]+3) & ~3;
+            if (total > INT32_MAX)
+                throw Error(IXH_ERROR_CAPACITY, "ixh_pair_execute: the candidate pair list exceeds 2^31 entries");
+            offsets[t+1] = (int) total;
+        }
+        p->listOffsets.upload(offsets.data(), offsets.size(), stream);
+        p->lists.reserve((size_t) std::max<long long>(total, 4));
+        hipLaunchKernelGGL(k_candidate_lists, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->masks.ptr, p->tileCandidates.ptr,
+                           (const int*) p->listOffsets.ptr, p->lists.ptr);
         IXH_HIP_CHECK(hipGetLastError());
     }
     if (cutoff) {
@@ -1659,7 +1824,10 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     a.tiles = p->tiles.ptr;
     a.numTiles = p->numTiles;
     a.partners = p->partners.ptr;
+    a.tileSlots = p->tileSlots.ptr;
     a.masks = p->masks.ptr;
+    a.listOffsets = p->listOffsets.ptr;
+    a.lists = p->lists.ptr;
     a.local = (const R4*) p->local.ptr;
     a.centers = p->centers.ptr;
     a.pA = (const R4*) p->sortedA.ptr;
@@ -1680,6 +1848,9 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     a.boxR[6] = Real(a.periodic ? 1.0/b->box[0] : 0.0);
     a.boxR[7] = Real(a.periodic ? 1.0/b->box[4] : 0.0);
     a.boxR[8] = Real(a.periodic ? 1.0/b->box[8] : 0.0);
+    a.boxInv[0] = (a.periodic ? 1.0/b->box[0] : 0.0);
+    a.boxInv[1] = (a.periodic ? 1.0/b->box[4] : 0.0);
+    a.boxInv[2] = (a.periodic ? 1.0/b->box[8] : 0.0);
     a.cutoff = p->cutoff;
     a.cutoff2 = Real(p->cutoff*p->cutoff);
     a.band = Real(sizeof(Real) == 4 ? 1e-5 : 1e-12);
@@ -1967,6 +2138,32 @@ int ixh_pair_last_pair_count(ixh_pair* p, long long* count) {
         DeviceGuard guard(p->device);
         IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
         IXH_HIP_CHECK(hipMemcpy(count, p->pairCount.ptr, sizeof(long long), hipMemcpyDeviceToHost));
+    });
+}
+
+int ixh_pair_list_statistics(ixh_pair* p, long long* tiles, long long* candidates) {
+    return guarded([&] {
+        if (p == nullptr || tiles == nullptr || candidates == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_list_statistics: null argument");
+        DeviceGuard guard(p->device);
+        if (p->lastStream != nullptr)
+            IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
+        *tiles = p->numTiles;
+        *candidates = 0;
+        if (!p->built || p->numTiles == 0)
+            return;
+        std::vector<unsigned long long> masks((size_t) p->numTiles*BLOCK);
+        std::vector<int2> tileList(p->numTiles);
+        IXH_HIP_CHECK(hipMemcpy(masks.data(), p->masks.ptr, sizeof(unsigned long long)*masks.size(), hipMemcpyDeviceToHost));
+        IXH_HIP_CHECK(hipMemcpy(tileList.data(), p->tiles.ptr, sizeof(int2)*tileList.size(), hipMemcpyDeviceToHost));
+        long long total = 0;
+        for (int t = 0; t < p->numTiles; t++) {
+            long long c = 0;
+            for (int l = 0; l < BLOCK; l++)
+                c += __builtin_popcountll(masks[(size_t) t*BLOCK+l]);
+            total += (tileList[t].y < 0 ? c/2 : c);
+        }
+        *candidates = total;
     });
 }
 

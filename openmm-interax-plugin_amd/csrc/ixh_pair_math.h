@@ -310,21 +310,25 @@ __device__ __forceinline__ void pair_energy(Real r, Real invR, const PairParticl
 constexpr int MAX_DISPERSION_CLASSES = 16;
 constexpr int MAX_WIDTH_CLASSES = 8;
 
-// One ordered pair of Slater widths (a, b): everything of S(r; a, b) that does not depend on r.  For the
-// InteraxOrder2 scheme a is the width of the lower-index particle; the Exact scheme is symmetric in a and b.
+// One representation of S(r; a, b) for an ordered pair of Slater widths, in a form shared by both branches:
+//   S(r) = e1(r) (m1/r + sum_{k<=8} p_k r^k) + e2(r) (q1/r + q0),   e_i(r) = exp(-r/w_i).
+// The Taylor branch about l (InteraxPairTerms.h:189-208) is e1 = exp(-r/l), p_k = c_k(t)/(pi l^3 l^k) and no e2 part;
+// the closed form (:176-187) is S = [4a^2b^2/D^3 (e_a - e_b) + r/D^2 (a e_a + b e_b)]/(8 pi r), i.e.
+// m1 = 4a^2b^2/(8 pi D^3), p_0 = a/(8 pi D^2), q1 = -m1, q0 = b/(8 pi D^2).  Every pair then runs the same
+// instructions whichever branch its widths select: no divergence between Taylor and closed-form pairs of a wave.
 struct alignas(16) OverlapRow {
-    float c[9];            // Taylor coefficients c_k(t) of x^k, x = r/l
-    float invL, pref;        // 1/l and 1/(pi l^3)
-    float kTaylor, txLimit;  // the Taylor branch is taken where r*kTaylor < txLimit
-    float ka, kb;            // log2(e)/a and log2(e)/b
-    float lIsA;              // 1 if the expansion is about a
-    float cc, cd;            // 4 a^2 b^2/(8 pi D^3) and 1/(8 pi D^2), D = b^2 - a^2
-    float a, b, ia, ib;
+    float kTaylor, txLimit;  // the Taylor row is used where r*kTaylor < txLimit, the closed-form row elsewhere
+    float k1, k2;            // log2(e)/w1, log2(e)/w2
+    float n1, n2;            // 1/w1, 1/w2
+    float m1;
+    float p[9];
+    float q1, q0;
     float norm;              // 8 pi (a+b)^3, the short-range normalisation
     float gate;              // 0 if a or b is zero: the pair has no such term
 };
-static_assert(sizeof(OverlapRow) == 24*sizeof(float), "OverlapRow layout");
-constexpr int OVERLAP_ROW_FLOATS = 24;
+static_assert(sizeof(OverlapRow) == 20*sizeof(float), "OverlapRow layout");
+// Per ordered width pair: the Taylor row, then the closed-form row (both identical when one branch is never taken).
+constexpr int OVERLAP_PAIR_FLOATS = 40;
 
 // Per-particle values the tables do not cover (FP32): dispersion width, exchange amplitude, K/2 and L/2.
 struct TableParticle {
@@ -335,36 +339,31 @@ struct TableParticle {
 // Pointers into LDS and the sizes of the tables of a launch.
 struct PairTables {
     const float4* disp;      // numDisp^2 rows: c6, c8, c10, gate
-    const OverlapRow* exch;  // numExch^2 rows
-    const OverlapRow* sr;    // numSr^2 rows (== exch when the overlap is shared)
+    const OverlapRow* exch;  // 2 numExch^2 rows
+    const OverlapRow* sr;    // 2 numSr^2 rows (== exch when the overlap is shared)
     int numDisp, numExch, numSr;
     float beta2;             // 2 * damping scale
     int shared;
 };
 
-// S and dS/dr from a table row.
-__device__ __forceinline__ void overlap_from_row(const OverlapRow& w, float r, float invR, float& S, float& dS) {
-    const float eA = fast_exp2(-r*w.ka);
-    const float eB = fast_exp2(-r*w.kb);
-    if (r*w.kTaylor < w.txLimit) {
-        const float x = r*w.invL;
-        float P = w.c[8], dP = 0;
+// S and dS/dr from the row pair of a width pair.
+__device__ __forceinline__ void overlap_from_rows(const OverlapRow* rows, float r, float invR, float& S, float& dS) {
+    const OverlapRow& w = rows[(r*rows[0].kTaylor < rows[0].txLimit) ? 0 : 1];
+    const float e1 = fast_exp2(-r*w.k1);
+    const float e2 = fast_exp2(-r*w.k2);
+    float P = w.p[8], dP = 0;
 #pragma unroll
-        for (int k = 7; k >= 0; k--) {
-            dP = dP*x + P;
-            P = P*x + w.c[k];
-        }
-        const float pre = (w.lIsA != 0.0f ? eA : eB)*w.pref;
-        S = pre*P;
-        dS = pre*w.invL*(dP-P);
+    for (int k = 7; k >= 0; k--) {
+        dP = dP*r + P;
+        P = P*r + w.p[k];
     }
-    else {
-        const float sum = w.a*eA + w.b*eB;
-        const float f = w.cc*(eA-eB) + r*w.cd*sum;
-        const float dfdr = w.cc*(eB*w.ib-eA*w.ia) + w.cd*(sum-r*(eA+eB));
-        S = f*invR;
-        dS = (r*dfdr-f)*invR*invR;
-    }
+    const float invR2 = invR*invR;
+    const float P1 = P + w.m1*invR;
+    const float dP1 = dP - w.m1*invR2;
+    const float P2 = w.q1*invR + w.q0;
+    const float dP2 = -w.q1*invR2;
+    S = e1*P1 + e2*P2;
+    dS = e1*(dP1 - w.n1*P1) + e2*(dP2 - w.n2*P2);
 }
 
 template <int TERMS>
@@ -372,17 +371,19 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
                                                   const PairTables& tb, float& energy, float& dEdR) {
     energy = 0;
     dEdR = 0;
-    const int di = pi.cls & 0xff, dj = pj.cls & 0xff;
     if (TERMS & IXH_TERM_DISPERSION) {
-        const float4 row = tb.disp[di*tb.numDisp+dj];
+        const int di = pi.cls & 0xff, dj = pj.cls & 0xff;
+        const float4 row = tb.disp[__umul24(di, tb.numDisp)+dj];
         if (row.w != 0.0f) {
             const float dxdr = tb.beta2*fast_rcp(pi.wd+pj.wd);
             float f[3], df[3];
             tang_toennies<float>(dxdr*r, f, df);
             const float invR2 = invR*invR;
-            const float e6 = row.x*invR2*invR2*invR2;
-            const float e8 = row.y*invR2*invR2*invR2*invR2;
-            const float e10 = row.z*invR2*invR2*invR2*invR2*invR2;
+            const float invR6 = invR2*invR2*invR2;
+            const float invR8 = invR6*invR2;
+            const float e6 = row.x*invR6;
+            const float e8 = row.y*invR8;
+            const float e10 = row.z*(invR8*invR2);
             energy -= f[0]*e6 + f[1]*e8 + f[2]*e10;
             dEdR -= e6*(df[0]*dxdr - 6.0f*f[0]*invR) + e8*(df[1]*dxdr - 8.0f*f[1]*invR) +
                     e10*(df[2]*dxdr - 10.0f*f[2]*invR);
@@ -393,9 +394,9 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     float sx = 0, dsx = 0;
     if (EXCH) {
         const int xi = (pi.cls >> 8) & 0xff, xj = (pj.cls >> 8) & 0xff;
-        const OverlapRow& w = tb.exch[iFirst ? xi*tb.numExch+xj : xj*tb.numExch+xi];
-        if (w.gate != 0.0f) {
-            overlap_from_row(w, r, invR, sx, dsx);
+        const OverlapRow* rows = tb.exch + 2*(iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi);
+        if (rows[0].gate != 0.0f) {
+            overlap_from_rows(rows, r, invR, sx, dsx);
             const float amp = pi.amp*pj.amp;
             energy += amp*sx;
             dEdR += amp*dsx;
@@ -403,20 +404,21 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     }
     if (SR) {
         const int si = (pi.cls >> 16) & 0xff, sj = (pj.cls >> 16) & 0xff;
-        const OverlapRow& w = tb.sr[iFirst ? si*tb.numSr+sj : sj*tb.numSr+si];
-        if (w.gate != 0.0f) {
+        const OverlapRow* rows = tb.sr + 2*(iFirst ? __umul24(si, tb.numSr)+sj : __umul24(sj, tb.numSr)+si);
+        if (rows[0].gate != 0.0f) {
             float ss, dss;
             if (EXCH && tb.shared) {
                 ss = sx;
                 dss = dsx;
             }
             else
-                overlap_from_row(w, r, invR, ss, dss);
-            const float st = w.norm*ss;
+                overlap_from_rows(rows, r, invR, ss, dss);
+            const float norm = rows[0].norm;
+            const float st = norm*ss;
             const float kk = pi.kh+pj.kh;
             const float ll = pi.lh+pj.lh;
             energy += (kk+ll*st)*st;
-            dEdR += (kk+2.0f*ll*st)*w.norm*dss;
+            dEdR += (kk+2.0f*ll*st)*norm*dss;
         }
     }
 }

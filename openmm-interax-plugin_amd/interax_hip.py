@@ -93,6 +93,7 @@ SIGNATURES = {
     "ixh_pair_long_range_coefficients": [ctypes.c_void_p, _dp, _dp],
     "ixh_pair_long_range_energy": [ctypes.c_void_p, _dp, ctypes.c_double, _dp],
     "ixh_pair_last_pair_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
+    "ixh_pair_list_statistics": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "ixh_threebody_create": [ctypes.c_int, ctypes.POINTER(ThreeBodyDesc), ctypes.POINTER(ctypes.c_void_p)],
     "ixh_threebody_destroy": [ctypes.c_void_p],
     "ixh_threebody_set_parameters": [ctypes.c_void_p, ctypes.POINTER(ThreeBodyDesc)],
