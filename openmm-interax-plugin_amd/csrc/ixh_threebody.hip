@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <rocprim/rocprim.hpp>
 #include <vector>
 
 using namespace ixh;
@@ -90,30 +91,103 @@ __global__ void k_sites(const void* posq, const float4* correction, int precisio
     sites[3*i+2] = __dmul_rn(s[2], scale);
 }
 
-// 2. Half neighbour lists: pass 0 counts, pass 1 writes (ascending order).
-__global__ void k_neighbors(const double* sites, const int* molecule, int numSites, int periodic, BoxT box, double cutoff2,
-                            int* count, const int* start, int* nbrIndex, double4* nbrDelta) {
+// 2. Half neighbour lists from a cell grid.  Each axis has either one cell (searched alone) or at least three cells
+// of at least the cutoff (the 27 neighbouring cells are searched, with periodic wrap or clipped at the grid's edge).
+struct CellGrid {
+    int n[3];
+    int periodic;
+    double origin[3];      // nonperiodic: lower corner of the grid
+    double scale[3];       // cells per nm along each axis (periodic: cells per box length, applied to x/L)
+};
+
+__device__ __forceinline__ int cell_coordinate(double x, int axis, const CellGrid& g) {
+    if (g.n[axis] == 1)
+        return 0;
+    int c;
+    if (g.periodic) {
+        double f = x*g.scale[axis];      // x / L
+        f -= floor(f);
+        c = (int) (f*g.n[axis]);
+    }
+    else
+        c = (int) ((x-g.origin[axis])*g.scale[axis]);
+    return min(max(c, 0), g.n[axis]-1);
+}
+
+__global__ void k_site_cells(const double* sites, int numSites, CellGrid g, unsigned* keys, int* values) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= numSites)
+        return;
+    int cx = cell_coordinate(sites[3*i], 0, g), cy = cell_coordinate(sites[3*i+1], 1, g), cz = cell_coordinate(sites[3*i+2], 2, g);
+    keys[i] = (unsigned) ((cz*g.n[1]+cy)*g.n[0]+cx);
+    values[i] = i;
+}
+
+__global__ void k_fill_int(int* p, int n, int value) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        p[i] = value;
+}
+
+// [cellStart, cellEnd) of every cell in the sorted order (empty cells keep 0, 0)
+__global__ void k_cell_ranges(const unsigned* sortedKeys, int numSites, int* cellStart, int* cellEnd) {
+    int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (s >= numSites)
+        return;
+    unsigned key = sortedKeys[s];
+    if (s == 0 || sortedKeys[s-1] != key)
+        cellStart[key] = s;
+    if (s == numSites-1 || sortedKeys[s+1] != key)
+        cellEnd[key] = s+1;
+}
+
+// One wave per site i: the sites j > i of another molecule within the cutoff (ReferenceInteraxThreeBodyIxn.cpp:97-103,
+// distance with the reference's rounding).  Pass 0 counts, pass 1 writes, in a fixed order (cells, then site index).
+__global__ void k_neighbors(const double* sites, const int* molecule, int numSites, int periodic, BoxT box, double cutoff2, CellGrid g,
+                            const int* cellStart, const int* cellEnd, const int* sortedSites, int* count, const int* start,
+                            int* nbrIndex, double4* nbrDelta) {
     int i = blockIdx.x;
     int lane = threadIdx.x;
     const double* si = sites+3*i;
     int mi = molecule[i];
-    int written = 0;
-    for (int j0 = i+1; j0 < numSites; j0 += 64) {
-        int j = j0+lane;
-        bool hit = false;
-        double d[3] = {0, 0, 0};
-        if (j < numSites && molecule[j] != mi) {
-            ref_delta(si, sites+3*j, periodic, box, d);
-            hit = ref_norm2(d) < cutoff2;
-        }
-        unsigned long long ballot = __ballot(hit);
-        if (start != nullptr && hit) {
-            int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
-            nbrIndex[pos] = j;
-            nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
-        }
-        written += __popcll(ballot);
+    int c[3] = {cell_coordinate(si[0], 0, g), cell_coordinate(si[1], 1, g), cell_coordinate(si[2], 2, g)};
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = (g.n[k] == 1 ? 0 : -1);
+        hi[k] = (g.n[k] == 1 ? 0 : 1);
     }
+    int written = 0;
+    for (int dz = lo[2]; dz <= hi[2]; dz++)
+        for (int dy = lo[1]; dy <= hi[1]; dy++)
+            for (int dx = lo[0]; dx <= hi[0]; dx++) {
+                int x = c[0]+dx, y = c[1]+dy, z = c[2]+dz;
+                if (g.periodic) {
+                    x = (x+g.n[0]) % g.n[0];
+                    y = (y+g.n[1]) % g.n[1];
+                    z = (z+g.n[2]) % g.n[2];
+                }
+                else if (x < 0 || y < 0 || z < 0 || x >= g.n[0] || y >= g.n[1] || z >= g.n[2])
+                    continue;
+                int cell = (z*g.n[1]+y)*g.n[0]+x;
+                int end = cellEnd[cell];
+                for (int k0 = cellStart[cell]; k0 < end; k0 += 64) {
+                    int k = k0+lane;
+                    int j = (k < end ? sortedSites[k] : -1);
+                    bool hit = false;
+                    double d[3] = {0, 0, 0};
+                    if (j > i && molecule[j] != mi) {
+                        ref_delta(si, sites+3*j, periodic, box, d);
+                        hit = ref_norm2(d) < cutoff2;
+                    }
+                    unsigned long long ballot = __ballot(hit);
+                    if (start != nullptr && hit) {
+                        int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
+                        nbrIndex[pos] = j;
+                        nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
+                    }
+                    written += __popcll(ballot);
+                }
+            }
     if (start == nullptr && lane == 0)
         count[i] = written;
 }
@@ -427,6 +501,11 @@ struct ixh_threebody {
     std::vector<unsigned char> alchemical;
     std::vector<double> params;   // alpha, frequency, width, alchemical
     DeviceArray<int> offsetsDev, particlesDev, moleculeDev, start, count, nbrIndex, siteCount;
+    // cell grid of the neighbour search
+    DeviceArray<unsigned> cellKeys, cellKeysSorted;
+    DeviceArray<int> cellValues, sortedSites, cellStart, cellEnd;
+    DeviceArray<char> sortTemp;
+    DeviceArray<double> bbox;
     DeviceArray<double> paramsDev, sites, siteEnergy;
     DeviceArray<double4> nbrDelta;
     DeviceArray<long long> siteForce, tripleCount;
@@ -476,6 +555,76 @@ void load_threebody_params(ixh_threebody* h, const ixh_threebody_desc* d) {
     h->centroidDampingRadius = d->centroid_damping_radius;
     h->paramsDev.upload(h->params.data(), h->params.size(), nullptr);
     IXH_HIP_CHECK(hipStreamSynchronize(nullptr));
+}
+
+__global__ void k_site_bounds(const double* sites, int numSites, double* out) {
+    __shared__ double lo[3][256], hi[3][256];
+    double l[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, u[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int i = threadIdx.x; i < numSites; i += blockDim.x)
+        for (int k = 0; k < 3; k++) {
+            l[k] = fmin(l[k], sites[3*i+k]);
+            u[k] = fmax(u[k], sites[3*i+k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        lo[k][threadIdx.x] = l[k];
+        hi[k][threadIdx.x] = u[k];
+    }
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride)
+            for (int k = 0; k < 3; k++) {
+                lo[k][threadIdx.x] = fmin(lo[k][threadIdx.x], lo[k][threadIdx.x+stride]);
+                hi[k][threadIdx.x] = fmax(hi[k][threadIdx.x], hi[k][threadIdx.x+stride]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) {
+            out[k] = lo[k][0];
+            out[3+k] = hi[k][0];
+        }
+}
+
+// The cell grid of one evaluation.  Periodic boxes: rectangular boxes with at least three cells of the cutoff per
+// axis; other boxes, and axes with fewer cells, get one cell along that axis.  Nonperiodic systems of more than
+// 4096 sites: a grid of cutoff-sized cells over the sites' bounding box (at most 64 per axis).
+CellGrid make_grid(ixh_threebody* h, const double* box, hipStream_t stream) {
+    CellGrid g;
+    g.periodic = h->periodic;
+    for (int k = 0; k < 3; k++) {
+        g.n[k] = 1;
+        g.origin[k] = 0;
+        g.scale[k] = 0;
+    }
+    const double cell = h->cutoff*(1+1e-12);
+    if (h->periodic) {
+        bool rectangular = (box[3] == 0 && box[6] == 0 && box[7] == 0);
+        if (rectangular)
+            for (int k = 0; k < 3; k++) {
+                double L = box[4*k];
+                int cells = (int) std::min(1024.0, std::floor(L/cell));
+                if (cells >= 3) {
+                    g.n[k] = cells;
+                    g.scale[k] = 1.0/L;
+                }
+            }
+    }
+    else if (h->numSites > 4096) {
+        h->bbox.reserve(6);
+        hipLaunchKernelGGL(k_site_bounds, dim3(1), dim3(256), 0, stream, h->sites.ptr, h->numSites, h->bbox.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        double bb[6];
+        IXH_HIP_CHECK(hipMemcpyAsync(bb, h->bbox.ptr, sizeof(bb), hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int k = 0; k < 3; k++) {
+            double extent = bb[3+k]-bb[k];
+            int cells = (int) std::min(64.0, std::max(1.0, std::floor(extent/cell)));
+            g.n[k] = cells;
+            g.origin[k] = bb[k];
+            g.scale[k] = (cells > 1 ? cells/(extent*(1+1e-12)) : 0.0);
+        }
+    }
+    return g;
 }
 
 template <typename Real>
@@ -621,8 +770,33 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
                            h->particlesDev.ptr, n, h->sites.ptr);
         IXH_HIP_CHECK(hipGetLastError());
         double cutoff2 = h->cutoff*h->cutoff;
-        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2,
-                           h->count.ptr, (const int*) nullptr, (int*) nullptr, (double4*) nullptr);
+        CellGrid g = make_grid(h, b->box, stream);
+        const int numCells = g.n[0]*g.n[1]*g.n[2];
+        h->cellKeys.reserve(n);
+        h->cellKeysSorted.reserve(n);
+        h->cellValues.reserve(n);
+        h->sortedSites.reserve(n);
+        h->cellStart.reserve(numCells);
+        h->cellEnd.reserve(numCells);
+        hipLaunchKernelGGL(k_site_cells, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->sites.ptr, n, g, h->cellKeys.ptr, h->cellValues.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        int bits = 1;
+        while ((1LL << bits) < numCells)
+            bits++;
+        size_t tempBytes = 0;
+        IXH_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
+                                                h->sortedSites.ptr, (size_t) n, 0, bits, stream));
+        h->sortTemp.reserve(tempBytes);
+        IXH_HIP_CHECK(rocprim::radix_sort_pairs(h->sortTemp.ptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
+                                                h->sortedSites.ptr, (size_t) n, 0, bits, stream));
+        hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellStart.ptr, numCells, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellEnd.ptr, numCells, 0);
+        hipLaunchKernelGGL(k_cell_ranges, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->cellKeysSorted.ptr, n, h->cellStart.ptr,
+                           h->cellEnd.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
+                           h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) nullptr, (int*) nullptr,
+                           (double4*) nullptr);
         IXH_HIP_CHECK(hipGetLastError());
         std::vector<int> counts(n), starts(n+1, 0);
         IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), h->count.ptr, sizeof(int)*(size_t) n, hipMemcpyDeviceToHost, stream));
@@ -635,8 +809,9 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
         h->start.upload(starts.data(), starts.size(), stream);
         h->nbrIndex.reserve(std::max(starts[n], 1));
         h->nbrDelta.reserve(std::max(starts[n], 1));
-        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2,
-                           h->count.ptr, (const int*) h->start.ptr, h->nbrIndex.ptr, h->nbrDelta.ptr);
+        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
+                           h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) h->start.ptr,
+                           h->nbrIndex.ptr, h->nbrDelta.ptr);
         IXH_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
         IXH_HIP_CHECK(hipGetLastError());

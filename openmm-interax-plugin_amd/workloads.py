@@ -118,37 +118,45 @@ def perturbed(positions: np.ndarray, evaluation: int, sigma: float = 0.005) -> n
 
 def rsa_points(rng: np.random.Generator, n: int, box: float, min_sep: float, periodic: bool = True,
                sphere_radius: Optional[float] = None) -> np.ndarray:
-    """Random sequential addition with a minimum separation, accelerated by a cell grid."""
+    """Random sequential addition with a minimum separation, accelerated by a cell grid.  Candidates are drawn in
+    blocks from the generator (the same stream as one draw of 3 per attempt) and tested with scalar arithmetic."""
     cells = max(1, int(box/min_sep))
     size = box/cells
+    min2 = min_sep*min_sep
     grid = {}
-    points = np.zeros((n, 3))
-    count = 0
+    points = []
     attempts = 0
-    while count < n:
+    lo, hi = (0.0, box) if sphere_radius is None else (-sphere_radius, sphere_radius)
+    r2max = None if sphere_radius is None else sphere_radius**2
+    shift = 0.0 if sphere_radius is None else box/2
+    block = []
+    while len(points) < n:
+        if not block:
+            block = rng.uniform(lo, hi, size=(4096, 3)).tolist()
+            block.reverse()
         attempts += 1
         if attempts > 200*n:
             raise RuntimeError("random sequential addition did not converge")
-        if sphere_radius is None:
-            p = rng.uniform(0, box, size=3)
-        else:
-            p = rng.uniform(-sphere_radius, sphere_radius, size=3)
-            if p @ p > sphere_radius**2:
+        x, y, z = block.pop()
+        if r2max is not None:
+            if x*x+y*y+z*z > r2max:
                 continue
-            p = p+box/2
-        c = tuple(int(x//size) for x in p)
+            x, y, z = x+shift, y+shift, z+shift
+        cx, cy, cz = int(x//size), int(y//size), int(z//size)
         ok = True
         for dx in (-1, 0, 1):
             for dy in (-1, 0, 1):
                 for dz in (-1, 0, 1):
-                    key = (c[0]+dx, c[1]+dy, c[2]+dz)
+                    key = (cx+dx, cy+dy, cz+dz)
                     if periodic:
                         key = (key[0] % cells, key[1] % cells, key[2] % cells)
-                    for q in grid.get(key, ()):
-                        d = p-points[q]
+                    for (qx, qy, qz) in grid.get(key, ()):
+                        ex, ey, ez = x-qx, y-qy, z-qz
                         if periodic:
-                            d -= box*np.round(d/box)
-                        if d @ d < min_sep*min_sep:
+                            ex -= box*round(ex/box)
+                            ey -= box*round(ey/box)
+                            ez -= box*round(ez/box)
+                        if ex*ex+ey*ey+ez*ez < min2:
                             ok = False
                             break
                     if not ok:
@@ -159,11 +167,10 @@ def rsa_points(rng: np.random.Generator, n: int, box: float, min_sep: float, per
                 break
         if not ok:
             continue
-        key = (c[0] % cells, c[1] % cells, c[2] % cells) if periodic else c
-        grid.setdefault(key, []).append(count)
-        points[count] = p
-        count += 1
-    return points
+        key = (cx % cells, cy % cells, cz % cells) if periodic else (cx, cy, cz)
+        grid.setdefault(key, []).append((x, y, z))
+        points.append((x, y, z))
+    return np.array(points, dtype=np.float64).reshape(n, 3)
 
 
 def atm_fluid(num_sites: int = 32768, density: float = 21.0, min_sep: float = 0.3, cutoff: float = 0.7, seed: int = 0):

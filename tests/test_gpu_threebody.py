@@ -279,3 +279,42 @@ def test_atm_fluid_against_oracle():
     assert context.triple_counts() == [oracle.threebody_count(f, pos, box)]
     assert np.linalg.norm(state.getForces()-f_ref)/np.linalg.norm(f_ref) <= 1e-5
     assert state.getPotentialEnergy() == pytest.approx(e_ref, rel=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_nonperiodic_cell_grid(precision):
+    # more than 4096 sites without periodic boundaries: the neighbour search runs on a grid over the bounding box;
+    # membership and triples must match the oracle's all-pairs search
+    rng = np.random.default_rng(7)
+    n = 5000
+    pos = workloads.rsa_points(rng, n, 6.0, 0.25, periodic=False)
+    system, f = single_particle_sites(n)
+    f.setCutoffDistance(0.55)
+    context = ih.Context(system, {"Precision": precision})
+    context.setPositions(pos)
+    compare(context, system, f, pos, precision)
+    assert context.triple_counts() == [oracle.threebody_count(f, pos)]
+
+
+@pytest.mark.gpu
+def test_grid_with_one_cell_along_an_axis():
+    # a periodic box that is wide along x and y but thinner than three cutoffs along z: one cell along z
+    rng = np.random.default_rng(3)
+    lx, lz = 5.0, 1.9
+    grid = np.stack(np.meshgrid(np.arange(20), np.arange(20), np.arange(7), indexing="ij"), -1).reshape(-1, 3)
+    pos = (grid+0.5)*np.array([lx/20, lx/20, lz/7])+rng.uniform(-0.04, 0.04, size=grid.shape)
+    n = len(pos)
+    system = ih.System(n, [lx, 0, 0, 0, lx, 0, 0, 0, lz])
+    f = ih.InteraxThreeBodyForce()
+    for i in range(n):
+        a, w, s = SITE_TYPES[i % 3]
+        f.addSite([i], a, w, s)
+    f.setCutoffDistance(0.7)
+    f.setUsesPeriodicBoundaryConditions(True)
+    system.addForce(f)
+    box = system.getDefaultPeriodicBoxVectors()
+    context = ih.Context(system, {"Precision": "mixed"})
+    context.setPositions(pos)
+    compare(context, system, f, pos, "mixed", box=box)
+    assert context.triple_counts() == [oracle.threebody_count(f, pos, box)]
