@@ -141,53 +141,101 @@ __global__ void k_cell_ranges(const unsigned* sortedKeys, int numSites, int* cel
         cellEnd[key] = s+1;
 }
 
-// One wave per site i: the sites j > i of another molecule within the cutoff (ReferenceInteraxThreeBodyIxn.cpp:97-103,
-// distance with the reference's rounding).  Pass 0 counts, pass 1 writes, in a fixed order (cells, then site index).
-__global__ void k_neighbors(const double* sites, const int* molecule, int numSites, int periodic, BoxT box, double cutoff2, CellGrid g,
-                            const int* cellStart, const int* cellEnd, const int* sortedSites, int* count, const int* start,
-                            int* nbrIndex, double4* nbrDelta) {
-    int i = blockIdx.x;
-    int lane = threadIdx.x;
-    const double* si = sites+3*i;
-    int mi = molecule[i];
-    int c[3] = {cell_coordinate(si[0], 0, g), cell_coordinate(si[1], 1, g), cell_coordinate(si[2], 2, g)};
-    int lo[3], hi[3];
-    for (int k = 0; k < 3; k++) {
-        lo[k] = (g.n[k] == 1 ? 0 : -1);
-        hi[k] = (g.n[k] == 1 ? 0 : 1);
+// The reference's minimum image with the divisions replaced by multiplications with the reciprocal box lengths.
+// When the cutoff is below half of every box length this gives the same image as ref_delta for every pair within the
+// cutoff: the two can differ only where d/L + 1/2 rounds across an integer, i.e. at |d| = L/2 > cutoff.
+__device__ __forceinline__ void fast_delta(const double* from, const double* to, int periodic, const BoxT& b, const double inv[3],
+                                           double d[3]) {
+    d[0] = __dsub_rn(to[0], from[0]);
+    d[1] = __dsub_rn(to[1], from[1]);
+    d[2] = __dsub_rn(to[2], from[2]);
+    if (periodic) {
+        double s2 = floor(__dadd_rn(__dmul_rn(d[2], inv[2]), 0.5));
+        d[0] = __dsub_rn(d[0], __dmul_rn(b.cx, s2));
+        d[1] = __dsub_rn(d[1], __dmul_rn(b.cy, s2));
+        d[2] = __dsub_rn(d[2], __dmul_rn(b.cz, s2));
+        double s1 = floor(__dadd_rn(__dmul_rn(d[1], inv[1]), 0.5));
+        d[0] = __dsub_rn(d[0], __dmul_rn(b.bx, s1));
+        d[1] = __dsub_rn(d[1], __dmul_rn(b.by, s1));
+        double s0 = floor(__dadd_rn(__dmul_rn(d[0], inv[0]), 0.5));
+        d[0] = __dsub_rn(d[0], __dmul_rn(b.ax, s0));
     }
-    int written = 0;
-    for (int dz = lo[2]; dz <= hi[2]; dz++)
-        for (int dy = lo[1]; dy <= hi[1]; dy++)
-            for (int dx = lo[0]; dx <= hi[0]; dx++) {
-                int x = c[0]+dx, y = c[1]+dy, z = c[2]+dz;
-                if (g.periodic) {
-                    x = (x+g.n[0]) % g.n[0];
-                    y = (y+g.n[1]) % g.n[1];
-                    z = (z+g.n[2]) % g.n[2];
-                }
-                else if (x < 0 || y < 0 || z < 0 || x >= g.n[0] || y >= g.n[1] || z >= g.n[2])
-                    continue;
-                int cell = (z*g.n[1]+y)*g.n[0]+x;
-                int end = cellEnd[cell];
-                for (int k0 = cellStart[cell]; k0 < end; k0 += 64) {
-                    int k = k0+lane;
-                    int j = (k < end ? sortedSites[k] : -1);
-                    bool hit = false;
-                    double d[3] = {0, 0, 0};
-                    if (j > i && molecule[j] != mi) {
-                        ref_delta(si, sites+3*j, periodic, box, d);
-                        hit = ref_norm2(d) < cutoff2;
-                    }
-                    unsigned long long ballot = __ballot(hit);
-                    if (start != nullptr && hit) {
-                        int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
-                        nbrIndex[pos] = j;
-                        nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
-                    }
-                    written += __popcll(ballot);
-                }
+}
+
+// One wave per site i: the sites j > i of another molecule within the cutoff (ReferenceInteraxThreeBodyIxn.cpp:97-103,
+// distance with the reference's rounding).  The candidates of the (up to) 27 neighbouring cells are taken as one
+// list, 64 at a time.  Pass 0 counts, pass 1 writes, in a fixed order (cells, then site index).
+__global__ void k_neighbors(const double* sites, const int* molecule, int numSites, int periodic, BoxT box, double cutoff2, CellGrid g,
+                            int exactImage, const int* cellStart, const int* cellEnd, const int* sortedSites, int* count,
+                            const int* start, int* nbrIndex, double4* nbrDelta) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double* si = sites+3*i;
+    const int mi = molecule[i];
+    const double inv[3] = {1.0/box.ax, 1.0/box.by, 1.0/box.cz};
+    // lane k < 27: the range of neighbouring cell k
+    int first = 0, size = 0;
+    {
+        const int c[3] = {cell_coordinate(si[0], 0, g), cell_coordinate(si[1], 1, g), cell_coordinate(si[2], 2, g)};
+        const int span[3] = {g.n[0] == 1 ? 1 : 3, g.n[1] == 1 ? 1 : 3, g.n[2] == 1 ? 1 : 3};
+        if (lane < span[0]*span[1]*span[2]) {
+            int x = c[0]+(span[0] == 1 ? 0 : lane%3-1);
+            int y = c[1]+(span[1] == 1 ? 0 : (lane/span[0])%3-1);
+            int z = c[2]+(span[2] == 1 ? 0 : lane/(span[0]*span[1])-1);
+            bool inside = true;
+            if (g.periodic) {
+                x = (x+g.n[0]) % g.n[0];
+                y = (y+g.n[1]) % g.n[1];
+                z = (z+g.n[2]) % g.n[2];
             }
+            else
+                inside = (x >= 0 && y >= 0 && z >= 0 && x < g.n[0] && y < g.n[1] && z < g.n[2]);
+            if (inside) {
+                const int cell = (z*g.n[1]+y)*g.n[0]+x;
+                first = cellStart[cell];
+                size = cellEnd[cell]-first;
+            }
+        }
+    }
+    // inclusive prefix of the range sizes over the lanes
+    int prefix = size;
+    for (int offset = 1; offset < 64; offset <<= 1) {
+        int v = __shfl_up(prefix, offset, 64);
+        if (lane >= offset)
+            prefix += v;
+    }
+    const int total = __shfl(prefix, 63, 64);
+    int written = 0;
+    for (int c0 = 0; c0 < total; c0 += 64) {
+        const int c = c0+lane;
+        int j = -1;
+        // the cell of candidate c: the first lane whose inclusive prefix exceeds c
+        int k = 0;
+        for (int step = 32; step > 0; step >>= 1) {
+            int probe = __shfl(prefix, k+step-1, 64);
+            if (probe <= c)
+                k += step;
+        }
+        const int kFirst = __shfl(first, k, 64), kPrefix = __shfl(prefix, k, 64), kSize = __shfl(size, k, 64);
+        if (c < total)
+            j = sortedSites[kFirst+(c-(kPrefix-kSize))];
+        bool hit = false;
+        double d[3] = {0, 0, 0};
+        if (j > i && molecule[j] != mi) {
+            if (exactImage)
+                ref_delta(si, sites+3*j, periodic, box, d);
+            else
+                fast_delta(si, sites+3*j, periodic, box, inv, d);
+            hit = ref_norm2(d) < cutoff2;
+        }
+        const unsigned long long ballot = __ballot(hit);
+        if (start != nullptr && hit) {
+            const int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
+            nbrIndex[pos] = j;
+            nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
+        }
+        written += __popcll(ballot);
+    }
     if (start == nullptr && lane == 0)
         count[i] = written;
 }
@@ -794,9 +842,10 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
         hipLaunchKernelGGL(k_cell_ranges, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->cellKeysSorted.ptr, n, h->cellStart.ptr,
                            h->cellEnd.ptr);
         IXH_HIP_CHECK(hipGetLastError());
+        const int exactImage = (h->periodic && 2*h->cutoff >= std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
         hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
-                           h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) nullptr, (int*) nullptr,
-                           (double4*) nullptr);
+                           exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) nullptr,
+                           (int*) nullptr, (double4*) nullptr);
         IXH_HIP_CHECK(hipGetLastError());
         std::vector<int> counts(n), starts(n+1, 0);
         IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), h->count.ptr, sizeof(int)*(size_t) n, hipMemcpyDeviceToHost, stream));
@@ -810,7 +859,7 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
         h->nbrIndex.reserve(std::max(starts[n], 1));
         h->nbrDelta.reserve(std::max(starts[n], 1));
         hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
-                           h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) h->start.ptr,
+                           exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) h->start.ptr,
                            h->nbrIndex.ptr, h->nbrDelta.ptr);
         IXH_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
