@@ -171,6 +171,16 @@ int ixh_pair_execute(ixh_pair* pair, const ixh_buffers* buffers, int terms, int 
    (coefficient + lambda*alchemicalCoefficient)/V (ReferenceInteraxKernels.cpp:93-94). */
 int ixh_pair_long_range_coefficients(ixh_pair* pair, double* coefficient, double* alchemical_coefficient);
 int ixh_pair_long_range_energy(ixh_pair* pair, const double* box, double lambda, double* energy);
+/*
+ * Batches of independent systems (no counterpart in the reference; used by the batched and served forms): the
+ * particles of num_segments systems are concatenated, segment s owning particles [particle_offsets[s],
+ * particle_offsets[s+1]); pairs never cross segments.  The descriptor describes the concatenated system (exclusions in
+ * global indices); the NoCutoff method only.  ixh_pair_execute_segments also ADDS the energy of every segment to
+ * segment_energies, a device array of num_segments doubles (the per-segment sums have a fixed order, so a segment's
+ * energy and forces do not depend on the other segments of the batch).
+ */
+int ixh_pair_create_segmented(int device, const ixh_pair_desc* desc, int num_segments, const int* particle_offsets, ixh_pair** pair);
+int ixh_pair_execute_segments(ixh_pair* pair, const ixh_buffers* buffers, int terms, int flags, double* segment_energies);
 /* Number of interacting pairs of the last execute() (waits for it). */
 int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
 /* Size of the current neighbour list (no counterpart in the reference): the number of 64 x 64 tiles and of candidate
@@ -215,10 +225,43 @@ int ixh_threebody_destroy(ixh_threebody* threebody);
 int ixh_threebody_set_parameters(ixh_threebody* threebody, const ixh_threebody_desc* desc);
 /* execute (ReferenceInteraxKernels.cpp:210-213). */
 int ixh_threebody_execute(ixh_threebody* threebody, const ixh_buffers* buffers, int flags);
+/* Batches of independent nonperiodic systems (no counterpart in the reference): segment s owns sites
+   [site_offsets[s], site_offsets[s+1]); triples never cross segments; the per-segment energies are ADDED to the device
+   array segment_energies.  An evaluation of a segmented handle needs no host synchronization. */
+int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* desc, int num_segments, const int* site_offsets,
+                                   ixh_threebody** threebody);
+int ixh_threebody_execute_segments(ixh_threebody* threebody, const ixh_buffers* buffers, int flags, double* segment_energies);
 /* Number of interacting triples of the last execute() (waits for it). */
 int ixh_threebody_last_triple_count(ixh_threebody* threebody, long long* count);
 int ixh_threebody_kernel_time(ixh_threebody* threebody, double* milliseconds, long long* launches);
 int ixh_threebody_reset_kernel_time(ixh_threebody* threebody);
+
+/*
+ * Batched evaluation of many independent systems (no counterpart in the reference: its kernels evaluate one System per
+ * Context).  The systems are concatenated: system s owns particles [particle_offsets[s], particle_offsets[s+1]) and
+ * three-body sites [site_offsets[s], site_offsets[s+1]); `pair` describes the concatenated pair forces (NoCutoff,
+ * global particle indices) and `threebody` the concatenated nonperiodic three-body force; either may be NULL.  The
+ * batch owns its device state.  ixh_batch_execute is asynchronous; from the second call on it replays a HIP graph of
+ * the whole evaluation.  ixh_batch_autotune times the launch shapes of the pair tiles and keeps the fastest.
+ */
+typedef struct ixh_batch ixh_batch;
+typedef struct {
+    int num_systems;
+    const int* particle_offsets;
+    const ixh_pair_desc* pair;
+    const int* site_offsets;
+    const ixh_threebody_desc* threebody;
+    int precision;
+} ixh_batch_desc;
+int ixh_batch_create(int device, const ixh_batch_desc* desc, ixh_batch** batch);
+int ixh_batch_destroy(ixh_batch* batch);
+int ixh_batch_set_positions(ixh_batch* batch, const double* positions);   /* host, 3 x total particles, nm */
+int ixh_batch_get_buffers(ixh_batch* batch, ixh_buffers* buffers);      /* the batch's device state */
+int ixh_batch_execute(ixh_batch* batch, int flags);
+int ixh_batch_get_energies(ixh_batch* batch, double* energies);          /* waits; one per system, kJ/mol */
+int ixh_batch_get_forces(ixh_batch* batch, double* forces);              /* waits; 3 x total particles */
+int ixh_batch_set_graph(ixh_batch* batch, int enable);                   /* 0: launch the kernels eagerly */
+int ixh_batch_autotune(ixh_batch* batch, int flags, int* chosen_workgroups);
 
 #ifdef __cplusplus
 }

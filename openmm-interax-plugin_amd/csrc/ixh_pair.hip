@@ -24,6 +24,7 @@
 // integer atomics; each tile writes its FP64 energy to its own slot and the slots are summed in tile order.  Results
 // therefore do not depend on the launch shape, on which wave ran which tile, or on other systems sharing a launch.
 #include "ixh_pair_math.h"
+#include "ixh_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -567,6 +568,7 @@ struct TileArgs {
     int stride;
     double* waveEnergy;   // per wave of the launch: energy and pair count of its tiles, in tile order
     int* waveCount;
+    double* tileEnergy;   // segmented handles: the energy of every tile, for the per-segment sums
 };
 
 // Copy the type tables into LDS (all threads of the workgroup) and describe them.
@@ -794,6 +796,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
         const int origI = flagI & INDEX_MASK;
         Real fi[3] = {0, 0, 0};
         Real fj[3] = {0, 0, 0};
+        double tileE = 0;
         for (int st = sBegin; st < sEnd; st++) {
             const int jl = (lane+st) & 63;
             bool interact = ((mrow >> jl) & 1ULL) != 0;
@@ -807,7 +810,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
                 fetch_particle(s, jl, Q);
                 Real e, fs;
                 evaluate_pair<Real, TERMS>(a, tb, r2, P, Q, alchI, flagJ, origI, e, fs);
-                energy += (double) e;
+                tileE += (double) e;
                 fi[0] += dx*fs;
                 fi[1] += dy*fs;
                 fi[2] += dz*fs;
@@ -834,6 +837,12 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
         fixI[0] += to_fixed(fi[0]);
         fixI[1] += to_fixed(fi[1]);
         fixI[2] += to_fixed(fi[2]);
+        energy += tileE;
+        if (a.tileEnergy != nullptr) {
+            const double te = wave_sum(tileE);
+            if (lane == 0)
+                a.tileEnergy[t] = te;
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     if (curI >= 0 && flagI >= 0) {
@@ -1103,6 +1112,26 @@ __global__ void __launch_bounds__(1024) k_reduce_waves(const double* waveEnergy,
     }
 }
 
+// Energy of every segment: the sum of its tiles' energies in tile order (one workgroup per segment, fixed tree),
+// added to energies[s].
+__global__ void __launch_bounds__(256) k_reduce_segments(const double* tileEnergy, const int* segTiles, double* energies) {
+    __shared__ double sE[256];
+    const int seg = blockIdx.x;
+    const int begin = segTiles[seg], end = segTiles[seg+1];
+    double e = 0;
+    for (int t = begin+threadIdx.x; t < end; t += 256)
+        e += tileEnergy[t];
+    sE[threadIdx.x] = e;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride)
+            sE[threadIdx.x] += sE[threadIdx.x+stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        energies[seg] += sE[0];
+}
+
 // ------------------------------------------------------------------------------------------------------------
 // Long range correction (InteraxDispersionForceImpl.cpp:86-149), FP64 on the device
 // ------------------------------------------------------------------------------------------------------------
@@ -1254,6 +1283,13 @@ struct ixh_pair {
         partnerCount, partnerStart, partners, waveCount, tileSlots, tileCandidates, listOffsets;
     DeviceArray<unsigned short> lists;
     long long numCandidates = 0;
+    // independent systems (segments) of a batch: particles [segOffsets[s], segOffsets[s+1]), blocks from segBlocks[s],
+    // tiles [segTiles[s], segTiles[s+1]); pairs never cross segments
+    int numSegments = 0;
+    std::vector<int> segOffsets, segBlocks, segTiles;
+    DeviceArray<int> segTilesDev;
+    DeviceArray<double> tileEnergy;
+    int gridLimit = 4096;   // largest number of workgroups of a tile launch
     // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
     bool useTables = false;
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
@@ -1636,19 +1672,38 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     p->slotOf.reserve(n);
     std::vector<int2> tiles;
     if (!cutoff) {
-        // all pairs: the particle order is kept, every block J > I is a partner chunk
-        std::vector<int> perm(p->padded);
-        for (int s = 0; s < p->padded; s++)
-            perm[s] = (s < n ? s : -1);
+        // all pairs: the particle order is kept, every block J > I is a partner chunk; with segments, each segment
+        // starts a new block and only blocks of the same segment pair up
+        std::vector<int> perm(p->padded, -1);
+        if (p->numSegments > 0) {
+            p->segTiles.assign(p->numSegments+1, 0);
+            for (int g = 0; g < p->numSegments; g++) {
+                const int first = p->segOffsets[g], count = p->segOffsets[g+1]-first;
+                const int B0 = p->segBlocks[g], B1 = p->segBlocks[g+1];
+                for (int k = 0; k < count; k++)
+                    perm[B0*BLOCK+k] = first+k;
+                for (int I = B0; I < B1; I++) {
+                    tiles.push_back(make_int2(I, -1));
+                    for (int J = I+1; J < B1; J++)
+                        tiles.push_back(make_int2(I, J*BLOCK));
+                }
+                p->segTiles[g+1] = (int) tiles.size();
+            }
+            p->segTilesDev.upload(p->segTiles.data(), p->segTiles.size(), stream);
+        }
+        else {
+            for (int s = 0; s < n; s++)
+                perm[s] = s;
+            for (int I = 0; I < Nb; I++) {
+                tiles.push_back(make_int2(I, -1));
+                for (int J = I+1; J < Nb; J++)
+                    tiles.push_back(make_int2(I, J*BLOCK));
+            }
+        }
         p->perm.upload(perm.data(), perm.size(), stream);
         std::vector<int> partners(p->padded);
         std::iota(partners.begin(), partners.end(), 0);
         p->partners.upload(partners.data(), partners.size(), stream);
-        for (int I = 0; I < Nb; I++) {
-            tiles.push_back(make_int2(I, -1));
-            for (int J = I+1; J < Nb; J++)
-                tiles.push_back(make_int2(I, J*BLOCK));
-        }
         p->pairImage = 0;
     }
     else {
@@ -1684,8 +1739,11 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         IXH_HIP_CHECK(rocprim::radix_sort_pairs(p->sortTemp.ptr, tempBytes, p->keys.ptr, p->keysOut.ptr, p->values.ptr, p->perm.ptr,
                                                 (size_t) n, 0, 30, stream));
     }
-    hipLaunchKernelGGL(k_finish_perm, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, n, p->padded, p->slotOf.ptr);
-    IXH_HIP_CHECK(hipGetLastError());
+    if (cutoff) {
+        // the sort fills slots [0, n): the rest are padding
+        hipLaunchKernelGGL(k_finish_perm, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, n, p->padded, p->slotOf.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
     size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
     p->sortedA.reserve(r4*p->padded);
     p->sortedB.reserve(r4*p->padded);
@@ -1818,7 +1876,7 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
 }
 
 template <typename Real>
-void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, hipStream_t stream) {
+void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, double* segmentEnergies, hipStream_t stream) {
     typedef typename Vec4T<Real>::type R4;
     TileArgs<Real> a;
     a.tiles = p->tiles.ptr;
@@ -1879,7 +1937,12 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
     a.srOffset = p->srOffset;
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
-    const int blocks = std::max(1, std::min((p->numTiles+3)/4, 4096));
+    const int blocks = std::max(1, std::min((p->numTiles+3)/4, p->gridLimit));
+    a.tileEnergy = nullptr;
+    if (p->numSegments > 0) {
+        p->tileEnergy.reserve(std::max(p->numTiles, 1));
+        a.tileEnergy = p->tileEnergy.ptr;
+    }
     const int numWaves = 4*blocks;
     p->waveEnergy.reserve(numWaves);
     p->waveCount.reserve(numWaves);
@@ -1919,17 +1982,49 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, h
         hipLaunchKernelGGL(k_reduce_waves, dim3(1), dim3(1024), 0, stream, p->waveEnergy.ptr, p->waveCount.ptr, 0, b->energy,
                            p->pairCount.ptr, addEnergy);
     IXH_HIP_CHECK(hipGetLastError());
+    if (segmentEnergies != nullptr && p->numSegments > 0) {
+        hipLaunchKernelGGL(k_reduce_segments, dim3(p->numSegments), dim3(256), 0, stream, p->tileEnergy.ptr, p->segTilesDev.ptr,
+                           segmentEnergies);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
 }
 
 }  // namespace
 
+namespace ixh {
+
+void pair_set_grid_limit(ixh_pair* pair, int workgroups) {
+    pair->gridLimit = std::max(1, workgroups);
+}
+
+int pair_num_tiles(const ixh_pair* pair) {
+    return pair->numTiles;
+}
+
+}  // namespace ixh
+
 extern "C" {
 
 int ixh_pair_create(int device, const ixh_pair_desc* d, ixh_pair** out) {
+    return ixh_pair_create_segmented(device, d, 0, nullptr, out);
+}
+
+int ixh_pair_create_segmented(int device, const ixh_pair_desc* d, int num_segments, const int* particle_offsets, ixh_pair** out) {
     return guarded([&] {
         if (out == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_create: null output pointer");
         check_desc(d);
+        if (num_segments < 0 || (num_segments > 0 && particle_offsets == nullptr))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_create_segmented: invalid segments");
+        if (num_segments > 0) {
+            if (d->method != IXH_NO_CUTOFF)
+                throw Error(IXH_ERROR_UNSUPPORTED, "ixh_pair_create_segmented: segments require the NoCutoff method");
+            if (particle_offsets[0] != 0 || particle_offsets[num_segments] != d->num_particles)
+                throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_create_segmented: the segments must cover the particles");
+            for (int g = 0; g < num_segments; g++)
+                if (particle_offsets[g+1] < particle_offsets[g])
+                    throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_create_segmented: decreasing segment offsets");
+        }
         DeviceGuard guard(device);
         ixh_pair* p = new ixh_pair();
         try {
@@ -1970,7 +2065,16 @@ int ixh_pair_create(int device, const ixh_pair_desc* d, ixh_pair** out) {
                 p->exclusionPartners.upload(partners.data(), std::max<size_t>(partners.size(), 0), nullptr);
                 p->exclusionPartners.reserve(std::max<size_t>(partners.size(), 1));
             }
-            p->numBlocks = std::max(1, (p->n+BLOCK-1)/BLOCK);
+            if (num_segments > 0) {
+                p->numSegments = num_segments;
+                p->segOffsets.assign(particle_offsets, particle_offsets+num_segments+1);
+                p->segBlocks.assign(num_segments+1, 0);
+                for (int g = 0; g < num_segments; g++)
+                    p->segBlocks[g+1] = p->segBlocks[g]+(particle_offsets[g+1]-particle_offsets[g]+BLOCK-1)/BLOCK;
+                p->numBlocks = std::max(1, p->segBlocks[num_segments]);
+            }
+            else
+                p->numBlocks = std::max(1, (p->n+BLOCK-1)/BLOCK);
             p->padded = p->numBlocks*BLOCK;
             hipStream_t stream = nullptr;
             p->flagDev.reserve(1);
@@ -2026,6 +2130,10 @@ int ixh_pair_set_parameters(ixh_pair* p, const ixh_pair_desc* d) {
 }
 
 int ixh_pair_execute(ixh_pair* p, const ixh_buffers* b, int terms, int flags) {
+    return ixh_pair_execute_segments(p, b, terms, flags, nullptr);
+}
+
+int ixh_pair_execute_segments(ixh_pair* p, const ixh_buffers* b, int terms, int flags, double* segmentEnergies) {
     return guarded([&] {
         if (p == nullptr || b == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: null argument");
@@ -2071,13 +2179,13 @@ int ixh_pair_execute(ixh_pair* p, const ixh_buffers* b, int terms, int flags) {
             hipLaunchKernelGGL(k_local_positions<double>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
                                p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (double4*) p->local.ptr);
             IXH_HIP_CHECK(hipGetLastError());
-            launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, stream);
+            launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream);
         }
         else {
             hipLaunchKernelGGL(k_local_positions<float>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
                                p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (float4*) p->local.ptr);
             IXH_HIP_CHECK(hipGetLastError());
-            launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, stream);
+            launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream);
         }
         IXH_HIP_CHECK(hipEventRecord(p->done, stream));
     });

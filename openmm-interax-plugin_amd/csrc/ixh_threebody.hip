@@ -257,6 +257,7 @@ struct TripleArgs {
     const int* molecule;
     const double* siteParams;    // alpha, frequency, width, alchemical per site
     const int* start;
+    const int* count;            // segmented handles: neighbours of site i = [start[i], start[i]+count[i])
     const int* nbrIndex;
     const double4* nbrDelta;
     int numSites;
@@ -313,7 +314,7 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     int* sMol = sIndex+t.capacity;
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
-    const int begin = t.start[i], m = t.start[i+1]-begin;
+    const int begin = t.start[i], m = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
     for (int a = lane; a < m; a += 64) {
         int j = t.nbrIndex[begin+a];
         double4 d = t.nbrDelta[begin+a];
@@ -475,6 +476,57 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     }
 }
 
+// Segmented neighbour lists: site i's candidates are the later sites of its own segment, [i+1, siteEnd[i]); the hits
+// (another molecule, within the cutoff, no periodic images) are written from start[i], their number to count[i].
+__global__ void k_neighbors_segments(const double* sites, const int* molecule, const int* siteEnd, double cutoff2, const int* start,
+                                     int* count, int* nbrIndex, double4* nbrDelta) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double* si = sites+3*i;
+    const int mi = molecule[i];
+    const int end = siteEnd[i];
+    int written = 0;
+    for (int j0 = i+1; j0 < end; j0 += 64) {
+        const int j = j0+lane;
+        bool hit = false;
+        double d[3] = {0, 0, 0};
+        if (j < end && molecule[j] != mi) {
+            d[0] = __dsub_rn(sites[3*j], si[0]);
+            d[1] = __dsub_rn(sites[3*j+1], si[1]);
+            d[2] = __dsub_rn(sites[3*j+2], si[2]);
+            hit = ref_norm2(d) < cutoff2;
+        }
+        const unsigned long long ballot = __ballot(hit);
+        if (hit) {
+            const int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
+            nbrIndex[pos] = j;
+            nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
+        }
+        written += __popcll(ballot);
+    }
+    if (lane == 0)
+        count[i] = written;
+}
+
+// Energy of every segment: its sites' energies summed in site order (one workgroup per segment, fixed tree), added to
+// energies[s].
+__global__ void __launch_bounds__(256) k_reduce_site_segments(const double* siteEnergy, const int* segOffsets, double* energies) {
+    __shared__ double sE[256];
+    const int seg = blockIdx.x;
+    double e = 0;
+    for (int k = segOffsets[seg]+threadIdx.x; k < segOffsets[seg+1]; k += 256)
+        e += siteEnergy[k];
+    sE[threadIdx.x] = e;
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride)
+            sE[threadIdx.x] += sE[threadIdx.x+stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        energies[seg] += sE[0];
+}
+
 __global__ void k_zero_long(long long* p, long long n) {
     long long i = (long long) blockIdx.x*blockDim.x+threadIdx.x;
     if (i < n)
@@ -554,6 +606,12 @@ struct ixh_threebody {
     DeviceArray<int> cellValues, sortedSites, cellStart, cellEnd;
     DeviceArray<char> sortTemp;
     DeviceArray<double> bbox;
+    // independent systems (segments) of a batch: sites [segOffsets[s], segOffsets[s+1]); neighbour lists of static
+    // capacity (the later sites of the same segment), so that an evaluation needs no host synchronization
+    int numSegments = 0;
+    std::vector<int> segOffsets;
+    DeviceArray<int> segOffsetsDev, siteEnd;
+    int maxSegment = 0;
     DeviceArray<double> paramsDev, sites, siteEnergy;
     DeviceArray<double4> nbrDelta;
     DeviceArray<long long> siteForce, tripleCount;
@@ -683,6 +741,7 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.molecule = h->moleculeDev.ptr;
     t.siteParams = h->paramsDev.ptr;
     t.start = h->start.ptr;
+    t.count = (h->numSegments > 0 ? h->count.ptr : nullptr);
     t.nbrIndex = h->nbrIndex.ptr;
     t.nbrDelta = h->nbrDelta.ptr;
     t.numSites = h->numSites;
@@ -724,10 +783,26 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
 extern "C" {
 
 int ixh_threebody_create(int device, const ixh_threebody_desc* d, ixh_threebody** out) {
+    return ixh_threebody_create_segmented(device, d, 0, nullptr, out);
+}
+
+int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* d, int num_segments, const int* site_offsets,
+                                   ixh_threebody** out) {
     return guarded([&] {
         if (out == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_create: null output pointer");
         check_threebody(d);
+        if (num_segments < 0 || (num_segments > 0 && site_offsets == nullptr))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_create_segmented: invalid segments");
+        if (num_segments > 0) {
+            if (d->periodic)
+                throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_create_segmented: segments require a nonperiodic force");
+            if (site_offsets[0] != 0 || site_offsets[num_segments] != d->num_sites)
+                throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_create_segmented: the segments must cover the sites");
+            for (int g = 0; g < num_segments; g++)
+                if (site_offsets[g+1] < site_offsets[g])
+                    throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_create_segmented: decreasing segment offsets");
+        }
         DeviceGuard guard(device);
         ixh_threebody* h = new ixh_threebody();
         try {
@@ -758,6 +833,28 @@ int ixh_threebody_create(int device, const ixh_threebody_desc* d, ixh_threebody*
             h->siteCount.reserve(std::max(n, 1));
             h->siteForce.reserve(3*(size_t) std::max(n, 1));
             h->tripleCount.reserve(1);
+            if (num_segments > 0) {
+                h->numSegments = num_segments;
+                h->segOffsets.assign(site_offsets, site_offsets+num_segments+1);
+                std::vector<int> end(n), start(n+1, 0);
+                long long total = 0;
+                for (int g = 0; g < num_segments; g++) {
+                    h->maxSegment = std::max(h->maxSegment, site_offsets[g+1]-site_offsets[g]);
+                    for (int i = site_offsets[g]; i < site_offsets[g+1]; i++) {
+                        end[i] = site_offsets[g+1];
+                        start[i] = (int) total;
+                        total += site_offsets[g+1]-i-1;
+                        if (total > INT32_MAX)
+                            throw Error(IXH_ERROR_CAPACITY, "ixh_threebody_create_segmented: the neighbour lists exceed 2^31 entries");
+                    }
+                }
+                start[n] = (int) total;
+                h->segOffsetsDev.upload(h->segOffsets.data(), h->segOffsets.size(), nullptr);
+                h->siteEnd.upload(end.data(), std::max<size_t>(end.size(), 1), nullptr);
+                h->start.upload(start.data(), start.size(), nullptr);
+                h->nbrIndex.reserve((size_t) std::max<long long>(total, 1));
+                h->nbrDelta.reserve((size_t) std::max<long long>(total, 1));
+            }
             load_threebody_params(h, d);
         }
         catch (...) {
@@ -797,6 +894,10 @@ int ixh_threebody_set_parameters(ixh_threebody* h, const ixh_threebody_desc* d) 
 }
 
 int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
+    return ixh_threebody_execute_segments(h, b, flags, nullptr);
+}
+
+int ixh_threebody_execute_segments(ixh_threebody* h, const ixh_buffers* b, int flags, double* segmentEnergies) {
     return guarded([&] {
         if (h == nullptr || b == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute: null argument");
@@ -818,6 +919,33 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
                            h->particlesDev.ptr, n, h->sites.ptr);
         IXH_HIP_CHECK(hipGetLastError());
         double cutoff2 = h->cutoff*h->cutoff;
+        if (h->numSegments > 0) {
+            // segmented: static list capacity, no host synchronization
+            hipLaunchKernelGGL(k_neighbors_segments, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, h->siteEnd.ptr,
+                               cutoff2, h->start.ptr, h->count.ptr, h->nbrIndex.ptr, h->nbrDelta.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
+            IXH_HIP_CHECK(hipGetLastError());
+            if (h->precision == IXH_DOUBLE)
+                launch_triples<double>(h, b, box, cutoff2, h->maxSegment, stream);
+            else
+                launch_triples<float>(h, b, box, cutoff2, h->maxSegment, stream);
+            if (flags & IXH_INCLUDE_FORCES) {
+                hipLaunchKernelGGL(k_apply_site_forces, dim3(grid_of(n, 128)), dim3(128), 0, stream, h->siteForce.ptr, n,
+                                   h->offsetsDev.ptr, h->particlesDev.ptr, b->force, b->force_stride);
+                IXH_HIP_CHECK(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
+                               h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
+            IXH_HIP_CHECK(hipGetLastError());
+            if (segmentEnergies != nullptr) {
+                hipLaunchKernelGGL(k_reduce_site_segments, dim3(h->numSegments), dim3(256), 0, stream, h->siteEnergy.ptr,
+                                   h->segOffsetsDev.ptr, segmentEnergies);
+                IXH_HIP_CHECK(hipGetLastError());
+            }
+            IXH_HIP_CHECK(hipEventRecord(h->done, stream));
+            return;
+        }
         CellGrid g = make_grid(h, b->box, stream);
         const int numCells = g.n[0]*g.n[1]*g.n[2];
         h->cellKeys.reserve(n);

@@ -71,6 +71,11 @@ class ThreeBodyDesc(ctypes.Structure):
                 ("precision", ctypes.c_int)]
 
 
+class BatchDesc(ctypes.Structure):
+    _fields_ = [("num_systems", ctypes.c_int), ("particle_offsets", _ip), ("pair", ctypes.POINTER(PairDesc)),
+                ("site_offsets", _ip), ("threebody", ctypes.POINTER(ThreeBodyDesc)), ("precision", ctypes.c_int)]
+
+
 # Every exported symbol of include/interax_hip.h: (name, argtypes).
 SIGNATURES = {
     "ixh_last_error": [],
@@ -104,6 +109,20 @@ SIGNATURES = {
     "ixh_pair_reset_kernel_time": [ctypes.c_void_p],
     "ixh_threebody_kernel_time": [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)],
     "ixh_threebody_reset_kernel_time": [ctypes.c_void_p],
+    "ixh_pair_create_segmented": [ctypes.c_int, ctypes.POINTER(PairDesc), ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_void_p)],
+    "ixh_pair_execute_segments": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+    "ixh_threebody_create_segmented": [ctypes.c_int, ctypes.POINTER(ThreeBodyDesc), ctypes.c_int, _ip,
+                                       ctypes.POINTER(ctypes.c_void_p)],
+    "ixh_threebody_execute_segments": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_void_p],
+    "ixh_batch_create": [ctypes.c_int, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_void_p)],
+    "ixh_batch_destroy": [ctypes.c_void_p],
+    "ixh_batch_set_positions": [ctypes.c_void_p, _dp],
+    "ixh_batch_get_buffers": [ctypes.c_void_p, ctypes.POINTER(Buffers)],
+    "ixh_batch_execute": [ctypes.c_void_p, ctypes.c_int],
+    "ixh_batch_get_energies": [ctypes.c_void_p, _dp],
+    "ixh_batch_get_forces": [ctypes.c_void_p, _dp],
+    "ixh_batch_set_graph": [ctypes.c_void_p, ctypes.c_int],
+    "ixh_batch_autotune": [ctypes.c_void_p, ctypes.c_int, _ip],
 }
 
 _lib = None
@@ -778,3 +797,191 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many independent nonperiodic Systems evaluated together (one segmented pair handle, one segmented three-body
+    handle, one device state, graph replay).  Every System must have the same force layout: pair forces with the
+    NoCutoff method and identical settings, and optionally one nonperiodic InteraxThreeBodyForce with identical
+    settings.  evaluate() returns the energy of every System and the forces of all particles, System after System."""
+
+    def __init__(self, systems: Sequence["System"], precision: str = "mixed", device: int = 0):
+        load_library()
+        if not systems:
+            raise InteraxError("Batch: no systems")
+        self.systems = list(systems)
+        self.precision = PRECISIONS[precision]
+        counts = [s.getNumParticles() for s in self.systems]
+        self.particle_offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        self.num_particles = int(self.particle_offsets[-1])
+        keep = []
+        pair_desc = self._pair_desc(keep)
+        tb_desc, site_offsets = self._threebody_desc(keep)
+        d = BatchDesc()
+        d.num_systems = len(self.systems)
+        d.particle_offsets = _iptr(self.particle_offsets)
+        d.pair = ctypes.pointer(pair_desc) if pair_desc is not None else None
+        d.site_offsets = _iptr(site_offsets) if site_offsets is not None else None
+        d.threebody = ctypes.pointer(tb_desc) if tb_desc is not None else None
+        d.precision = self.precision
+        self.handle = ctypes.c_void_p()
+        _check(_lib.ixh_batch_create(device, ctypes.byref(d), ctypes.byref(self.handle)))
+
+    @staticmethod
+    def _forces(system, kind):
+        return [f for f in system.getForces() if isinstance(f, kind)]
+
+    def _pair_desc(self, keep):
+        first = self._forces(self.systems[0], InteraxPairForce)
+        if not first:
+            return None
+        kinds = [type(f) for f in first]
+        d = PairDesc()
+        d.num_particles = self.num_particles
+        d.terms = 0
+        f0 = first[0]
+        if f0.getNonbondedMethod() != InteraxPairForce.NoCutoff:
+            raise InteraxError("Batch: the pair forces of a batch must use the NoCutoff method")
+        d.method = f0.getNonbondedMethod()
+        d.cutoff = f0.getCutoffDistance()
+        d.softcore_radius = f0.getSoftcoreRadius()
+        alch, excl = [], []
+        params = {k: [] for k in kinds}
+        for s, system in enumerate(self.systems):
+            forces = self._forces(system, InteraxPairForce)
+            if [type(f) for f in forces] != kinds:
+                raise InteraxError("Batch: every System must have the same pair forces")
+            offset = int(self.particle_offsets[s])
+            alch.append(forces[0]._alchemical_array())
+            ex = forces[0]._exclusion_array()
+            for f in forces:
+                check_pair_force(f, system)
+                if f.getNonbondedMethod() != d.method or f.getSoftcoreRadius() != d.softcore_radius:
+                    raise InteraxError("Batch: the pair forces of a batch must have identical settings")
+                params[type(f)].append(f.parameter_array())
+            excl.append(ex+offset)
+        alch = np.ascontiguousarray(np.concatenate(alch), dtype=np.uint8)
+        excl = np.ascontiguousarray(np.concatenate(excl).reshape(-1), dtype=np.int32)
+        keep += [alch, excl]
+        d.alchemical = _uptr(alch)
+        d.num_exclusions = len(excl)//2
+        d.exclusions = _iptr(excl)
+        d.precision = self.precision
+        d.default_box = None
+        for f in first:
+            arr = np.ascontiguousarray(np.concatenate(params[type(f)]).reshape(-1))
+            keep.append(arr)
+            d.terms |= f.TERM
+            if isinstance(f, InteraxDispersionForce):
+                d.dispersion = _dptr(arr)
+                d.damping_scale = f.getDampingScale()
+                d.use_long_range_correction = 0
+            elif isinstance(f, InteraxExchangeForce):
+                d.exchange = _dptr(arr)
+                d.exchange_overlap_method = f.getOverlapMethod()
+            else:
+                d.short_range = _dptr(arr)
+                d.short_range_overlap_method = f.getOverlapMethod()
+        return d
+
+    def _threebody_desc(self, keep):
+        forces = [self._forces(s, InteraxThreeBodyForce) for s in self.systems]
+        if not forces[0]:
+            return None, None
+        f0 = forces[0][0]
+        site_counts, offsets, parts, alpha, freq, width, mol, alch = [], [], [], [], [], [], [], []
+        site_base = 0
+        part_base = 0
+        for s, fl in enumerate(forces):
+            if len(fl) != 1:
+                raise InteraxError("Batch: every System must have one InteraxThreeBodyForce")
+            f = fl[0]
+            if (f.usesPeriodicBoundaryConditions() or f.getCutoffDistance() != f0.getCutoffDistance() or
+                    f.getScaleFactor() != f0.getScaleFactor() or f.getDampingMethod() != f0.getDampingMethod() or
+                    f.getDampingScale() != f0.getDampingScale() or
+                    f.getCentroidDampingRadius() != f0.getCentroidDampingRadius()):
+                raise InteraxError("Batch: the three-body forces of a batch must be nonperiodic with identical settings")
+            o, p, a, w, wd, m, al = f.site_arrays()
+            offsets.append(o[:-1]+part_base)
+            part_base += len(p)
+            parts.append(p+int(self.particle_offsets[s]))
+            alpha.append(a)
+            freq.append(w)
+            width.append(wd)
+            # negative molecule indices become -1 - (global site index), as a single System would number them
+            m = m.copy()
+            neg = m < 0
+            m[neg] = -1-(site_base+np.nonzero(neg)[0])
+            mol.append(m)
+            alch.append(al)
+            site_counts.append(f.getNumSites())
+            site_base += f.getNumSites()
+        site_offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(site_counts)]), dtype=np.int32)
+        parts_all = np.ascontiguousarray(np.concatenate(parts), dtype=np.int32)
+        offs = np.ascontiguousarray(np.concatenate(offsets+[[len(parts_all)]]), dtype=np.int32)
+        arrays = [offs, parts_all] + [np.ascontiguousarray(np.concatenate(x)) for x in (alpha, freq, width)]
+        arrays += [np.ascontiguousarray(np.concatenate(mol), dtype=np.int32), np.ascontiguousarray(np.concatenate(alch), dtype=np.uint8)]
+        keep += arrays + [site_offsets]
+        d = ThreeBodyDesc()
+        d.num_particles = self.num_particles
+        d.num_sites = int(site_offsets[-1])
+        d.site_offsets = _iptr(arrays[0])
+        d.site_particles = _iptr(arrays[1])
+        d.alpha = _dptr(arrays[2])
+        d.frequency = _dptr(arrays[3])
+        d.width = _dptr(arrays[4])
+        d.molecule = _iptr(arrays[5])
+        d.alchemical = _uptr(arrays[6])
+        d.cutoff = f0.getCutoffDistance()
+        d.scale_factor = f0.getScaleFactor()
+        d.damping_method = f0.getDampingMethod()
+        d.damping_scale = f0.getDampingScale()
+        d.centroid_damping_radius = f0.getCentroidDampingRadius()
+        d.periodic = 0
+        d.precision = self.precision
+        return d, site_offsets
+
+    def setPositions(self, positions: Sequence[np.ndarray]) -> None:
+        """positions: one N_s x 3 array (nm) per System, or one array of all particles."""
+        if isinstance(positions, np.ndarray) and positions.shape == (self.num_particles, 3):
+            pos = positions
+        else:
+            pos = np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 3) for p in positions])
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        if pos.size != 3*self.num_particles:
+            raise InteraxError("Batch.setPositions: wrong number of particles")
+        _check(_lib.ixh_batch_set_positions(self.handle, _dptr(pos)))
+
+    def execute(self, forces: bool = True) -> None:
+        flags = IXH_INCLUDE_ENERGY | (IXH_INCLUDE_FORCES if forces else 0)
+        _check(_lib.ixh_batch_execute(self.handle, flags))
+
+    def energies(self) -> np.ndarray:
+        e = np.zeros(len(self.systems))
+        _check(_lib.ixh_batch_get_energies(self.handle, _dptr(e)))
+        return e
+
+    def forces(self) -> List[np.ndarray]:
+        f = np.zeros(3*self.num_particles)
+        _check(_lib.ixh_batch_get_forces(self.handle, _dptr(f)))
+        f = f.reshape(-1, 3)
+        return [f[self.particle_offsets[s]:self.particle_offsets[s+1]] for s in range(len(self.systems))]
+
+    def evaluate(self, positions=None, forces: bool = True):
+        if positions is not None:
+            self.setPositions(positions)
+        self.execute(forces)
+        return self.energies(), (self.forces() if forces else None)
+
+    def set_graph(self, enable: bool) -> None:
+        _check(_lib.ixh_batch_set_graph(self.handle, int(enable)))
+
+    def autotune(self) -> int:
+        chosen = ctypes.c_int()
+        _check(_lib.ixh_batch_autotune(self.handle, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY, ctypes.byref(chosen)))
+        return chosen.value
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.ixh_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()

@@ -1,0 +1,91 @@
+"""Batched evaluation of independent clusters (the batched configuration) against the oracle, and the properties the
+batch promises: every System's result is the same alone and in a batch, and graph replays equal eager launches."""
+import numpy as np
+import pytest
+
+import interax_hip as ih
+import oracle
+import workloads
+
+# mixed precision: the reference's single/mixed export tolerances (TestInteraxExportValues.h:195-196, energy 2e-6
+# relative, forces 5e-5 of the RMS force) and the 1e-5 force RMS target; double precision against the FP64 oracle
+ENERGY_TOL = {"mixed": 2e-6, "double": 1e-9}
+FORCE_TOL = {"mixed": 5e-5, "double": 1e-8}
+
+
+def clusters(sizes, seed=0):
+    out = [workloads.water_cluster(m, seed=seed+k) for k, m in enumerate(sizes)]
+    return [s for s, _ in out], [p for _, p in out]
+
+
+def check_against_oracle(systems, positions, energies, forces, precision):
+    for system, pos, e, f in zip(systems, positions, energies, forces):
+        e_ref, f_ref, scale = oracle.evaluate(system, pos, with_scale=True)
+        assert abs(e-e_ref) <= ENERGY_TOL[precision]*max(scale, abs(e_ref)), (e, e_ref, scale)
+        rms = np.sqrt(np.mean(f_ref**2))
+        assert np.abs(f-f_ref).max() <= FORCE_TOL[precision]*rms+1e-8
+        if precision == "mixed":
+            assert np.linalg.norm(f-f_ref) <= 1e-5*np.linalg.norm(f_ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_batch_against_oracle(precision):
+    systems, positions = clusters([2, 5, 13, 1, 40, 22, 64, 3])
+    batch = ih.Batch(systems, precision)
+    energies, forces = batch.evaluate(positions)
+    check_against_oracle(systems, positions, energies, forces, precision)
+    batch.close()
+
+
+@pytest.mark.gpu
+def test_same_result_alone_and_in_a_batch():
+    systems, positions = clusters([7, 30, 2, 90, 11], seed=5)
+    batch = ih.Batch(systems, "mixed")
+    energies, forces = batch.evaluate(positions)
+    for k in (0, 3, 4):
+        alone = ih.Batch([systems[k]], "mixed")
+        e1, f1 = alone.evaluate([positions[k]])
+        assert e1[0] == energies[k]
+        assert np.array_equal(f1[0], forces[k])
+        alone.close()
+    batch.close()
+
+
+@pytest.mark.gpu
+def test_graph_replays_equal_eager_launches():
+    systems, positions = clusters([12, 3, 48, 20], seed=9)
+    graph = ih.Batch(systems, "mixed")
+    eager = ih.Batch(systems, "mixed")
+    eager.set_graph(False)
+    rng = np.random.default_rng(2)
+    for step in range(5):
+        pos = [p+rng.normal(0, 0.003, size=p.shape) for p in positions]
+        e_g, f_g = graph.evaluate(pos)
+        e_e, f_e = eager.evaluate(pos)
+        assert np.array_equal(e_g, e_e), step
+        for a, b in zip(f_g, f_e):
+            assert np.array_equal(a, b), step
+    graph.close()
+    eager.close()
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results():
+    systems, positions = clusters([16, 40, 5, 64], seed=3)
+    batch = ih.Batch(systems, "mixed")
+    e0, f0 = batch.evaluate(positions)
+    chosen = batch.autotune()
+    assert chosen > 0
+    e1, f1 = batch.evaluate(positions)
+    assert np.array_equal(e0, e1)
+    for a, b in zip(f0, f1):
+        assert np.array_equal(a, b)
+    batch.close()
+
+
+@pytest.mark.gpu
+def test_batch_rejects_cutoff_methods():
+    system, pos = workloads.config0()
+    with pytest.raises(ih.InteraxError, match="NoCutoff"):
+        ih.Batch([system])
