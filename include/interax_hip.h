@@ -181,6 +181,10 @@ int ixh_pair_long_range_energy(ixh_pair* pair, const double* box, double lambda,
  */
 int ixh_pair_create_segmented(int device, const ixh_pair_desc* desc, int num_segments, const int* particle_offsets, ixh_pair** pair);
 int ixh_pair_execute_segments(ixh_pair* pair, const ixh_buffers* buffers, int terms, int flags, double* segment_energies);
+/* The same for the listed segments only (host array of num_selected indices); the others are not evaluated and their
+   forces and energies are left as they are. */
+int ixh_pair_execute_subset(ixh_pair* pair, const ixh_buffers* buffers, int terms, int flags, double* segment_energies,
+                            const int* segments, int num_selected);
 /* Number of interacting pairs of the last execute() (waits for it). */
 int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
 /* Size of the current neighbour list (no counterpart in the reference): the number of 64 x 64 tiles and of candidate
@@ -231,6 +235,8 @@ int ixh_threebody_execute(ixh_threebody* threebody, const ixh_buffers* buffers, 
 int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* desc, int num_segments, const int* site_offsets,
                                    ixh_threebody** threebody);
 int ixh_threebody_execute_segments(ixh_threebody* threebody, const ixh_buffers* buffers, int flags, double* segment_energies);
+int ixh_threebody_execute_subset(ixh_threebody* threebody, const ixh_buffers* buffers, int flags, double* segment_energies,
+                                 const int* segments, int num_selected);
 /* Number of interacting triples of the last execute() (waits for it). */
 int ixh_threebody_last_triple_count(ixh_threebody* threebody, long long* count);
 int ixh_threebody_kernel_time(ixh_threebody* threebody, double* milliseconds, long long* launches);
@@ -262,6 +268,30 @@ int ixh_batch_get_energies(ixh_batch* batch, double* energies);          /* wait
 int ixh_batch_get_forces(ixh_batch* batch, double* forces);              /* waits; 3 x total particles */
 int ixh_batch_set_graph(ixh_batch* batch, int enable);                   /* 0: launch the kernels eagerly */
 int ixh_batch_autotune(ixh_batch* batch, int flags, int* chosen_workgroups);
+
+/*
+ * Serving (no counterpart in the reference): a request loop over device-resident systems, with its own thread.
+ * Register systems, start, then submit requests (coordinates, lambda) and wait for their tickets.  Cluster systems
+ * (NoCutoff pair forces and an optional nonperiodic three-body force, all with the same settings) form a catalog that
+ * is evaluated in batches: whenever one of two execution lanes is free, the pending cluster requests are admitted
+ * FIFO up to max_batch requests and memory_budget bytes of work in flight.  Periodic systems (cutoff methods, e.g.
+ * alchemical boxes evaluated at a lambda) have their own handle and are evaluated one request at a time.  A
+ * request's result does not depend on the other requests of its batch.
+ * ixh_server_replay submits n requests at the given arrival times (seconds from the call) and returns each one's
+ * latency (scheduled arrival to completion) and energy, and the time from the call to the last completion.
+ */
+typedef struct ixh_server ixh_server;
+int ixh_server_create(int device, int precision, int max_batch, long long memory_budget, ixh_server** server);
+int ixh_server_add_cluster(ixh_server* server, const ixh_pair_desc* pair, const ixh_threebody_desc* threebody, int* entry);
+int ixh_server_add_periodic(ixh_server* server, const ixh_pair_desc* pair, const double* box, int* entry);
+int ixh_server_start(ixh_server* server);
+int ixh_server_submit(ixh_server* server, int entry, const double* positions, double lambda, long long* ticket);
+int ixh_server_wait(ixh_server* server, long long ticket, double* energy, double* forces, double* latency_ms);
+int ixh_server_statistics(ixh_server* server, long long* batches, long long* batched_requests);
+int ixh_server_replay(ixh_server* server, int n, const double* arrival_s, const int* entries, const double* lambdas,
+                      const long long* position_offsets, const double* positions, double* latency_ms, double* energies,
+                      double* makespan_s);
+int ixh_server_destroy(ixh_server* server);
 
 #ifdef __cplusplus
 }

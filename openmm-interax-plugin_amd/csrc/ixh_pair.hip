@@ -569,6 +569,7 @@ struct TileArgs {
     double* waveEnergy;   // per wave of the launch: energy and pair count of its tiles, in tile order
     int* waveCount;
     double* tileEnergy;   // segmented handles: the energy of every tile, for the per-segment sums
+    const int* tileIndex;  // subsets of segments: the tiles to evaluate (numTiles of them); nullptr: all tiles
 };
 
 // Copy the type tables into LDS (all threads of the workgroup) and describe them.
@@ -770,7 +771,8 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
     double energy = 0;
     int count = 0;
 
-    for (long long t = begin; t < end; t++) {
+    for (long long item = begin; item < end; item++) {
+        const long long t = (a.tileIndex != nullptr ? a.tileIndex[item] : item);
         const int2 tile = a.tiles[t];
         const int I = tile.x;
         if (I != curI) {
@@ -1114,9 +1116,10 @@ __global__ void __launch_bounds__(1024) k_reduce_waves(const double* waveEnergy,
 
 // Energy of every segment: the sum of its tiles' energies in tile order (one workgroup per segment, fixed tree),
 // added to energies[s].
-__global__ void __launch_bounds__(256) k_reduce_segments(const double* tileEnergy, const int* segTiles, double* energies) {
+__global__ void __launch_bounds__(256) k_reduce_segments(const double* tileEnergy, const int* segTiles, const int* segList,
+                                                          double* energies) {
     __shared__ double sE[256];
-    const int seg = blockIdx.x;
+    const int seg = (segList != nullptr ? segList[blockIdx.x] : blockIdx.x);
     const int begin = segTiles[seg], end = segTiles[seg+1];
     double e = 0;
     for (int t = begin+threadIdx.x; t < end; t += 256)
@@ -1290,6 +1293,11 @@ struct ixh_pair {
     DeviceArray<int> segTilesDev;
     DeviceArray<double> tileEnergy;
     int gridLimit = 4096;   // largest number of workgroups of a tile launch
+    // subsets of segments: the tile list and segment list of the last call (pinned staging, device copies)
+    int* subsetStaging = nullptr;
+    size_t subsetCapacity = 0;
+    hipEvent_t subsetCopied = nullptr;
+    DeviceArray<int> subsetTiles, subsetSegments;
     // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
     bool useTables = false;
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
@@ -1876,7 +1884,8 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
 }
 
 template <typename Real>
-void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, double* segmentEnergies, hipStream_t stream) {
+void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, double* segmentEnergies, hipStream_t stream,
+                  int subsetTiles = -1, int subsetSegments = 0) {
     typedef typename Vec4T<Real>::type R4;
     TileArgs<Real> a;
     a.tiles = p->tiles.ptr;
@@ -1937,7 +1946,12 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
     a.srOffset = p->srOffset;
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
-    const int blocks = std::max(1, std::min((p->numTiles+3)/4, p->gridLimit));
+    a.tileIndex = nullptr;
+    if (subsetTiles >= 0) {
+        a.numTiles = subsetTiles;
+        a.tileIndex = p->subsetTiles.ptr;
+    }
+    const int blocks = std::max(1, std::min((a.numTiles+3)/4, p->gridLimit));
     a.tileEnergy = nullptr;
     if (p->numSegments > 0) {
         p->tileEnergy.reserve(std::max(p->numTiles, 1));
@@ -1948,7 +1962,7 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
     p->waveCount.reserve(numWaves);
     a.waveEnergy = p->waveEnergy.ptr;
     a.waveCount = p->waveCount.ptr;
-    if (p->numTiles > 0) {
+    if (a.numTiles > 0) {
         bool timed = p->timer.start(stream);
         bool compact = (p->method != IXH_NO_CUTOFF);
         switch (terms) {
@@ -1983,9 +1997,13 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
                            p->pairCount.ptr, addEnergy);
     IXH_HIP_CHECK(hipGetLastError());
     if (segmentEnergies != nullptr && p->numSegments > 0) {
-        hipLaunchKernelGGL(k_reduce_segments, dim3(p->numSegments), dim3(256), 0, stream, p->tileEnergy.ptr, p->segTilesDev.ptr,
-                           segmentEnergies);
-        IXH_HIP_CHECK(hipGetLastError());
+        const bool subset = (subsetTiles >= 0);
+        const int count = (subset ? subsetSegments : p->numSegments);
+        if (count > 0) {
+            hipLaunchKernelGGL(k_reduce_segments, dim3(count), dim3(256), 0, stream, p->tileEnergy.ptr, p->segTilesDev.ptr,
+                               subset ? (const int*) p->subsetSegments.ptr : (const int*) nullptr, segmentEnergies);
+            IXH_HIP_CHECK(hipGetLastError());
+        }
     }
 }
 
@@ -2103,6 +2121,10 @@ int ixh_pair_destroy(ixh_pair* p) {
             (void) hipHostFree(p->hostFlag);
         if (p->done != nullptr)
             (void) hipEventDestroy(p->done);
+        if (p->subsetCopied != nullptr)
+            (void) hipEventDestroy(p->subsetCopied);
+        if (p->subsetStaging != nullptr)
+            (void) hipHostFree(p->subsetStaging);
         delete p;
     });
 }
@@ -2134,7 +2156,14 @@ int ixh_pair_execute(ixh_pair* p, const ixh_buffers* b, int terms, int flags) {
 }
 
 int ixh_pair_execute_segments(ixh_pair* p, const ixh_buffers* b, int terms, int flags, double* segmentEnergies) {
+    return ixh_pair_execute_subset(p, b, terms, flags, segmentEnergies, nullptr, -1);
+}
+
+int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int flags, double* segmentEnergies, const int* segments,
+                            int numSelected) {
     return guarded([&] {
+        if (numSelected >= 0 && (p == nullptr || p->numSegments == 0 || (numSelected > 0 && segments == nullptr)))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute_subset: a subset needs a segmented handle and a segment list");
         if (p == nullptr || b == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: null argument");
         if (b->num_particles != p->n)
@@ -2175,17 +2204,57 @@ int ixh_pair_execute_segments(ixh_pair* p, const ixh_buffers* b, int terms, int 
         p->local.reserve(r4*p->padded);
         p->centers.reserve(3*(size_t) p->numBlocks);
         int periodic = (p->method == IXH_CUTOFF_PERIODIC);
+        int subsetTileCount = -1;
+        if (numSelected >= 0) {
+            // the tiles of the selected segments, in segment order
+            size_t needed = 0;
+            for (int k = 0; k < numSelected; k++) {
+                const int g = segments[k];
+                if (g < 0 || g >= p->numSegments)
+                    throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute_subset: segment index out of range");
+                needed += p->segTiles[g+1]-p->segTiles[g];
+            }
+            needed += numSelected;
+            if (p->subsetCopied == nullptr)
+                IXH_HIP_CHECK(hipEventCreateWithFlags(&p->subsetCopied, hipEventDisableTiming));
+            else
+                IXH_HIP_CHECK(hipEventSynchronize(p->subsetCopied));   // the staging of the previous call has been read
+            if (needed > p->subsetCapacity) {
+                if (p->subsetStaging != nullptr)
+                    IXH_HIP_CHECK(hipHostFree(p->subsetStaging));
+                p->subsetStaging = nullptr;
+                p->subsetCapacity = std::max<size_t>(needed, 1024);
+                IXH_HIP_CHECK(hipHostMalloc((void**) &p->subsetStaging, sizeof(int)*p->subsetCapacity, hipHostMallocDefault));
+            }
+            int* tileList = p->subsetStaging;
+            int count = 0;
+            for (int k = 0; k < numSelected; k++)
+                for (int t = p->segTiles[segments[k]]; t < p->segTiles[segments[k]+1]; t++)
+                    tileList[count++] = t;
+            std::copy(segments, segments+numSelected, tileList+count);
+            p->subsetTiles.reserve(std::max(count, 1));
+            p->subsetSegments.reserve(std::max(numSelected, 1));
+            if (count > 0)
+                IXH_HIP_CHECK(hipMemcpyAsync(p->subsetTiles.ptr, tileList, sizeof(int)*count, hipMemcpyHostToDevice, stream));
+            if (numSelected > 0)
+                IXH_HIP_CHECK(hipMemcpyAsync(p->subsetSegments.ptr, tileList+count, sizeof(int)*numSelected, hipMemcpyHostToDevice,
+                                             stream));
+            IXH_HIP_CHECK(hipEventRecord(p->subsetCopied, stream));
+            subsetTileCount = count;
+        }
         if (p->precision == IXH_DOUBLE) {
             hipLaunchKernelGGL(k_local_positions<double>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
                                p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (double4*) p->local.ptr);
             IXH_HIP_CHECK(hipGetLastError());
-            launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream);
+            launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, subsetTileCount,
+                                 std::max(numSelected, 0));
         }
         else {
             hipLaunchKernelGGL(k_local_positions<float>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
                                p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (float4*) p->local.ptr);
             IXH_HIP_CHECK(hipGetLastError());
-            launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream);
+            launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, subsetTileCount,
+                                std::max(numSelected, 0));
         }
         IXH_HIP_CHECK(hipEventRecord(p->done, stream));
     });

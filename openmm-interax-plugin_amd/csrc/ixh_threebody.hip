@@ -258,6 +258,7 @@ struct TripleArgs {
     const double* siteParams;    // alpha, frequency, width, alchemical per site
     const int* start;
     const int* count;            // segmented handles: neighbours of site i = [start[i], start[i]+count[i])
+    const int* siteIndex;        // subsets: the site of every workgroup; nullptr: site = workgroup index
     const int* nbrIndex;
     const double4* nbrDelta;
     int numSites;
@@ -312,7 +313,7 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     unsigned long long* sForce = (unsigned long long*) (sParam+t.capacity);   // 3 per neighbour
     int* sIndex = (int*) (sForce+3*(size_t) t.capacity);
     int* sMol = sIndex+t.capacity;
-    const int i = blockIdx.x;
+    const int i = (t.siteIndex != nullptr ? t.siteIndex[blockIdx.x] : (int) blockIdx.x);
     const int lane = threadIdx.x;
     const int begin = t.start[i], m = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
     for (int a = lane; a < m; a += 64) {
@@ -479,8 +480,8 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
 // Segmented neighbour lists: site i's candidates are the later sites of its own segment, [i+1, siteEnd[i]); the hits
 // (another molecule, within the cutoff, no periodic images) are written from start[i], their number to count[i].
 __global__ void k_neighbors_segments(const double* sites, const int* molecule, const int* siteEnd, double cutoff2, const int* start,
-                                     int* count, int* nbrIndex, double4* nbrDelta) {
-    const int i = blockIdx.x;
+                                     int* count, int* nbrIndex, double4* nbrDelta, const int* siteIndex) {
+    const int i = (siteIndex != nullptr ? siteIndex[blockIdx.x] : (int) blockIdx.x);
     const int lane = threadIdx.x;
     const double* si = sites+3*i;
     const int mi = molecule[i];
@@ -510,9 +511,10 @@ __global__ void k_neighbors_segments(const double* sites, const int* molecule, c
 
 // Energy of every segment: its sites' energies summed in site order (one workgroup per segment, fixed tree), added to
 // energies[s].
-__global__ void __launch_bounds__(256) k_reduce_site_segments(const double* siteEnergy, const int* segOffsets, double* energies) {
+__global__ void __launch_bounds__(256) k_reduce_site_segments(const double* siteEnergy, const int* segOffsets, const int* segList,
+                                                               double* energies) {
     __shared__ double sE[256];
-    const int seg = blockIdx.x;
+    const int seg = (segList != nullptr ? segList[blockIdx.x] : (int) blockIdx.x);
     double e = 0;
     for (int k = segOffsets[seg]+threadIdx.x; k < segOffsets[seg+1]; k += 256)
         e += siteEnergy[k];
@@ -612,6 +614,11 @@ struct ixh_threebody {
     std::vector<int> segOffsets;
     DeviceArray<int> segOffsetsDev, siteEnd;
     int maxSegment = 0;
+    // subsets of segments: site list and segment list of the last call
+    int* subsetStaging = nullptr;
+    size_t subsetCapacity = 0;
+    hipEvent_t subsetCopied = nullptr;
+    DeviceArray<int> subsetSites, subsetSegments;
     DeviceArray<double> paramsDev, sites, siteEnergy;
     DeviceArray<double4> nbrDelta;
     DeviceArray<long long> siteForce, tripleCount;
@@ -734,7 +741,8 @@ CellGrid make_grid(ixh_threebody* h, const double* box, hipStream_t stream) {
 }
 
 template <typename Real>
-void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, double cutoff2, int maxCount, hipStream_t stream) {
+void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, double cutoff2, int maxCount, hipStream_t stream,
+                    int subsetSites = -1) {
     typedef typename Vec4Of<Real>::type R4;
     TripleArgs<Real> t;
     t.sites = h->sites.ptr;
@@ -742,6 +750,7 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.siteParams = h->paramsDev.ptr;
     t.start = h->start.ptr;
     t.count = (h->numSegments > 0 ? h->count.ptr : nullptr);
+    t.siteIndex = (subsetSites >= 0 ? h->subsetSites.ptr : nullptr);
     t.nbrIndex = h->nbrIndex.ptr;
     t.nbrDelta = h->nbrDelta.ptr;
     t.numSites = h->numSites;
@@ -772,7 +781,9 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     if (shared > 160*1024)
         throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_execute: a site has too many neighbours for one workgroup");
     bool timed = h->timer.start(stream);
-    hipLaunchKernelGGL(k_triples<Real>, dim3(h->numSites), dim3(64), shared, stream, t);
+    const int workgroups = (subsetSites >= 0 ? subsetSites : h->numSites);
+    if (workgroups > 0)
+        hipLaunchKernelGGL(k_triples<Real>, dim3(workgroups), dim3(64), shared, stream, t);
     IXH_HIP_CHECK(hipGetLastError());
     if (timed)
         h->timer.stop(stream);
@@ -869,6 +880,12 @@ int ixh_threebody_destroy(ixh_threebody* h) {
     return guarded([&] {
         if (h == nullptr)
             return;
+        if (h->subsetCopied != nullptr) {
+            (void) hipEventSynchronize(h->subsetCopied);
+            (void) hipEventDestroy(h->subsetCopied);
+        }
+        if (h->subsetStaging != nullptr)
+            (void) hipHostFree(h->subsetStaging);
         DeviceGuard guard(h->device);
         (void) hipDeviceSynchronize();
         if (h->done != nullptr)
@@ -898,7 +915,14 @@ int ixh_threebody_execute(ixh_threebody* h, const ixh_buffers* b, int flags) {
 }
 
 int ixh_threebody_execute_segments(ixh_threebody* h, const ixh_buffers* b, int flags, double* segmentEnergies) {
+    return ixh_threebody_execute_subset(h, b, flags, segmentEnergies, nullptr, -1);
+}
+
+int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int flags, double* segmentEnergies, const int* segments,
+                                 int numSelected) {
     return guarded([&] {
+        if (numSelected >= 0 && (h == nullptr || h->numSegments == 0 || (numSelected > 0 && segments == nullptr)))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute_subset: a subset needs a segmented handle and a segment list");
         if (h == nullptr || b == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute: null argument");
         if (b->num_particles != h->numParticles)
@@ -921,26 +945,67 @@ int ixh_threebody_execute_segments(ixh_threebody* h, const ixh_buffers* b, int f
         double cutoff2 = h->cutoff*h->cutoff;
         if (h->numSegments > 0) {
             // segmented: static list capacity, no host synchronization
-            hipLaunchKernelGGL(k_neighbors_segments, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, h->siteEnd.ptr,
-                               cutoff2, h->start.ptr, h->count.ptr, h->nbrIndex.ptr, h->nbrDelta.ptr);
+            int subsetSites = -1;
+            if (numSelected >= 0) {
+                size_t needed = numSelected;
+                for (int k = 0; k < numSelected; k++) {
+                    const int g = segments[k];
+                    if (g < 0 || g >= h->numSegments)
+                        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute_subset: segment index out of range");
+                    needed += h->segOffsets[g+1]-h->segOffsets[g];
+                }
+                if (h->subsetCopied == nullptr)
+                    IXH_HIP_CHECK(hipEventCreateWithFlags(&h->subsetCopied, hipEventDisableTiming));
+                else
+                    IXH_HIP_CHECK(hipEventSynchronize(h->subsetCopied));
+                if (needed > h->subsetCapacity) {
+                    if (h->subsetStaging != nullptr)
+                        IXH_HIP_CHECK(hipHostFree(h->subsetStaging));
+                    h->subsetStaging = nullptr;
+                    h->subsetCapacity = std::max<size_t>(needed, 1024);
+                    IXH_HIP_CHECK(hipHostMalloc((void**) &h->subsetStaging, sizeof(int)*h->subsetCapacity, hipHostMallocDefault));
+                }
+                int count = 0;
+                for (int k = 0; k < numSelected; k++)
+                    for (int i = h->segOffsets[segments[k]]; i < h->segOffsets[segments[k]+1]; i++)
+                        h->subsetStaging[count++] = i;
+                std::copy(segments, segments+numSelected, h->subsetStaging+count);
+                h->subsetSites.reserve(std::max(count, 1));
+                h->subsetSegments.reserve(std::max(numSelected, 1));
+                if (count > 0)
+                    IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSites.ptr, h->subsetStaging, sizeof(int)*count, hipMemcpyHostToDevice, stream));
+                if (numSelected > 0)
+                    IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSegments.ptr, h->subsetStaging+count, sizeof(int)*numSelected,
+                                                 hipMemcpyHostToDevice, stream));
+                IXH_HIP_CHECK(hipEventRecord(h->subsetCopied, stream));
+                subsetSites = count;
+            }
+            const int workgroups = (subsetSites >= 0 ? subsetSites : n);
+            if (workgroups > 0)
+                hipLaunchKernelGGL(k_neighbors_segments, dim3(workgroups), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr,
+                                   h->siteEnd.ptr, cutoff2, h->start.ptr, h->count.ptr, h->nbrIndex.ptr, h->nbrDelta.ptr,
+                                   subsetSites >= 0 ? (const int*) h->subsetSites.ptr : (const int*) nullptr);
             IXH_HIP_CHECK(hipGetLastError());
             hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
             IXH_HIP_CHECK(hipGetLastError());
             if (h->precision == IXH_DOUBLE)
-                launch_triples<double>(h, b, box, cutoff2, h->maxSegment, stream);
+                launch_triples<double>(h, b, box, cutoff2, h->maxSegment, stream, subsetSites);
             else
-                launch_triples<float>(h, b, box, cutoff2, h->maxSegment, stream);
+                launch_triples<float>(h, b, box, cutoff2, h->maxSegment, stream, subsetSites);
             if (flags & IXH_INCLUDE_FORCES) {
                 hipLaunchKernelGGL(k_apply_site_forces, dim3(grid_of(n, 128)), dim3(128), 0, stream, h->siteForce.ptr, n,
                                    h->offsetsDev.ptr, h->particlesDev.ptr, b->force, b->force_stride);
                 IXH_HIP_CHECK(hipGetLastError());
             }
-            hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
-                               h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
-            IXH_HIP_CHECK(hipGetLastError());
-            if (segmentEnergies != nullptr) {
-                hipLaunchKernelGGL(k_reduce_site_segments, dim3(h->numSegments), dim3(256), 0, stream, h->siteEnergy.ptr,
-                                   h->segOffsetsDev.ptr, segmentEnergies);
+            if (subsetSites < 0) {
+                hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
+                                   h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
+                IXH_HIP_CHECK(hipGetLastError());
+            }
+            const int reduced = (subsetSites >= 0 ? numSelected : h->numSegments);
+            if (segmentEnergies != nullptr && reduced > 0) {
+                hipLaunchKernelGGL(k_reduce_site_segments, dim3(reduced), dim3(256), 0, stream, h->siteEnergy.ptr, h->segOffsetsDev.ptr,
+                                   subsetSites >= 0 ? (const int*) h->subsetSegments.ptr : (const int*) nullptr, segmentEnergies);
                 IXH_HIP_CHECK(hipGetLastError());
             }
             IXH_HIP_CHECK(hipEventRecord(h->done, stream));

@@ -123,6 +123,18 @@ SIGNATURES = {
     "ixh_batch_get_forces": [ctypes.c_void_p, _dp],
     "ixh_batch_set_graph": [ctypes.c_void_p, ctypes.c_int],
     "ixh_batch_autotune": [ctypes.c_void_p, ctypes.c_int, _ip],
+    "ixh_pair_execute_subset": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int, ctypes.c_void_p, _ip,
+                                ctypes.c_int],
+    "ixh_threebody_execute_subset": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_void_p, _ip, ctypes.c_int],
+    "ixh_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)],
+    "ixh_server_add_cluster": [ctypes.c_void_p, ctypes.POINTER(PairDesc), ctypes.POINTER(ThreeBodyDesc), _ip],
+    "ixh_server_add_periodic": [ctypes.c_void_p, ctypes.POINTER(PairDesc), _dp, _ip],
+    "ixh_server_start": [ctypes.c_void_p],
+    "ixh_server_submit": [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.POINTER(ctypes.c_longlong)],
+    "ixh_server_wait": [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _dp],
+    "ixh_server_statistics": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
+    "ixh_server_replay": [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _dp, ctypes.POINTER(ctypes.c_longlong), _dp, _dp, _dp, _dp],
+    "ixh_server_destroy": [ctypes.c_void_p],
 }
 
 _lib = None
@@ -984,4 +996,95 @@ class Batch:
     def close(self) -> None:
         if self.handle:
             _lib.ixh_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+class Server:
+    """The serving loop of the native library (its own thread): register Systems, start, then submit coordinates and
+    wait for the results.  Cluster Systems (NoCutoff pair forces and an optional nonperiodic three-body force, all with
+    the same settings) are evaluated in batches admitted as execution lanes and the memory budget free up; periodic
+    Systems (cutoff methods, e.g. alchemical boxes evaluated at a given lambda) one request at a time."""
+
+    def __init__(self, precision: str = "mixed", max_batch: int = 64, memory_budget: int = 8 << 30, device: int = 0):
+        load_library()
+        self.handle = ctypes.c_void_p()
+        _check(_lib.ixh_server_create(device, PRECISIONS[precision], max_batch, memory_budget, ctypes.byref(self.handle)))
+        self.precision = PRECISIONS[precision]
+        self.sizes: List[int] = []
+        self._keep = []
+
+    def add_system(self, system: "System") -> int:
+        """Register a System; returns its entry index."""
+        pair_forces = [f for f in system.getForces() if isinstance(f, InteraxPairForce)]
+        tbs = [f for f in system.getForces() if isinstance(f, InteraxThreeBodyForce)]
+        if not pair_forces:
+            raise InteraxError("Server.add_system: the System has no pair force")
+        group = _PairGroup.__new__(_PairGroup)
+        group.forces, group.system, group.precision = pair_forces, system, self.precision
+        desc, keep = group._describe()
+        self._keep.append(keep)
+        entry = ctypes.c_int()
+        if pair_forces[0].getNonbondedMethod() == InteraxPairForce.NoCutoff:
+            tdesc = None
+            if tbs:
+                tb = _ThreeBody.__new__(_ThreeBody)
+                tb.force, tb.system, tb.precision = tbs[0], system, self.precision
+                tdesc, tkeep = tb._describe()
+                self._keep.append(tkeep)
+            _check(_lib.ixh_server_add_cluster(self.handle, ctypes.byref(desc), ctypes.byref(tdesc) if tdesc is not None else None,
+                                               ctypes.byref(entry)))
+        else:
+            if tbs:
+                raise InteraxError("Server.add_system: periodic Systems with a three-body force are not served")
+            box = np.ascontiguousarray(system.getDefaultPeriodicBoxVectors().reshape(-1))
+            self._keep.append(box)
+            _check(_lib.ixh_server_add_periodic(self.handle, ctypes.byref(desc), _dptr(box), ctypes.byref(entry)))
+        self.sizes.append(system.getNumParticles())
+        return entry.value
+
+    def start(self) -> None:
+        _check(_lib.ixh_server_start(self.handle))
+
+    def submit(self, entry: int, positions: np.ndarray, lam: float = 1.0) -> int:
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
+        if pos.size != 3*self.sizes[entry]:
+            raise InteraxError("Server.submit: wrong number of particles")
+        ticket = ctypes.c_longlong()
+        _check(_lib.ixh_server_submit(self.handle, entry, _dptr(pos), lam, ctypes.byref(ticket)))
+        return ticket.value
+
+    def wait(self, ticket: int, entry: int):
+        """(energy, forces, latency in ms) of a submitted request."""
+        e = ctypes.c_double()
+        ms = ctypes.c_double()
+        f = np.zeros(3*self.sizes[entry])
+        _check(_lib.ixh_server_wait(self.handle, ticket, ctypes.byref(e), _dptr(f), ctypes.byref(ms)))
+        return e.value, f.reshape(-1, 3), ms.value
+
+    def statistics(self):
+        b, r = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(_lib.ixh_server_statistics(self.handle, ctypes.byref(b), ctypes.byref(r)))
+        return b.value, r.value
+
+    def replay(self, arrival_s: np.ndarray, entries: np.ndarray, positions: List[np.ndarray], lambdas=None):
+        """Submit requests at the given arrival times (seconds from now) from a native thread; returns the latency (ms)
+        and energy of every request and the time to the last completion (s)."""
+        n = len(entries)
+        arrival = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        ent = np.ascontiguousarray(entries, dtype=np.int32)
+        lam = np.ascontiguousarray(np.ones(n) if lambdas is None else lambdas, dtype=np.float64)
+        flat = [np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(-1)) for p in positions]
+        offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum([f.size for f in flat])[:-1]]), dtype=np.int64)
+        allpos = np.ascontiguousarray(np.concatenate(flat))
+        latency = np.zeros(n)
+        energies = np.zeros(n)
+        makespan = ctypes.c_double()
+        _check(_lib.ixh_server_replay(self.handle, n, _dptr(arrival), _iptr(ent), _dptr(lam),
+                                      offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _dptr(allpos),
+                                      _dptr(latency), _dptr(energies), ctypes.byref(makespan)))
+        return latency, energies, makespan.value
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.ixh_server_destroy(self.handle)
             self.handle = ctypes.c_void_p()

@@ -215,3 +215,64 @@ def cluster_sizes(num_clusters: int = 1024, seed: int = 0, low: int = 2, high: i
     """Molecule counts log-uniform in [low, high]."""
     rng = np.random.default_rng(seed)
     return [int(round(math.exp(x))) for x in rng.uniform(math.log(low), math.log(high), size=num_clusters)]
+
+
+# A benzene-like rigid solute (6 heavy atoms on a 0.139 nm ring, 6 hydrogens at 0.248 nm): heavy-atom parameters
+# C6 1.8e-3 kJ mol^-1 nm^6, valence charge -4.0, Slater width 1/26 nm; hydrogens as the water hydrogens.
+SOLUTE_C6, SOLUTE_QV, SOLUTE_SLATER = 1.8e-3, -4.0, 1.0/26
+
+
+def alchemical_box(softcore_radius: float = 0.0, seed: int = 0, num_waters: int = 1000, box: float = 3.106,
+                   cutoff: float = 0.9):
+    """An alchemical job of the served configuration: a rigid 12-atom solute, every atom alchemical (its 66
+    intramolecular pairs excluded), at the centre of a periodic cube with water-like molecules (RSA-placed oxygens,
+    at least 0.25 nm apart and 0.3 nm from the solute), cutoff 0.9 nm, soft-core radius R."""
+    rng = np.random.default_rng(seed)
+    angles = np.arange(6)*np.pi/3
+    ring = np.stack([np.cos(angles), np.sin(angles), np.zeros(6)], -1)
+    solute = np.concatenate([0.139*ring, 0.248*ring])@random_rotations(rng, 1)[0].T+box/2
+    centers = []
+    while len(centers) < num_waters:
+        p = rng.uniform(0, box, size=3)
+        d = solute-p
+        d -= box*np.round(d/box)
+        if (d*d).sum(1).min() < 0.3**2:
+            continue
+        if centers:
+            c = np.array(centers)-p
+            c -= box*np.round(c/box)
+            if (c*c).sum(1).min() < 0.25**2:
+                continue
+        centers.append(p)
+    waters = water_sites(rng, np.array(centers))
+    positions = np.concatenate([solute, waters])
+    n = len(positions)
+    system = ih.System(n, [box, 0, 0, 0, box, 0, 0, 0, box])
+    disp, exch, sr = ih.InteraxDispersionForce(), ih.InteraxExchangeForce(), ih.InteraxShortRangeForce()
+    rates = rng.uniform(30.0, 45.0, size=n)
+    for i in range(n):
+        alch = i < 12
+        if i < 6:
+            m1 = math.sqrt(2*SOLUTE_C6/K_POL)
+            moments, qv, width = (m1, (0.01/3)*m1, ((1.2e-4-4.2*(0.01/3)**2)/4)*m1, K_POL*m1), SOLUTE_QV, SOLUTE_SLATER
+        else:
+            el = "H" if i < 12 or (i-12) % 3 != 0 else "O"
+            moments, qv, width = dispersion_moments(el), QV[el], SLATER[el]
+        disp.addParticle(*moments, 1.0/rates[i], alch)
+        exch.addParticle(1.5*abs(qv), width, alch)
+        sr.addParticle(-170*abs(qv), 900*abs(qv), width, alch)
+    for f in (disp, exch, sr):
+        f.setNonbondedMethod(ih.InteraxPairForce.CutoffPeriodic)
+        f.setCutoffDistance(cutoff)
+        f.setSoftcoreRadius(softcore_radius)
+        for i in range(12):
+            for j in range(i+1, 12):
+                f.addExclusion(i, j)
+        for m in range(num_waters):
+            o = 12+3*m
+            f.addExclusion(o, o+1)
+            f.addExclusion(o, o+2)
+            f.addExclusion(o+1, o+2)
+        system.addForce(f)
+    disp.setDampingScale(1.0)
+    return system, positions
