@@ -5,7 +5,7 @@ box (9.94 nm, cutoff 1.0 nm, long range correction), FP32 arithmetic with FP64 p
 forces.  A step is one force+energy evaluation on coordinates perturbed by N(0, 0.005 nm); the coordinates of every
 step are resident in HBM before the timed region starts.  One JSON line is printed.
 
-  python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--workload pair|triplet|batch|copy]
+  python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--workload pair|triplet|batch|serve|copy]
 """
 import argparse
 import ctypes
@@ -398,6 +398,65 @@ def bench_batch(args, sample_every=16):
     return result
 
 
+def bench_serve(args, catalog_size=256, requests_per_rate=2000, rates=(1000, 2000, 5000, 10000, 20000, 50000)):
+    """The served configuration: Poisson request streams at several rates, mixing cluster evaluations (a catalog of
+    water-like clusters with 2-1000 molecules, log-uniform, all pair forces without cutoff plus the molecular ATM force)
+    and alchemical lambda windows (a 12-atom alchemical solute in 1000 water-like molecules, periodic 3.106 nm box,
+    cutoff 0.9 nm, soft-core radius 0 or 0.3 nm, lambda_sterics in linspace(0, 1, 16)); 3 in 4 requests are clusters.
+    Coordinates of every request are the registered ones perturbed by N(0, 0.005 nm)."""
+    import interax_hip as ih
+    import workloads
+    t0 = time.time()
+    sizes = workloads.cluster_sizes(catalog_size, seed=7, low=2, high=1000)
+    catalog = [workloads.water_cluster(m, seed=10000+k) for k, m in enumerate(sizes)]
+    boxes = [workloads.alchemical_box(0.0, seed=1), workloads.alchemical_box(0.3, seed=2)]
+    server = ih.Server("mixed", max_batch=64)
+    entries = [server.add_system(s) for s, _ in catalog]
+    alch = [server.add_system(s) for s, _ in boxes]
+    server.start()
+    base = [p for _, p in catalog]+[p for _, p in boxes]
+    log(f"[bench] served catalog: {len(catalog)} clusters ({sum(len(p) for _, p in catalog)} atoms) and 2 alchemical "
+        f"boxes of {len(boxes[0][1])} atoms, built in {time.time()-t0:.1f} s")
+    # warm up every system once (list builds)
+    for e, p in zip(entries+alch, base):
+        server.wait(server.submit(e, p, 0.5), e)
+    rng = np.random.default_rng(11)
+    lambdas_grid = np.linspace(0, 1, 16)
+    table = []
+    for rate in rates:
+        n = requests_per_rate
+        arrival = np.cumsum(rng.exponential(1.0/rate, size=n))
+        is_cluster = rng.random(n) < 0.75
+        which = np.where(is_cluster, rng.integers(0, len(entries), size=n), len(entries)+rng.integers(0, 2, size=n))
+        lam = np.where(is_cluster, 1.0, lambdas_grid[rng.integers(0, 16, size=n)])
+        positions = [base[k]+rng.normal(0, 0.005, size=base[k].shape) for k in which]
+        b0, r0 = server.statistics()
+        latency, _, makespan = server.replay(arrival, np.array([(entries+alch)[k] for k in which]), positions, lam)
+        b1, r1 = server.statistics()
+        row = {"rate": rate, "requests": n, "throughput": n/makespan, "p50_ms": float(np.percentile(latency, 50)),
+               "p99_ms": float(np.percentile(latency, 99)),
+               "cluster_p50_ms": float(np.percentile(latency[is_cluster], 50)),
+               "cluster_p99_ms": float(np.percentile(latency[is_cluster], 99)),
+               "alchemical_p50_ms": float(np.percentile(latency[~is_cluster], 50)),
+               "alchemical_p99_ms": float(np.percentile(latency[~is_cluster], 99)),
+               "mean_cluster_batch": (r1-r0)/max(1, b1-b0)}
+        table.append(row)
+        log(f"[bench] rate {rate}/s: throughput {row['throughput']:.0f}/s, p50 {row['p50_ms']:.2f} ms, p99 {row['p99_ms']:.2f} ms, "
+            f"clusters p50/p99 {row['cluster_p50_ms']:.2f}/{row['cluster_p99_ms']:.2f} ms, alchemical p50/p99 "
+            f"{row['alchemical_p50_ms']:.2f}/{row['alchemical_p99_ms']:.2f} ms, mean batch {row['mean_cluster_batch']:.1f}")
+    server.close()
+    best = max(table, key=lambda r: r["throughput"])
+    return {"metric": "served requests/s with p50/p99 latency", "value": best["throughput"], "unit": "requests/s",
+            "n_gpus": 1, "steps": requests_per_rate, "warmup": 1, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "Poisson request streams: 3/4 cluster evaluations from a catalog of 256 water-like "
+                                   "clusters (2-1000 molecules, log-uniform, pair forces without cutoff + ATM), 1/4 "
+                                   "alchemical lambda windows (12-atom solute in 1000 waters, 3.106 nm box, cutoff 0.9 nm, "
+                                   "soft-core radius 0 or 0.3 nm, 16 lambda values)",
+                       "max_batch": 64, "lanes": 2},
+            "rates": table}
+
+
 def copy_calibration(args):
     """A device-to-device copy of a known size: the HBM counters of its blit kernel calibrate FETCH_SIZE and
     WRITE_SIZE (bytes per counted unit) for the traffic figures of the other kernels."""
@@ -420,7 +479,7 @@ def main():
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=100)
     parser.add_argument("--warmup", type=int, default=5)
-    parser.add_argument("--workload", default="pair", choices=["pair", "triplet", "batch", "copy"])
+    parser.add_argument("--workload", default="pair", choices=["pair", "triplet", "batch", "serve", "copy"])
     parser.add_argument("--skip-oracle", action="store_true", help="no accuracy check and no CPU baseline")
     args = parser.parse_args()
     if args.gpus != 1:
@@ -433,6 +492,8 @@ def main():
         result = bench_triplet(args)
     elif args.workload == "batch":
         result = bench_batch(args)
+    elif args.workload == "serve":
+        result = bench_serve(args)
     else:
         result = bench_pair(args)
     print(json.dumps(result), flush=True)
