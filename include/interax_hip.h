@@ -72,6 +72,10 @@ extern "C" {
 /* execute() flags */
 #define IXH_INCLUDE_FORCES 1
 #define IXH_INCLUDE_ENERGY 2
+/* Cutoff methods: check the neighbour list against the positions on the device without waiting for the answer; the
+   evaluation proceeds with the current list.  ixh_pair_list_was_valid() then tells whether it was still valid (if not,
+   the results must be discarded and the evaluation repeated without this flag, which rebuilds the list). */
+#define IXH_DEFER_LIST_CHECK 4
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* ixh_last_error(void);
@@ -185,6 +189,9 @@ int ixh_pair_execute_segments(ixh_pair* pair, const ixh_buffers* buffers, int te
    forces and energies are left as they are. */
 int ixh_pair_execute_subset(ixh_pair* pair, const ixh_buffers* buffers, int terms, int flags, double* segment_energies,
                             const int* segments, int num_selected);
+/* Whether the neighbour list used by the last execute() with IXH_DEFER_LIST_CHECK was valid for its positions
+   (waits for the check); 1 for any other execute(). */
+int ixh_pair_list_was_valid(ixh_pair* pair, int* valid);
 /* Number of interacting pairs of the last execute() (waits for it). */
 int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
 /* Size of the current neighbour list (no counterpart in the reference): the number of 64 x 64 tiles and of candidate

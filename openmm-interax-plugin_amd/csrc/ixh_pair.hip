@@ -469,6 +469,10 @@ __global__ void k_candidate_lists(const int2* tiles, const unsigned long long* m
         lists[start+count+lane] = 0xffff;
 }
 
+__global__ void k_set_int(int* p, int value) {
+    *p = value;
+}
+
 // Largest squared displacement since the list was built exceeds (skin/2)^2?
 __global__ void k_check_displacement(const void* posq, const float4* correction, int precision, int n, const double* buildPos,
                                      int periodic, BoxD box, double limit2, int* flag) {
@@ -1312,6 +1316,8 @@ struct ixh_pair {
     DeviceArray<int> flagDev;
     DeviceArray<long long> pairCount;
     int* hostFlag = nullptr;
+    hipEvent_t flagCopied = nullptr;   // deferred list checks: the flag has reached hostFlag
+    bool deferredCheck = false;
     hipStream_t lastStream = nullptr;
     hipEvent_t done = nullptr;   // recorded at the end of every execute: calls on other streams wait for it
     KernelTimer timer;
@@ -2123,6 +2129,8 @@ int ixh_pair_destroy(ixh_pair* p) {
             (void) hipEventDestroy(p->done);
         if (p->subsetCopied != nullptr)
             (void) hipEventDestroy(p->subsetCopied);
+        if (p->flagCopied != nullptr)
+            (void) hipEventDestroy(p->flagCopied);
         if (p->subsetStaging != nullptr)
             (void) hipHostFree(p->subsetStaging);
         delete p;
@@ -2189,14 +2197,23 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
         p->lastStream = stream;
         const float4* corr = (const float4*) b->posq_correction;
         bool rebuild = !p->built;
+        p->deferredCheck = false;
         if (!rebuild && p->method != IXH_NO_CUTOFF) {
-            IXH_HIP_CHECK(hipMemsetAsync(p->flagDev.ptr, 0, sizeof(int), stream));
+            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, stream, p->flagDev.ptr, 0);
             hipLaunchKernelGGL(k_check_displacement, dim3(grid_for(p->n, 256)), dim3(256), 0, stream, b->posq, corr, b->precision, p->n,
                                p->buildPos.ptr, (int) (p->method == IXH_CUTOFF_PERIODIC), make_box(b->box), 0.25*SKIN*SKIN, p->flagDev.ptr);
             IXH_HIP_CHECK(hipGetLastError());
             IXH_HIP_CHECK(hipMemcpyAsync(p->hostFlag, p->flagDev.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
-            IXH_HIP_CHECK(hipStreamSynchronize(stream));
-            rebuild = (*p->hostFlag != 0);
+            if (flags & IXH_DEFER_LIST_CHECK) {
+                if (p->flagCopied == nullptr)
+                    IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
+                IXH_HIP_CHECK(hipEventRecord(p->flagCopied, stream));
+                p->deferredCheck = true;
+            }
+            else {
+                IXH_HIP_CHECK(hipStreamSynchronize(stream));
+                rebuild = (*p->hostFlag != 0);
+            }
         }
         if (rebuild)
             build_list(p, b, stream);
@@ -2305,6 +2322,19 @@ int ixh_pair_reset_kernel_time(ixh_pair* p) {
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_reset_kernel_time: null handle");
         DeviceGuard guard(p->device);
         p->timer.reset();
+    });
+}
+
+int ixh_pair_list_was_valid(ixh_pair* p, int* valid) {
+    return guarded([&] {
+        if (p == nullptr || valid == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_list_was_valid: null argument");
+        *valid = 1;
+        if (p->deferredCheck) {
+            DeviceGuard guard(p->device);
+            IXH_HIP_CHECK(hipEventSynchronize(p->flagCopied));
+            *valid = (*p->hostFlag == 0);
+        }
     });
 }
 

@@ -95,11 +95,24 @@ struct Entry {
     std::vector<double> dispersion, exchange, shortRange, alpha, frequency, width;
     std::vector<unsigned char> alchemical, siteAlchemical;
     std::vector<int> exclusions, siteOffsets, siteParticles, molecule;
-    // periodic: the handle and its state
+    // periodic: the box and the execution slots of the system
+    double box[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<int> slots;
+    bool inFlight = false;
+};
+
+// One execution slot of a periodic system: a handle, a state, staging and results; one request at a time.
+struct PeriodicSlot {
+    int entry = 0;
     ixh_pair* pair = nullptr;
     ixh_state* state = nullptr;
-    double box[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    bool inFlight = false;
+    ixh_buffers buffers;
+    float4* hostPosq = nullptr;          // pinned staging
+    float4* hostCorrection = nullptr;
+    double* hostResults = nullptr;       // forces (3n) and the energy
+    DeviceArray<double> results;
+    hipEvent_t done = nullptr;
+    std::shared_ptr<Request> request;
 };
 
 struct Lane {
@@ -138,10 +151,8 @@ struct ixh_server {
     ixh_threebody_desc threebodyTemplate;
     bool haveThreebody = false;
     Lane lanes[2];
-    hipStream_t periodicStream = nullptr;
-    hipEvent_t periodicDone = nullptr;
-    std::shared_ptr<Request> periodicRequest;
-    std::vector<double> periodicForces;
+    std::vector<std::unique_ptr<PeriodicSlot>> periodicSlots;
+    int slotsPerPeriodic = 4;
     std::thread worker;
     std::mutex mutex;
     std::condition_variable submitted, completed;
@@ -363,32 +374,62 @@ void finish_batch(ixh_server* s, Lane& lane) {
     lane.busy = false;
 }
 
-void launch_periodic(ixh_server* s, const std::shared_ptr<Request>& r) {
-    Entry& e = *s->entries[r->entry];
-    check(ixh_state_set_positions(e.state, r->positions.data()));
-    check(ixh_state_set_lambda(e.state, r->lambda));
-    check(ixh_state_clear(e.state));
-    ixh_buffers b;
-    check(ixh_state_get_buffers(e.state, &b));
-    check(ixh_pair_execute(e.pair, &b, 7, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY));
-    IXH_HIP_CHECK(hipEventRecord(s->periodicDone, (hipStream_t) b.stream));
-    s->periodicRequest = r;
-    e.inFlight = true;
+// Fixed-point forces of all particles and the energy of a periodic slot into one array.
+__global__ void k_periodic_results(const long long* force, int stride, int n, const double* energy, double* out) {
+    for (int k = blockIdx.x*blockDim.x+threadIdx.x; k < n; k += gridDim.x*blockDim.x)
+        for (int c = 0; c < 3; c++)
+            out[3*(size_t) k+c] = (double) force[(size_t) c*stride+k]/FIXED_POINT_SCALE;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        out[3*(size_t) n] = *energy;
 }
 
-void finish_periodic(ixh_server* s) {
-    auto r = s->periodicRequest;
-    Entry& e = *s->entries[r->entry];
-    double energy = 0, lrc = 0;
-    r->forces.resize(3*(size_t) e.numParticles);
-    check(ixh_state_get_forces(e.state, r->forces.data()));
-    check(ixh_state_get_energy(e.state, &energy));
-    check(ixh_pair_long_range_energy(e.pair, e.box, r->lambda, &lrc));
-    r->energy = energy+lrc;
+// Evaluate a periodic request on a slot, asynchronously; the neighbour list is checked on the device.
+void launch_periodic(ixh_server* s, PeriodicSlot& slot, const std::shared_ptr<Request>& r, bool deferCheck) {
+    Entry& e = *s->entries[slot.entry];
+    const int n = e.numParticles;
+    const double* p = r->positions.data();
+    for (int i = 0; i < n; i++) {
+        float hx, hy, hz, lx, ly, lz;
+        split(p[3*i], hx, lx);
+        split(p[3*i+1], hy, ly);
+        split(p[3*i+2], hz, lz);
+        slot.hostPosq[i] = make_float4(hx, hy, hz, 0.0f);
+        slot.hostCorrection[i] = make_float4(lx, ly, lz, 0.0f);
+    }
+    hipStream_t st = (hipStream_t) slot.buffers.stream;
+    IXH_HIP_CHECK(hipMemcpyAsync((void*) slot.buffers.posq, slot.hostPosq, sizeof(float4)*n, hipMemcpyHostToDevice, st));
+    if (s->precision == IXH_MIXED)
+        IXH_HIP_CHECK(hipMemcpyAsync((void*) slot.buffers.posq_correction, slot.hostCorrection, sizeof(float4)*n, hipMemcpyHostToDevice, st));
+    check(ixh_state_clear(slot.state));
+    ixh_buffers b = slot.buffers;
+    b.lambda = r->lambda;
+    check(ixh_pair_execute(slot.pair, &b, 7, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY | (deferCheck ? IXH_DEFER_LIST_CHECK : 0)));
+    hipLaunchKernelGGL(k_periodic_results, dim3(std::max(1, std::min(64, (n+255)/256))), dim3(256), 0, st,
+                       (const long long*) b.force, b.force_stride, n, (const double*) b.energy, slot.results.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    IXH_HIP_CHECK(hipMemcpyAsync(slot.hostResults, slot.results.ptr, sizeof(double)*(3*(size_t) n+1), hipMemcpyDeviceToHost, st));
+    IXH_HIP_CHECK(hipEventRecord(slot.done, st));
+    slot.request = r;
+}
+
+void finish_periodic(ixh_server* s, PeriodicSlot& slot) {
+    auto r = slot.request;
+    Entry& e = *s->entries[slot.entry];
+    int valid = 1;
+    check(ixh_pair_list_was_valid(slot.pair, &valid));
+    if (!valid) {
+        // the coordinates moved beyond the list's skin: evaluate again, rebuilding the list
+        launch_periodic(s, slot, r, false);
+        IXH_HIP_CHECK(hipEventSynchronize(slot.done));
+    }
+    const int n = e.numParticles;
+    double lrc = 0;
+    check(ixh_pair_long_range_energy(slot.pair, e.box, r->lambda, &lrc));
+    r->forces.assign(slot.hostResults, slot.hostResults+3*(size_t) n);
+    r->energy = slot.hostResults[3*(size_t) n]+lrc;
     r->completion = Clock::now();
     r->done = true;
-    e.inFlight = false;
-    s->periodicRequest.reset();
+    slot.request.reset();
 }
 
 void serve(ixh_server* s) {
@@ -397,7 +438,9 @@ void serve(ixh_server* s) {
     while (true) {
         {
             std::unique_lock<std::mutex> lock(s->mutex);
-            bool anyBusy = s->lanes[0].busy || s->lanes[1].busy || s->periodicRequest != nullptr;
+            bool anyBusy = s->lanes[0].busy || s->lanes[1].busy;
+            for (auto& slot : s->periodicSlots)
+                anyBusy = anyBusy || slot->request != nullptr;
             if (!s->stopping && s->incoming.empty() && pending.empty() && !anyBusy)
                 s->submitted.wait(lock, [&] { return s->stopping || !s->incoming.empty(); });
             if (s->stopping && s->incoming.empty() && pending.empty() && !anyBusy)
@@ -415,18 +458,18 @@ void serve(ixh_server* s) {
                 finish_batch(s, lane);
                 progress = true;
             }
-        if (s->periodicRequest != nullptr && hipEventQuery(s->periodicDone) == hipSuccess) {
-            std::lock_guard<std::mutex> lock(s->mutex);
-            try {
-                finish_periodic(s);
+        for (auto& slot : s->periodicSlots)
+            if (slot->request != nullptr && hipEventQuery(slot->done) == hipSuccess) {
+                std::lock_guard<std::mutex> lock(s->mutex);
+                try {
+                    finish_periodic(s, *slot);
+                }
+                catch (const std::exception& ex) {
+                    fail_request(s, slot->request, ex.what());
+                    slot->request.reset();
+                }
+                progress = true;
             }
-            catch (const std::exception& ex) {
-                fail_request(s, s->periodicRequest, ex.what());
-                s->entries[s->periodicRequest->entry]->inFlight = false;
-                s->periodicRequest.reset();
-            }
-            progress = true;
-        }
         if (progress)
             s->completed.notify_all();
         // admissions: every free lane takes the pending cluster requests that fit
@@ -462,24 +505,33 @@ void serve(ixh_server* s) {
                 progress = true;
             }
         }
-        if (s->periodicRequest == nullptr)
-            for (auto it = pending.begin(); it != pending.end(); ++it) {
-                Entry& e = *s->entries[(*it)->entry];
-                if (e.periodic && !e.inFlight) {
-                    auto r = *it;
-                    pending.erase(it);
-                    try {
-                        launch_periodic(s, r);
+        // periodic requests: each takes a free slot of its system, in arrival order
+        for (auto it = pending.begin(); it != pending.end();) {
+            Entry& e = *s->entries[(*it)->entry];
+            PeriodicSlot* free = nullptr;
+            if (e.periodic)
+                for (int k : e.slots)
+                    if (s->periodicSlots[k]->request == nullptr) {
+                        free = s->periodicSlots[k].get();
+                        break;
                     }
-                    catch (const std::exception& ex) {
-                        std::lock_guard<std::mutex> lock(s->mutex);
-                        fail_request(s, r, ex.what());
-                        s->completed.notify_all();
-                    }
-                    progress = true;
-                    break;
-                }
+            if (free == nullptr) {
+                ++it;
+                continue;
             }
+            auto r = *it;
+            it = pending.erase(it);
+            try {
+                launch_periodic(s, *free, r, true);
+            }
+            catch (const std::exception& ex) {
+                std::lock_guard<std::mutex> lock(s->mutex);
+                free->request.reset();
+                fail_request(s, r, ex.what());
+                s->completed.notify_all();
+            }
+            progress = true;
+        }
         if (!progress)
             std::this_thread::sleep_for(std::chrono::microseconds(5));
     }
@@ -585,11 +637,25 @@ int ixh_server_add_periodic(ixh_server* s, const ixh_pair_desc* pair, const doub
         ixh_pair_desc d = *pair;
         d.precision = s->precision;
         d.default_box = box;
-        check(ixh_pair_create(s->device, &d, &e->pair));
-        check(ixh_state_create(s->device, e->numParticles, s->precision, &e->state));
-        check(ixh_state_set_box(e->state, box));
+        const int index = (int) s->entries.size();
+        const int n = e->numParticles;
+        for (int k = 0; k < s->slotsPerPeriodic; k++) {
+            std::unique_ptr<PeriodicSlot> slot(new PeriodicSlot());
+            slot->entry = index;
+            check(ixh_pair_create(s->device, &d, &slot->pair));
+            check(ixh_state_create(s->device, n, s->precision, &slot->state));
+            check(ixh_state_set_box(slot->state, box));
+            check(ixh_state_get_buffers(slot->state, &slot->buffers));
+            IXH_HIP_CHECK(hipHostMalloc((void**) &slot->hostPosq, sizeof(float4)*std::max(n, 1), hipHostMallocDefault));
+            IXH_HIP_CHECK(hipHostMalloc((void**) &slot->hostCorrection, sizeof(float4)*std::max(n, 1), hipHostMallocDefault));
+            IXH_HIP_CHECK(hipHostMalloc((void**) &slot->hostResults, sizeof(double)*(3*(size_t) n+1), hipHostMallocDefault));
+            slot->results.reserve(3*(size_t) n+1);
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming));
+            e->slots.push_back((int) s->periodicSlots.size());
+            s->periodicSlots.push_back(std::move(slot));
+        }
         e->footprint = 200LL*e->numParticles;
-        *entry = (int) s->entries.size();
+        *entry = index;
         s->entries.push_back(std::move(e));
     });
 }
@@ -602,7 +668,11 @@ int ixh_server_start(ixh_server* s) {
             return;
         DeviceGuard guard(s->device);
         build_catalog(s);
-        IXH_HIP_CHECK(hipEventCreateWithFlags(&s->periodicDone, hipEventDisableTiming));
+        // the first evaluation of every periodic slot builds its neighbour list (synchronously, before serving)
+        for (auto& slot : s->periodicSlots) {
+            ixh_buffers b = slot->buffers;
+            check(ixh_pair_execute(slot->pair, &b, 7, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY));
+        }
         s->started = true;
         s->worker = std::thread([s] {
             try {
@@ -746,14 +816,17 @@ int ixh_server_destroy(ixh_server* s) {
             if (lane.done != nullptr)
                 (void) hipEventDestroy(lane.done);
         }
-        for (auto& e : s->entries) {
-            if (e->pair != nullptr)
-                ixh_pair_destroy(e->pair);
-            if (e->state != nullptr)
-                ixh_state_destroy(e->state);
+        for (auto& slot : s->periodicSlots) {
+            if (slot->pair != nullptr)
+                ixh_pair_destroy(slot->pair);
+            if (slot->state != nullptr)
+                ixh_state_destroy(slot->state);
+            for (void* p : {(void*) slot->hostPosq, (void*) slot->hostCorrection, (void*) slot->hostResults})
+                if (p != nullptr)
+                    (void) hipHostFree(p);
+            if (slot->done != nullptr)
+                (void) hipEventDestroy(slot->done);
         }
-        if (s->periodicDone != nullptr)
-            (void) hipEventDestroy(s->periodicDone);
         delete s;
     });
 }

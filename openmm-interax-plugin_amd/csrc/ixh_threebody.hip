@@ -24,6 +24,9 @@ using namespace ixh;
 
 namespace {
 
+// Neighbour pairs per work item of the triple kernel of segmented handles (64 steps of a wavefront).
+constexpr int TRIPLE_CHUNK = 4096;
+
 struct BoxT {
     double ax, bx, by, cx, cy, cz;
 };
@@ -258,7 +261,10 @@ struct TripleArgs {
     const double* siteParams;    // alpha, frequency, width, alchemical per site
     const int* start;
     const int* count;            // segmented handles: neighbours of site i = [start[i], start[i]+count[i])
-    const int* siteIndex;        // subsets: the site of every workgroup; nullptr: site = workgroup index
+    const int* itemSite;         // work items: the site of item w (nullptr: item w is site w, all of its pairs)
+    const int* itemChunk;        // work items: the chunk of chunkPairs neighbour pairs of item w
+    const int* itemIndex;        // subsets: the item of every workgroup (nullptr: item = workgroup index)
+    int chunkPairs;
     const int* nbrIndex;
     const double4* nbrDelta;
     int numSites;
@@ -275,8 +281,8 @@ struct TripleArgs {
     Real invDampingRadius;       // 0: no centroid damping
     int capacity;                // LDS entries
     long long* siteForce;        // 3 x numSites fixed point
-    double* siteEnergy;
-    int* siteCount;
+    double* itemEnergy;          // energy and triple count of every work item
+    int* itemCount;
 };
 
 // InteraxThreeBodyTerms.h:51-65
@@ -303,7 +309,8 @@ __device__ __forceinline__ Real c9_of(Real a1, Real w1, Real a2, Real w2, Real a
     return Real(1.5)*a1*a2*a3*w1*w2*w3*(w1+w2+w3)/((w1+w2)*(w2+w3)*(w3+w1));
 }
 
-// 3. One wavefront per site A.
+// 3. One wavefront per work item: a site A and all of its neighbour pairs, or (segmented handles) a chunk of them, so
+// that the first sites of a large cluster, with O(n^2) pairs each, are shared out over several workgroups.
 template <typename Real>
 __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     typedef typename Vec4Of<Real>::type R4;
@@ -313,7 +320,8 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     unsigned long long* sForce = (unsigned long long*) (sParam+t.capacity);   // 3 per neighbour
     int* sIndex = (int*) (sForce+3*(size_t) t.capacity);
     int* sMol = sIndex+t.capacity;
-    const int i = (t.siteIndex != nullptr ? t.siteIndex[blockIdx.x] : (int) blockIdx.x);
+    const int item = (t.itemIndex != nullptr ? t.itemIndex[blockIdx.x] : (int) blockIdx.x);
+    const int i = (t.itemSite != nullptr ? t.itemSite[item] : item);
     const int lane = threadIdx.x;
     const int begin = t.start[i], m = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
     for (int a = lane; a < m; a += 64) {
@@ -333,10 +341,14 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     Real fA[3] = {0, 0, 0};
     double energy = 0;
     int count = 0;
-    const long long numPairs = (long long) m*(m-1)/2;
-    for (long long p0 = 0; p0 < numPairs; p0 += 64) {
+    long long pBegin = 0, pEnd = (long long) m*(m-1)/2;
+    if (t.itemChunk != nullptr) {
+        pBegin = (long long) t.itemChunk[item]*t.chunkPairs;
+        pEnd = min(pEnd, pBegin+t.chunkPairs);
+    }
+    for (long long p0 = pBegin; p0 < pEnd; p0 += 64) {
         long long p = p0+lane;
-        if (p < numPairs) {
+        if (p < pEnd) {
             // p = b(b-1)/2 + a, 0 <= a < b < m
             int b = (int) ((1.0+sqrt(1.0+8.0*(double) p))*0.5);
             while ((long long) b*(b-1)/2 > p)
@@ -472,8 +484,8 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     double e = wave_sum(energy);
     int c = (int) wave_sum((double) count);
     if (lane == 0) {
-        t.siteEnergy[i] = e;
-        t.siteCount[i] = c;
+        t.itemEnergy[item] = e;
+        t.itemCount[item] = c;
     }
 }
 
@@ -509,8 +521,8 @@ __global__ void k_neighbors_segments(const double* sites, const int* molecule, c
         count[i] = written;
 }
 
-// Energy of every segment: its sites' energies summed in site order (one workgroup per segment, fixed tree), added to
-// energies[s].
+// Energy of every segment: the energies of its work items (ranges[s] .. ranges[s+1]) summed in order (one workgroup
+// per segment, fixed tree), added to energies[s].
 __global__ void __launch_bounds__(256) k_reduce_site_segments(const double* siteEnergy, const int* segOffsets, const int* segList,
                                                                double* energies) {
     __shared__ double sE[256];
@@ -614,6 +626,11 @@ struct ixh_threebody {
     std::vector<int> segOffsets;
     DeviceArray<int> segOffsetsDev, siteEnd;
     int maxSegment = 0;
+    // work items of the triple kernel (segmented handles): site and pair chunk, items of segment s in
+    // [segItems[s], segItems[s+1])
+    int numItems = 0;
+    std::vector<int> segItems;
+    DeviceArray<int> itemSite, itemChunk, segItemsDev, subsetItems;
     // subsets of segments: site list and segment list of the last call
     int* subsetStaging = nullptr;
     size_t subsetCapacity = 0;
@@ -742,7 +759,7 @@ CellGrid make_grid(ixh_threebody* h, const double* box, hipStream_t stream) {
 
 template <typename Real>
 void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, double cutoff2, int maxCount, hipStream_t stream,
-                    int subsetSites = -1) {
+                    int subsetItems = -1) {
     typedef typename Vec4Of<Real>::type R4;
     TripleArgs<Real> t;
     t.sites = h->sites.ptr;
@@ -750,7 +767,11 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.siteParams = h->paramsDev.ptr;
     t.start = h->start.ptr;
     t.count = (h->numSegments > 0 ? h->count.ptr : nullptr);
-    t.siteIndex = (subsetSites >= 0 ? h->subsetSites.ptr : nullptr);
+    const bool items = (h->numSegments > 0);
+    t.itemSite = (items ? h->itemSite.ptr : nullptr);
+    t.itemChunk = (items ? h->itemChunk.ptr : nullptr);
+    t.itemIndex = (subsetItems >= 0 ? h->subsetItems.ptr : nullptr);
+    t.chunkPairs = TRIPLE_CHUNK;
     t.nbrIndex = h->nbrIndex.ptr;
     t.nbrDelta = h->nbrDelta.ptr;
     t.numSites = h->numSites;
@@ -775,13 +796,13 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.invDampingRadius = Real(h->centroidDampingRadius > 0 ? 1.0/h->centroidDampingRadius : 0.0);
     t.capacity = std::max(maxCount, 1);
     t.siteForce = h->siteForce.ptr;
-    t.siteEnergy = h->siteEnergy.ptr;
-    t.siteCount = h->siteCount.ptr;
+    t.itemEnergy = h->siteEnergy.ptr;
+    t.itemCount = h->siteCount.ptr;
     size_t shared = (size_t) t.capacity*(2*sizeof(R4)+3*sizeof(long long)+2*sizeof(int));
     if (shared > 160*1024)
         throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_execute: a site has too many neighbours for one workgroup");
     bool timed = h->timer.start(stream);
-    const int workgroups = (subsetSites >= 0 ? subsetSites : h->numSites);
+    const int workgroups = (subsetItems >= 0 ? subsetItems : (items ? h->numItems : h->numSites));
     if (workgroups > 0)
         hipLaunchKernelGGL(k_triples<Real>, dim3(workgroups), dim3(64), shared, stream, t);
     IXH_HIP_CHECK(hipGetLastError());
@@ -860,6 +881,29 @@ int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* d, int 
                     }
                 }
                 start[n] = (int) total;
+                // work items: every site's neighbour pairs in chunks
+                std::vector<int> itemSite, itemChunk;
+                h->segItems.assign(num_segments+1, 0);
+                for (int g = 0; g < num_segments; g++) {
+                    for (int i = site_offsets[g]; i < site_offsets[g+1]; i++) {
+                        const long long cap = site_offsets[g+1]-i-1;
+                        const long long pairs = cap*(cap-1)/2;
+                        const long long chunks = std::max(1LL, (pairs+TRIPLE_CHUNK-1)/TRIPLE_CHUNK);
+                        for (long long c = 0; c < chunks; c++) {
+                            itemSite.push_back(i);
+                            itemChunk.push_back((int) c);
+                        }
+                    }
+                    if (itemSite.size() > (size_t) INT32_MAX)
+                        throw Error(IXH_ERROR_CAPACITY, "ixh_threebody_create_segmented: too many work items");
+                    h->segItems[g+1] = (int) itemSite.size();
+                }
+                h->numItems = (int) itemSite.size();
+                h->itemSite.upload(itemSite.data(), std::max<size_t>(itemSite.size(), 1), nullptr);
+                h->itemChunk.upload(itemChunk.data(), std::max<size_t>(itemChunk.size(), 1), nullptr);
+                h->segItemsDev.upload(h->segItems.data(), h->segItems.size(), nullptr);
+                h->siteEnergy.reserve(std::max(h->numItems, 1));
+                h->siteCount.reserve(std::max(h->numItems, 1));
                 h->segOffsetsDev.upload(h->segOffsets.data(), h->segOffsets.size(), nullptr);
                 h->siteEnd.upload(end.data(), std::max<size_t>(end.size(), 1), nullptr);
                 h->start.upload(start.data(), start.size(), nullptr);
@@ -945,14 +989,14 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         double cutoff2 = h->cutoff*h->cutoff;
         if (h->numSegments > 0) {
             // segmented: static list capacity, no host synchronization
-            int subsetSites = -1;
+            int subsetSites = -1, subsetItemCount = -1;
             if (numSelected >= 0) {
                 size_t needed = numSelected;
                 for (int k = 0; k < numSelected; k++) {
                     const int g = segments[k];
                     if (g < 0 || g >= h->numSegments)
                         throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute_subset: segment index out of range");
-                    needed += h->segOffsets[g+1]-h->segOffsets[g];
+                    needed += h->segOffsets[g+1]-h->segOffsets[g]+h->segItems[g+1]-h->segItems[g];
                 }
                 if (h->subsetCopied == nullptr)
                     IXH_HIP_CHECK(hipEventCreateWithFlags(&h->subsetCopied, hipEventDisableTiming));
@@ -969,16 +1013,25 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                 for (int k = 0; k < numSelected; k++)
                     for (int i = h->segOffsets[segments[k]]; i < h->segOffsets[segments[k]+1]; i++)
                         h->subsetStaging[count++] = i;
-                std::copy(segments, segments+numSelected, h->subsetStaging+count);
+                int itemCount = 0;
+                int* itemList = h->subsetStaging+count;
+                for (int k = 0; k < numSelected; k++)
+                    for (int w = h->segItems[segments[k]]; w < h->segItems[segments[k]+1]; w++)
+                        itemList[itemCount++] = w;
+                std::copy(segments, segments+numSelected, itemList+itemCount);
                 h->subsetSites.reserve(std::max(count, 1));
+                h->subsetItems.reserve(std::max(itemCount, 1));
                 h->subsetSegments.reserve(std::max(numSelected, 1));
                 if (count > 0)
                     IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSites.ptr, h->subsetStaging, sizeof(int)*count, hipMemcpyHostToDevice, stream));
+                if (itemCount > 0)
+                    IXH_HIP_CHECK(hipMemcpyAsync(h->subsetItems.ptr, itemList, sizeof(int)*itemCount, hipMemcpyHostToDevice, stream));
                 if (numSelected > 0)
-                    IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSegments.ptr, h->subsetStaging+count, sizeof(int)*numSelected,
+                    IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSegments.ptr, itemList+itemCount, sizeof(int)*numSelected,
                                                  hipMemcpyHostToDevice, stream));
                 IXH_HIP_CHECK(hipEventRecord(h->subsetCopied, stream));
                 subsetSites = count;
+                subsetItemCount = itemCount;
             }
             const int workgroups = (subsetSites >= 0 ? subsetSites : n);
             if (workgroups > 0)
@@ -989,22 +1042,22 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
             hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
             IXH_HIP_CHECK(hipGetLastError());
             if (h->precision == IXH_DOUBLE)
-                launch_triples<double>(h, b, box, cutoff2, h->maxSegment, stream, subsetSites);
+                launch_triples<double>(h, b, box, cutoff2, h->maxSegment, stream, subsetItemCount);
             else
-                launch_triples<float>(h, b, box, cutoff2, h->maxSegment, stream, subsetSites);
+                launch_triples<float>(h, b, box, cutoff2, h->maxSegment, stream, subsetItemCount);
             if (flags & IXH_INCLUDE_FORCES) {
                 hipLaunchKernelGGL(k_apply_site_forces, dim3(grid_of(n, 128)), dim3(128), 0, stream, h->siteForce.ptr, n,
                                    h->offsetsDev.ptr, h->particlesDev.ptr, b->force, b->force_stride);
                 IXH_HIP_CHECK(hipGetLastError());
             }
             if (subsetSites < 0) {
-                hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
-                                   h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
+                hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, h->numItems,
+                                   b->energy, h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
                 IXH_HIP_CHECK(hipGetLastError());
             }
             const int reduced = (subsetSites >= 0 ? numSelected : h->numSegments);
             if (segmentEnergies != nullptr && reduced > 0) {
-                hipLaunchKernelGGL(k_reduce_site_segments, dim3(reduced), dim3(256), 0, stream, h->siteEnergy.ptr, h->segOffsetsDev.ptr,
+                hipLaunchKernelGGL(k_reduce_site_segments, dim3(reduced), dim3(256), 0, stream, h->siteEnergy.ptr, h->segItemsDev.ptr,
                                    subsetSites >= 0 ? (const int*) h->subsetSegments.ptr : (const int*) nullptr, segmentEnergies);
                 IXH_HIP_CHECK(hipGetLastError());
             }

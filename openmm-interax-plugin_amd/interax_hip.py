@@ -28,6 +28,7 @@ IXH_OK = 0
 IXH_SINGLE, IXH_MIXED, IXH_DOUBLE = 0, 1, 2
 IXH_TERM_DISPERSION, IXH_TERM_EXCHANGE, IXH_TERM_SHORT_RANGE = 1, 2, 4
 IXH_INCLUDE_FORCES, IXH_INCLUDE_ENERGY = 1, 2
+IXH_DEFER_LIST_CHECK = 4
 PRECISIONS = {"single": IXH_SINGLE, "mixed": IXH_MIXED, "double": IXH_DOUBLE}
 
 
@@ -126,6 +127,7 @@ SIGNATURES = {
     "ixh_pair_execute_subset": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int, ctypes.c_void_p, _ip,
                                 ctypes.c_int],
     "ixh_threebody_execute_subset": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_void_p, _ip, ctypes.c_int],
+    "ixh_pair_list_was_valid": [ctypes.c_void_p, _ip],
     "ixh_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)],
     "ixh_server_add_cluster": [ctypes.c_void_p, ctypes.POINTER(PairDesc), ctypes.POINTER(ThreeBodyDesc), _ip],
     "ixh_server_add_periodic": [ctypes.c_void_p, ctypes.POINTER(PairDesc), _dp, _ip],
