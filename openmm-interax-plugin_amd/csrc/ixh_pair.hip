@@ -599,11 +599,11 @@ __device__ __forceinline__ PairTables load_tables(const TileArgs<Real>& a, float
 template <typename Real, bool TABLES>
 struct SideLds {
     typedef typename Vec4T<Real>::type R4;
-    R4 pos[BLOCK];
-    R4 A[BLOCK];
+    R4 pos[BLOCK];                  // with type tables the class indices are in the bits of pos.w
+    Real wd[TABLES ? BLOCK : 1];    // with type tables: the dispersion width
+    R4 A[TABLES ? 1 : BLOCK];
     R4 B[TABLES ? 1 : BLOCK];
     R4 C[TABLES ? 1 : BLOCK];
-    int cls[TABLES ? BLOCK : 1];
     int flags[BLOCK];
 };
 
@@ -616,12 +616,15 @@ struct ParticleType<float, true> {
     typedef TableParticle type;
 };
 
+// Stage the parameters of a slot whose position is already in s.pos[lane].
 template <typename Real, bool TABLES>
 __device__ __forceinline__ void stage_particle(SideLds<Real, TABLES>& s, int lane, const TileArgs<Real>& a, int slot) {
-    s.A[lane] = a.pA[slot];
-    if constexpr (TABLES)
-        s.cls[lane] = a.cls[slot];
+    if constexpr (TABLES) {
+        s.wd[lane] = a.pA[slot].x;
+        s.pos[lane].w = __int_as_float(a.cls[slot]);
+    }
     else {
+        s.A[lane] = a.pA[slot];
         s.B[lane] = a.pB[slot];
         s.C[lane] = a.pC[slot];
     }
@@ -629,12 +632,8 @@ __device__ __forceinline__ void stage_particle(SideLds<Real, TABLES>& s, int lan
 }
 
 __device__ __forceinline__ void fetch_particle(const SideLds<float, true>& s, int l, TableParticle& P) {
-    const float4 A = s.A[l];
-    P.wd = A.x;
-    P.amp = A.y;
-    P.kh = A.z;
-    P.lh = A.w;
-    P.cls = s.cls[l];
+    P.wd = s.wd[l];
+    P.cls = __float_as_int(s.pos[l].w);
 }
 template <typename Real>
 __device__ __forceinline__ void fetch_particle(const SideLds<Real, false>& s, int l, PairParticle<Real>& P) {
@@ -698,11 +697,7 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const Pai
 
 // The parameters of one slot, from global memory.
 __device__ __forceinline__ void load_particle(const TileArgs<float>& a, int slot, TableParticle& P) {
-    const float4 A = a.pA[slot];
-    P.wd = A.x;
-    P.amp = A.y;
-    P.kh = A.z;
-    P.lh = A.w;
+    P.wd = a.pA[slot].x;
     P.cls = a.cls[slot];
 }
 template <typename Real>
@@ -865,7 +860,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
 // then evaluated with all lanes busy, whatever the spread of neighbour counts between the particles of the tile.
 // The forces on i and j are added in LDS as 64-bit fixed point (the j side gets the exact negation of the i side's
 // value) and flushed to the force buffer per tile (j) and per block (i).
-constexpr int QUEUE = 512;   // pending pairs never exceed 63 + 256
+constexpr int QUEUE = 128;   // pending pairs never exceed 63 + 64
 
 template <typename Real, bool TABLES>
 struct QueueLds {
@@ -906,7 +901,7 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
     atomicAdd(&q.fJ[2][jl], (unsigned long long) -fz);
 }
 
-// One lane's partner particle of a tile, loaded from global memory one tile ahead of its use.
+// One lane's partner particle of a tile, loaded from global memory.
 template <typename Real, bool TABLES>
 struct PartnerFetch {
     typedef typename Vec4T<Real>::type R4;
@@ -958,19 +953,23 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
         pj.y += Real(sy);
         pj.z += Real(sz);
     }
-    s.pos[lane] = pj;
-    s.A[lane] = f.A;
-    if constexpr (TABLES)
-        s.cls[lane] = f.cls;
+    if constexpr (TABLES) {
+        pj.w = __int_as_float(f.cls);
+        s.wd[lane] = f.A.x;
+    }
     else {
+        s.A[lane] = f.A;
         s.B[lane] = f.B;
         s.C[lane] = f.C;
     }
+    s.pos[lane] = pj;
     s.flags[lane] = f.flags;
 }
 
+// With type tables the kernel is held to 5 waves per SIMD (at most 102 VGPRs; a few values spill): the pair batches
+// are latency-bound, and the extra wave hides more of it than the spills cost.
 template <typename Real, int TERMS, bool TABLES>
-__global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
+__global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
     __shared__ QueueLds<Real, TABLES> lds[4];
     extern __shared__ float tableLds[];   // the type tables, sized at launch
@@ -1032,10 +1031,10 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
             next = make_uint2(0xffffffffu, 0xffffffffu);
             if (nextBase+4*lane < cEnd)
                 next = *(const uint2*) (a.lists+nextBase+4*lane);
-#pragma unroll
+#pragma unroll 1
             for (int sub = 0; sub < 4; sub++) {
                 const unsigned word = (sub < 2 ? chunk.x : chunk.y);
-                const unsigned e = ((sub & 1) ? (word >> 16) : word) & 0xffffu;
+                const unsigned e = (word >> ((sub & 1)*16)) & 0xffffu;
                 bool in = (e != 0xffffu);
                 if (in) {
                     const int il = (int) (e >> 6), jl = (int) (e & 63);
@@ -1053,12 +1052,12 @@ __global__ void __launch_bounds__(256) k_pair_queue(TileArgs<Real> a) {
                 if (in)
                     q.queue[(head+count+__popcll(ballot & ((1ULL << lane)-1))) & (QUEUE-1)] = (unsigned short) e;
                 count += __popcll(ballot);
-            }
-            while (count >= BLOCK) {
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy, pairs);
-                head = (head+BLOCK) & (QUEUE-1);
-                count -= BLOCK;
+                if (count >= BLOCK) {
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy, pairs);
+                    head = (head+BLOCK) & (QUEUE-1);
+                    count -= BLOCK;
+                }
             }
             chunk = next;
         }
@@ -1466,54 +1465,48 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
 // more classes than the LDS tables hold.  packed receives wd, amp, K/2, L/2 per particle.
 bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls, std::vector<float>& tables) {
     const int n = p->n;
-    std::map<std::tuple<double, double, double, double>, int> dispMap;
-    std::map<double, int> exchMap, srMap;
-    std::vector<std::tuple<double, double, double, double>> dispKeys;
-    std::vector<double> exchWidths, srWidths;
+    typedef std::tuple<double, double, double, double> Key;
+    std::map<Key, int> dispMap, exchMap, srMap;
+    std::vector<Key> dispKeys, exchKeys, srKeys;    // exchange: (width, amplitude); short range: (width, K, L)
     cls.assign(n, 0);
     packed.assign(4*(size_t) n, 0.0f);
-    auto classOf = [](std::map<double, int>& m, std::vector<double>& keys, double w) {
-        auto it = m.find(w);
+    auto classOf = [](std::map<Key, int>& m, std::vector<Key>& keys, const Key& k) {
+        auto it = m.find(k);
         if (it != m.end())
             return it->second;
         int c = (int) keys.size();
-        m.emplace(w, c);
-        keys.push_back(w);
+        m.emplace(k, c);
+        keys.push_back(k);
         return c;
     };
+    const bool shared = p->settings.shared_overlap != 0;
     for (int i = 0; i < n; i++) {
         int d = 0, x = 0, s = 0;
         if (p->terms & IXH_TERM_DISPERSION) {
             const double* q = &p->dispersion[5*(size_t) i];
-            auto key = std::make_tuple(q[0], q[1], q[2], q[3]);
-            auto it = dispMap.find(key);
-            if (it == dispMap.end()) {
-                d = (int) dispKeys.size();
-                dispMap.emplace(key, d);
-                dispKeys.push_back(key);
-            }
-            else
-                d = it->second;
+            d = classOf(dispMap, dispKeys, std::make_tuple(q[0], q[1], q[2], q[3]));
             packed[4*i] = (float) q[4];
         }
-        if (p->terms & IXH_TERM_EXCHANGE) {
-            x = classOf(exchMap, exchWidths, p->exchange[2*i+1]);
-            packed[4*i+1] = (float) p->exchange[2*i];
+        const bool ex = (p->terms & IXH_TERM_EXCHANGE) != 0, sr = (p->terms & IXH_TERM_SHORT_RANGE) != 0;
+        if (shared) {
+            // one class of (width, amplitude, K, L) for both overlap terms
+            x = s = classOf(exchMap, exchKeys, std::make_tuple(p->exchange[2*i+1], p->exchange[2*i], p->shortRange[3*i],
+                                                               p->shortRange[3*i+1]));
         }
-        if (p->terms & IXH_TERM_SHORT_RANGE) {
-            s = classOf(srMap, srWidths, p->shortRange[3*i+2]);
-            packed[4*i+2] = (float) (0.5*p->shortRange[3*i]);
-            packed[4*i+3] = (float) (0.5*p->shortRange[3*i+1]);
+        else {
+            if (ex)
+                x = classOf(exchMap, exchKeys, std::make_tuple(p->exchange[2*i+1], p->exchange[2*i], 0.0, 0.0));
+            if (sr)
+                s = classOf(srMap, srKeys, std::make_tuple(p->shortRange[3*i+2], 0.0, p->shortRange[3*i], p->shortRange[3*i+1]));
         }
-        if (dispKeys.size() > (size_t) MAX_DISPERSION_CLASSES || exchWidths.size() > (size_t) MAX_WIDTH_CLASSES ||
-            srWidths.size() > (size_t) MAX_WIDTH_CLASSES)
+        if (dispKeys.size() > (size_t) MAX_DISPERSION_CLASSES || exchKeys.size() > (size_t) MAX_WIDTH_CLASSES ||
+            srKeys.size() > (size_t) MAX_WIDTH_CLASSES)
             return false;
         cls[i] = d | (x << 8) | (s << 16);
     }
-    const bool shared = p->settings.shared_overlap != 0;
     p->numDisp = std::max<int>(1, (int) dispKeys.size());
-    p->numExch = std::max<int>(1, (int) exchWidths.size());
-    p->numSr = (shared ? p->numExch : std::max<int>(1, (int) srWidths.size()));
+    p->numExch = std::max<int>(1, (int) exchKeys.size());
+    p->numSr = (shared ? p->numExch : std::max<int>(1, (int) srKeys.size()));
     p->exchOffset = 4*p->numDisp*p->numDisp;
     int exchFloats = OVERLAP_PAIR_FLOATS*p->numExch*p->numExch;
     p->srOffset = (shared ? p->exchOffset : p->exchOffset+exchFloats);
@@ -1532,19 +1525,30 @@ bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls
             row[2] = (float) (pre*(2*(m1i*m3j+m3i*m1j)+4.2*m2i*m2j));
             row[3] = (ai*aj != 0 ? 1.0f : 0.0f);
         }
-    for (int ci = 0; ci < (int) exchWidths.size(); ci++)
-        for (int cj = 0; cj < (int) exchWidths.size(); cj++) {
-            OverlapRow w[2];
-            overlap_rows(exchWidths[ci], exchWidths[cj], p->settings.exchange_scheme, w);
-            std::memcpy(&tables[p->exchOffset+OVERLAP_PAIR_FLOATS*(ci*p->numExch+cj)], w, sizeof(w));
-        }
-    if (!shared)
-        for (int ci = 0; ci < (int) srWidths.size(); ci++)
-            for (int cj = 0; cj < (int) srWidths.size(); cj++) {
+    // overlap rows of a class pair, with the amplitude product (exchange) and K, L averages (short range)
+    auto fill = [&](const std::vector<Key>& keys, int offset, int count, const OverlapScheme& scheme) {
+        for (int ci = 0; ci < (int) keys.size(); ci++)
+            for (int cj = 0; cj < (int) keys.size(); cj++) {
                 OverlapRow w[2];
-                overlap_rows(srWidths[ci], srWidths[cj], p->settings.short_range_scheme, w);
-                std::memcpy(&tables[p->srOffset+OVERLAP_PAIR_FLOATS*(ci*p->numSr+cj)], w, sizeof(w));
+                overlap_rows(std::get<0>(keys[ci]), std::get<0>(keys[cj]), scheme, w);
+                for (OverlapRow& row : w) {
+                    row.amplitude = (float) (std::get<1>(keys[ci])*std::get<1>(keys[cj]));
+                    row.kk = (float) (0.5*(std::get<2>(keys[ci])+std::get<2>(keys[cj])));
+                    row.ll = (float) (0.5*(std::get<3>(keys[ci])+std::get<3>(keys[cj])));
+                }
+                std::memcpy(&tables[offset+OVERLAP_PAIR_FLOATS*(ci*count+cj)], w, sizeof(w));
             }
+    };
+    if (p->terms & (IXH_TERM_EXCHANGE | IXH_TERM_SHORT_RANGE)) {
+        if (shared)
+            fill(exchKeys, p->exchOffset, p->numExch, p->settings.exchange_scheme);
+        else {
+            if (p->terms & IXH_TERM_EXCHANGE)
+                fill(exchKeys, p->exchOffset, p->numExch, p->settings.exchange_scheme);
+            if (p->terms & IXH_TERM_SHORT_RANGE)
+                fill(srKeys, p->srOffset, p->numSr, p->settings.short_range_scheme);
+        }
+    }
     return true;
 }
 

@@ -310,7 +310,8 @@ __device__ __forceinline__ void pair_energy(Real r, Real invR, const PairParticl
 constexpr int MAX_DISPERSION_CLASSES = 16;
 constexpr int MAX_WIDTH_CLASSES = 8;
 
-// One representation of S(r; a, b) for an ordered pair of Slater widths, in a form shared by both branches:
+// One representation of S(r; a, b) for an ordered pair of classes (Slater width, amplitude or K and L), in a form
+// shared by both branches:
 //   S(r) = e1(r) (m1/r + sum_{k<=8} p_k r^k) + e2(r) (q1/r + q0),   e_i(r) = exp(-r/w_i).
 // The Taylor branch about l (InteraxPairTerms.h:189-208) is e1 = exp(-r/l), p_k = c_k(t)/(pi l^3 l^k) and no e2 part;
 // the closed form (:176-187) is S = [4a^2b^2/D^3 (e_a - e_b) + r/D^2 (a e_a + b e_b)]/(8 pi r), i.e.
@@ -325,15 +326,18 @@ struct alignas(16) OverlapRow {
     float q1, q0;
     float norm;              // 8 pi (a+b)^3, the short-range normalisation
     float gate;              // 0 if a or b is zero: the pair has no such term
+    float amplitude;         // exchange: A_i A_j
+    float kk, ll;            // short range: (K_i+K_j)/2, (L_i+L_j)/2
+    float pad;
 };
-static_assert(sizeof(OverlapRow) == 20*sizeof(float), "OverlapRow layout");
-// Per ordered width pair: the Taylor row, then the closed-form row (both identical when one branch is never taken).
-constexpr int OVERLAP_PAIR_FLOATS = 40;
+static_assert(sizeof(OverlapRow) == 24*sizeof(float), "OverlapRow layout");
+// Per ordered class pair: the Taylor row, then the closed-form row (both identical when one branch is never taken).
+constexpr int OVERLAP_PAIR_FLOATS = 48;
 
-// Per-particle values the tables do not cover (FP32): dispersion width, exchange amplitude, K/2 and L/2.
+// Per-particle values the tables do not cover (FP32): the dispersion width (per atom) and the class indices.
 struct TableParticle {
-    float wd, amp, kh, lh;
-    int cls;                 // dispersion class | exchange width class << 8 | short-range width class << 16
+    float wd;
+    int cls;                 // dispersion class | exchange class << 8 | short-range class << 16
 };
 
 // Pointers into LDS and the sizes of the tables of a launch.
@@ -397,7 +401,7 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
         const OverlapRow* rows = tb.exch + 2*(iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi);
         if (rows[0].gate != 0.0f) {
             overlap_from_rows(rows, r, invR, sx, dsx);
-            const float amp = pi.amp*pj.amp;
+            const float amp = rows[0].amplitude;
             energy += amp*sx;
             dEdR += amp*dsx;
         }
@@ -415,8 +419,8 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
                 overlap_from_rows(rows, r, invR, ss, dss);
             const float norm = rows[0].norm;
             const float st = norm*ss;
-            const float kk = pi.kh+pj.kh;
-            const float ll = pi.lh+pj.lh;
+            const float kk = rows[0].kk;
+            const float ll = rows[0].ll;
             energy += (kk+ll*st)*st;
             dEdR += (kk+2.0f*ll*st)*norm*dss;
         }
