@@ -1086,19 +1086,29 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
     store_wave_totals(a.waveEnergy, a.waveCount, wave, lane, energy, pairs);
 }
 
-// Sum the per-wave energies and pair counts of a launch in wave order (one workgroup, contiguous chunks per thread
+// Sum the per-wave energies and pair counts of a launch (one workgroup, a fixed assignment of partials to threads
 // and a fixed reduction tree, so the result does not depend on timing) and add the energy to the energy buffer.
 __global__ void __launch_bounds__(1024) k_reduce_waves(const double* waveEnergy, const int* waveCount, int numWaves, double* energy,
                                                        long long* pairCount, int addEnergy) {
     __shared__ double sE[1024];
     __shared__ long long sC[1024];
+    // strided (coalesced) partial sums, four loads in flight per thread
     double e = 0;
     long long c = 0;
-    const int per = (numWaves+1023)/1024;
-    const int begin = threadIdx.x*per, end = min(numWaves, begin+per);
-    for (int k = begin; k < end; k++) {
-        e += waveEnergy[k];
-        c += waveCount[k];
+    for (int k0 = threadIdx.x; k0 < numWaves; k0 += 4*1024) {
+        double v[4] = {0, 0, 0, 0};
+        int n[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k0+u*1024 < numWaves) {
+                v[u] = waveEnergy[k0+u*1024];
+                n[u] = waveCount[k0+u*1024];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            e += v[u];
+            c += n[u];
+        }
     }
     sE[threadIdx.x] = e;
     sC[threadIdx.x] = c;
@@ -2193,16 +2203,15 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
         }
         DeviceGuard guard(p->device);
         hipStream_t stream = (hipStream_t) b->stream;
-        // the handle's buffers are reused by every call: a call on another stream waits for the previous one
-        if (p->done == nullptr)
-            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
-        else if (stream != p->lastStream)
-            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->done, 0));
-        p->lastStream = stream;
         const float4* corr = (const float4*) b->posq_correction;
         bool rebuild = !p->built;
+        // The displacement check reads only this call's positions and the positions of the last build, so it runs
+        // before this call waits for the previous one: its answer is back while the previous tiles still run.
+        const bool deferredBefore = p->deferredCheck;
         p->deferredCheck = false;
         if (!rebuild && p->method != IXH_NO_CUTOFF) {
+            if (deferredBefore && stream != p->lastStream)
+                IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->flagCopied, 0));   // the previous flag has been read back
             hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, stream, p->flagDev.ptr, 0);
             hipLaunchKernelGGL(k_check_displacement, dim3(grid_for(p->n, 256)), dim3(256), 0, stream, b->posq, corr, b->precision, p->n,
                                p->buildPos.ptr, (int) (p->method == IXH_CUTOFF_PERIODIC), make_box(b->box), 0.25*SKIN*SKIN, p->flagDev.ptr);
@@ -2219,6 +2228,12 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
                 rebuild = (*p->hostFlag != 0);
             }
         }
+        // the handle's buffers are reused by every call: a call on another stream waits for the previous one
+        if (p->done == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+        else if (stream != p->lastStream)
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->done, 0));
+        p->lastStream = stream;
         if (rebuild)
             build_list(p, b, stream);
         size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
