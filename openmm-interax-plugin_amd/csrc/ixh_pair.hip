@@ -668,9 +668,18 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const Pai
     Real rEff = r, invREff = invR;
     const bool softcore = a.useSoftcore && alchPair;
     if (softcore) {
-        Real r6 = r2*r2*r2;
-        rEff = pow(r6+a.softcoreTerm, Real(1.0/6.0));
-        invREff = Real(1)/rEff;
+        const Real x = r2*r2*r2+a.softcoreTerm;
+        if constexpr (std::is_same<Real, float>::value) {
+            // x^(1/6): v_log_f32/v_exp_f32 estimate, then one Newton step on y^6 = x (error a few ulp)
+            const float y = fast_exp2(__builtin_amdgcn_logf(x)*(1.0f/6.0f));
+            const float y2 = y*y;
+            rEff = y*(5.0f/6.0f) + x*fast_rcp(y2*y2*y)*(1.0f/6.0f);
+            invREff = fast_rcp(rEff);
+        }
+        else {
+            rEff = pow(x, Real(1.0/6.0));
+            invREff = Real(1)/rEff;
+        }
     }
     Real dEdR;
     const bool iFirst = origI < (flagJ & INDEX_MASK);
@@ -1468,6 +1477,10 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
         rows[k].txLimit = txLimit;
         rows[k].norm = (float) (EIGHT_PI*(a+b)*(a+b)*(a+b));
         rows[k].gate = 1;
+        rows[k].high = 0;
+        for (int j = 4; j < 9; j++)
+            if (rows[k].p[j] != 0.0f)
+                rows[k].high = 1;
     }
 }
 

@@ -328,7 +328,7 @@ struct alignas(16) OverlapRow {
     float gate;              // 0 if a or b is zero: the pair has no such term
     float amplitude;         // exchange: A_i A_j
     float kk, ll;            // short range: (K_i+K_j)/2, (L_i+L_j)/2
-    float pad;
+    float high;              // 1 if any of p[4..8] is non-zero (a Taylor row of unequal widths), else 0
 };
 static_assert(sizeof(OverlapRow) == 24*sizeof(float), "OverlapRow layout");
 // Per ordered class pair: the Taylor row, then the closed-form row (both identical when one branch is never taken).
@@ -355,11 +355,24 @@ __device__ __forceinline__ void overlap_from_rows(const OverlapRow* rows, float 
     const OverlapRow& w = rows[(r*rows[0].kTaylor < rows[0].txLimit) ? 0 : 1];
     const float e1 = fast_exp2(-r*w.k1);
     const float e2 = fast_exp2(-r*w.k2);
-    float P = w.p[8], dP = 0;
+    float P, dP = 0;
+    if (__builtin_amdgcn_ballot_w64(w.high != 0.0f) != 0) {
+        // some lane of the wave holds a degree-8 row
+        P = w.p[8];
 #pragma unroll
-    for (int k = 7; k >= 0; k--) {
-        dP = dP*r + P;
-        P = P*r + w.p[k];
+        for (int k = 7; k >= 0; k--) {
+            dP = dP*r + P;
+            P = P*r + w.p[k];
+        }
+    }
+    else {
+        // every row of the wave is a closed form or an equal-width expansion: p[4..8] are zero
+        P = w.p[3];
+#pragma unroll
+        for (int k = 2; k >= 0; k--) {
+            dP = dP*r + P;
+            P = P*r + w.p[k];
+        }
     }
     const float invR2 = invR*invR;
     const float P1 = P + w.m1*invR;
