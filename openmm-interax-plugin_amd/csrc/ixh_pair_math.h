@@ -383,6 +383,9 @@ __device__ __forceinline__ void overlap_from_rows(const OverlapRow* rows, float 
     dS = e1*(dP1 - w.n1*P1) + e2*(dP2 - w.n2*P2);
 }
 
+// Every table read is unconditional, so the compiler can issue them together: a class pair without a term has an
+// all-zero overlap row (S = dS = 0 exactly, amplitude and norm 0) or a zero dispersion row, whose damping argument
+// is replaced by a large constant so that every intermediate stays finite and the term contributes exactly 0.
 template <int TERMS>
 __device__ __forceinline__ void pair_energy_table(float r, float invR, const TableParticle& pi, const TableParticle& pj, bool iFirst,
                                                   const PairTables& tb, float& energy, float& dEdR) {
@@ -391,20 +394,19 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     if (TERMS & IXH_TERM_DISPERSION) {
         const int di = pi.cls & 0xff, dj = pj.cls & 0xff;
         const float4 row = tb.disp[__umul24(di, tb.numDisp)+dj];
-        if (row.w != 0.0f) {
-            const float dxdr = tb.beta2*fast_rcp(pi.wd+pj.wd);
-            float f[3], df[3];
-            tang_toennies<float>(dxdr*r, f, df);
-            const float invR2 = invR*invR;
-            const float invR6 = invR2*invR2*invR2;
-            const float invR8 = invR6*invR2;
-            const float e6 = row.x*invR6;
-            const float e8 = row.y*invR8;
-            const float e10 = row.z*(invR8*invR2);
-            energy -= f[0]*e6 + f[1]*e8 + f[2]*e10;
-            dEdR -= e6*(df[0]*dxdr - 6.0f*f[0]*invR) + e8*(df[1]*dxdr - 8.0f*f[1]*invR) +
-                    e10*(df[2]*dxdr - 10.0f*f[2]*invR);
-        }
+        const bool on = (row.w != 0.0f);
+        const float dxdr = on ? tb.beta2*fast_rcp(pi.wd+pj.wd) : 0.0f;
+        float f[3], df[3];
+        tang_toennies<float>(on ? dxdr*r : 64.0f, f, df);
+        const float invR2 = invR*invR;
+        const float invR6 = invR2*invR2*invR2;
+        const float invR8 = invR6*invR2;
+        const float e6 = row.x*invR6;
+        const float e8 = row.y*invR8;
+        const float e10 = row.z*(invR8*invR2);
+        energy -= f[0]*e6 + f[1]*e8 + f[2]*e10;
+        dEdR -= e6*(df[0]*dxdr - 6.0f*f[0]*invR) + e8*(df[1]*dxdr - 8.0f*f[1]*invR) +
+                e10*(df[2]*dxdr - 10.0f*f[2]*invR);
     }
     constexpr bool EXCH = (TERMS & IXH_TERM_EXCHANGE) != 0;
     constexpr bool SR = (TERMS & IXH_TERM_SHORT_RANGE) != 0;
@@ -412,31 +414,27 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     if (EXCH) {
         const int xi = (pi.cls >> 8) & 0xff, xj = (pj.cls >> 8) & 0xff;
         const OverlapRow* rows = tb.exch + 2*(iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi);
-        if (rows[0].gate != 0.0f) {
-            overlap_from_rows(rows, r, invR, sx, dsx);
-            const float amp = rows[0].amplitude;
-            energy += amp*sx;
-            dEdR += amp*dsx;
-        }
+        overlap_from_rows(rows, r, invR, sx, dsx);
+        const float amp = rows[0].amplitude;
+        energy += amp*sx;
+        dEdR += amp*dsx;
     }
     if (SR) {
         const int si = (pi.cls >> 16) & 0xff, sj = (pj.cls >> 16) & 0xff;
         const OverlapRow* rows = tb.sr + 2*(iFirst ? __umul24(si, tb.numSr)+sj : __umul24(sj, tb.numSr)+si);
-        if (rows[0].gate != 0.0f) {
-            float ss, dss;
-            if (EXCH && tb.shared) {
-                ss = sx;
-                dss = dsx;
-            }
-            else
-                overlap_from_rows(rows, r, invR, ss, dss);
-            const float norm = rows[0].norm;
-            const float st = norm*ss;
-            const float kk = rows[0].kk;
-            const float ll = rows[0].ll;
-            energy += (kk+ll*st)*st;
-            dEdR += (kk+2.0f*ll*st)*norm*dss;
+        float ss, dss;
+        if (EXCH && tb.shared) {
+            ss = sx;
+            dss = dsx;
         }
+        else
+            overlap_from_rows(rows, r, invR, ss, dss);
+        const float norm = rows[0].norm;
+        const float st = norm*ss;
+        const float kk = rows[0].kk;
+        const float ll = rows[0].ll;
+        energy += (kk+ll*st)*st;
+        dEdR += (kk+2.0f*ll*st)*norm*dss;
     }
 }
 
