@@ -546,6 +546,7 @@ struct TileArgs {
     const float* tables;
     int tableFloats;
     int numDisp, numExch, numSr, exchOffset, srOffset;
+    int highOrder;   // some overlap row of the tables is of degree 8
     float beta2;
     int n;
     // geometry
@@ -591,6 +592,7 @@ __device__ __forceinline__ PairTables load_tables(const TileArgs<Real>& a, float
     tb.numSr = a.numSr;
     tb.beta2 = a.beta2;
     tb.shared = a.settings.shared_overlap;
+    tb.highOrder = a.highOrder;
     return tb;
 }
 
@@ -1323,6 +1325,7 @@ struct ixh_pair {
     // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
     bool useTables = false;
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
+    int highOrder = 0;
     DeviceArray<int> classes, sortedClasses;
     DeviceArray<float> tables;
     DeviceArray<unsigned> keys, keysOut;
@@ -1488,6 +1491,7 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
 // more classes than the LDS tables hold.  packed receives wd, amp, K/2, L/2 per particle.
 bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls, std::vector<float>& tables) {
     const int n = p->n;
+    p->highOrder = 0;
     typedef std::tuple<double, double, double, double> Key;
     std::map<Key, int> dispMap, exchMap, srMap;
     std::vector<Key> dispKeys, exchKeys, srKeys;    // exchange: (width, amplitude); short range: (width, K, L)
@@ -1555,6 +1559,8 @@ bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls
                 OverlapRow w[2];
                 overlap_rows(std::get<0>(keys[ci]), std::get<0>(keys[cj]), scheme, w);
                 for (OverlapRow& row : w) {
+                    if (row.high != 0.0f)
+                        p->highOrder = 1;
                     row.amplitude = (float) (std::get<1>(keys[ci])*std::get<1>(keys[cj]));
                     row.kk = (float) (0.5*(std::get<2>(keys[ci])+std::get<2>(keys[cj])));
                     row.ll = (float) (0.5*(std::get<3>(keys[ci])+std::get<3>(keys[cj])));
@@ -1977,6 +1983,7 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
     a.numSr = p->numSr;
     a.exchOffset = p->exchOffset;
     a.srOffset = p->srOffset;
+    a.highOrder = p->highOrder;
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
     a.tileIndex = nullptr;

@@ -348,15 +348,16 @@ struct PairTables {
     int numDisp, numExch, numSr;
     float beta2;             // 2 * damping scale
     int shared;
+    int highOrder;           // some row has non-zero p[4..8]
 };
 
 // S and dS/dr from the row pair of a width pair.
-__device__ __forceinline__ void overlap_from_rows(const OverlapRow* rows, float r, float invR, float& S, float& dS) {
+__device__ __forceinline__ void overlap_from_rows(const PairTables& tb, const OverlapRow* rows, float r, float invR, float& S, float& dS) {
     const OverlapRow& w = rows[(r*rows[0].kTaylor < rows[0].txLimit) ? 0 : 1];
     const float e1 = fast_exp2(-r*w.k1);
     const float e2 = fast_exp2(-r*w.k2);
     float P, dP = 0;
-    if (__builtin_amdgcn_ballot_w64(w.high != 0.0f) != 0) {
+    if (tb.highOrder && __builtin_amdgcn_ballot_w64(w.high != 0.0f) != 0) {
         // some lane of the wave holds a degree-8 row
         P = w.p[8];
 #pragma unroll
@@ -414,7 +415,7 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     if (EXCH) {
         const int xi = (pi.cls >> 8) & 0xff, xj = (pj.cls >> 8) & 0xff;
         const OverlapRow* rows = tb.exch + 2*(iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi);
-        overlap_from_rows(rows, r, invR, sx, dsx);
+        overlap_from_rows(tb, rows, r, invR, sx, dsx);
         const float amp = rows[0].amplitude;
         energy += amp*sx;
         dEdR += amp*dsx;
@@ -428,7 +429,7 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
             dss = dsx;
         }
         else
-            overlap_from_rows(rows, r, invR, ss, dss);
+            overlap_from_rows(tb, rows, r, invR, ss, dss);
         const float norm = rows[0].norm;
         const float st = norm*ss;
         const float kk = rows[0].kk;
