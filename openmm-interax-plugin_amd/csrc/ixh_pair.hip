@@ -601,13 +601,29 @@ __device__ __forceinline__ PairTables load_tables(const TileArgs<Real>& a, float
 template <typename Real, bool TABLES>
 struct SideLds {
     typedef typename Vec4T<Real>::type R4;
-    R4 pos[BLOCK];                  // with type tables the class indices are in the bits of pos.w
+    // Positions in separate x, y, z (and w) arrays: lanes reading 64 different particles of a side then touch 64
+    // consecutive 4-byte words (no bank conflicts), where 16-byte records would put four particles in each bank.
+    Real px[BLOCK], py[BLOCK], pz[BLOCK];
+    float pw[BLOCK];                // with type tables the class indices, as bits
     Real wd[TABLES ? BLOCK : 1];    // with type tables: the dispersion width
     R4 A[TABLES ? 1 : BLOCK];
     R4 B[TABLES ? 1 : BLOCK];
     R4 C[TABLES ? 1 : BLOCK];
     int flags[BLOCK];
 };
+
+template <typename Real, bool TABLES>
+__device__ __forceinline__ void store_pos(SideLds<Real, TABLES>& s, int l, const typename Vec4T<Real>::type& v) {
+    s.px[l] = v.x;
+    s.py[l] = v.y;
+    s.pz[l] = v.z;
+    s.pw[l] = (float) v.w;
+}
+template <typename Real, bool TABLES>
+__device__ __forceinline__ typename Vec4T<Real>::type load_pos(const SideLds<Real, TABLES>& s, int l) {
+    typename Vec4T<Real>::type v = {s.px[l], s.py[l], s.pz[l], Real(0)};
+    return v;
+}
 
 template <typename Real, bool TABLES>
 struct ParticleType {
@@ -618,12 +634,12 @@ struct ParticleType<float, true> {
     typedef TableParticle type;
 };
 
-// Stage the parameters of a slot whose position is already in s.pos[lane].
+// Stage the parameters of a slot whose position is already in LDS.
 template <typename Real, bool TABLES>
 __device__ __forceinline__ void stage_particle(SideLds<Real, TABLES>& s, int lane, const TileArgs<Real>& a, int slot) {
     if constexpr (TABLES) {
         s.wd[lane] = a.pA[slot].x;
-        s.pos[lane].w = __int_as_float(a.cls[slot]);
+        s.pw[lane] = __int_as_float(a.cls[slot]);
     }
     else {
         s.A[lane] = a.pA[slot];
@@ -635,7 +651,7 @@ __device__ __forceinline__ void stage_particle(SideLds<Real, TABLES>& s, int lan
 
 __device__ __forceinline__ void fetch_particle(const SideLds<float, true>& s, int l, TableParticle& P) {
     P.wd = s.wd[l];
-    P.cls = __float_as_int(s.pos[l].w);
+    P.cls = __float_as_int(s.pw[l]);
 }
 template <typename Real>
 __device__ __forceinline__ void fetch_particle(const SideLds<Real, false>& s, int l, PairParticle<Real>& P) {
@@ -727,7 +743,7 @@ __device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 til
     const int slot = (tile.y < 0 ? I*BLOCK+lane : a.partners[tile.y+lane]);
     if (slot < 0) {
         R4 zero = {0, 0, 0, 0};
-        s.pos[lane] = zero;
+        store_pos(s, lane, zero);
         s.flags[lane] = -1;
         return;
     }
@@ -741,7 +757,7 @@ __device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 til
         pj.y += Real(sy);
         pj.z += Real(sz);
     }
-    s.pos[lane] = pj;
+    store_pos(s, lane, pj);
     stage_particle<Real, TABLES>(s, lane, a, slot);
 }
 
@@ -815,7 +831,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
             if (diag && st == 32 && lane >= 32)
                 interact = false;
             if (interact) {
-                const R4 pj = s.pos[jl];
+                const R4 pj = load_pos(s, jl);
                 const Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
                 const Real r2 = dx*dx+dy*dy+dz*dz;
                 const int flagJ = s.flags[jl];
@@ -871,7 +887,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
 // then evaluated with all lanes busy, whatever the spread of neighbour counts between the particles of the tile.
 // The forces on i and j are added in LDS as 64-bit fixed point (the j side gets the exact negation of the i side's
 // value) and flushed to the force buffer per tile (j) and per block (i).
-constexpr int QUEUE = 128;   // pending pairs never exceed 63 + 64
+constexpr int QUEUE = 512;   // pending pairs never exceed 63 + 256
 
 template <typename Real, bool TABLES>
 struct QueueLds {
@@ -889,7 +905,7 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
         return;
     const unsigned e = q.queue[(head+lane) & (QUEUE-1)];
     const int il = (int) (e >> 6), jl = (int) (e & 63);
-    const R4 pi = q.i.pos[il], pj = q.j.pos[jl];
+    const R4 pi = load_pos(q.i, il), pj = load_pos(q.j, jl);
     Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
     if (a.pairImage)
         min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6], a.boxR[7],
@@ -950,7 +966,7 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
     typedef typename Vec4T<Real>::type R4;
     if (f.flags < 0) {
         R4 zero = {0, 0, 0, 0};
-        s.pos[lane] = zero;
+        store_pos(s, lane, zero);
         s.flags[lane] = -1;
         return;
     }
@@ -973,7 +989,7 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
         s.B[lane] = f.B;
         s.C[lane] = f.C;
     }
-    s.pos[lane] = pj;
+    store_pos(s, lane, pj);
     s.flags[lane] = f.flags;
 }
 
@@ -1023,7 +1039,7 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
             cIx = a.centers[3*I];
             cIy = a.centers[3*I+1];
             cIz = a.centers[3*I+2];
-            q.i.pos[lane] = pi;
+            store_pos(q.i, lane, pi);
             stage_particle<Real, TABLES>(q.i, lane, a, slot);
             q.fI[0][lane] = q.fI[1][lane] = q.fI[2][lane] = 0;
         }
@@ -1031,25 +1047,29 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
         q.fJ[0][lane] = q.fJ[1][lane] = q.fJ[2][lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // The tile's candidate list, 256 entries (4 per lane) per step with the next step's entries in flight: each
-        // entry in range is pushed into the queue, and full batches of 64 queued pairs are evaluated.
+        // entry in range is pushed into the queue, then every full batch of 64 queued pairs is evaluated.  The four
+        // filter sub-steps are unrolled and hold no batch evaluation, so no loop inside a step makes the compiler
+        // wait for the next step's entries before the filter starts.
+        // Every lane loads (from the tile's last four entries, or the start of the array, when past the end) and
+        // keeps a validity flag instead of branching, so that the wait for a step's entries can count the next
+        // step's load as outstanding on every path.  Offsets are multiples of 4 and the array holds at least 4.
         const int cBegin = a.listOffsets[t], cEnd = a.listOffsets[t+1];
+        const int cLast = (cEnd >= 4 ? cEnd-4 : 0);
         int head = 0, count = 0;
-        uint2 chunk = make_uint2(0xffffffffu, 0xffffffffu), next = chunk;
-        if (cBegin+4*lane < cEnd)
-            chunk = *(const uint2*) (a.lists+cBegin+4*lane);
+        bool chunkValid = (cBegin+4*lane < cEnd);
+        uint2 chunk = *(const uint2*) (a.lists+(chunkValid ? cBegin+4*lane : cLast));
         for (int base = cBegin; base < cEnd; base += 4*BLOCK) {
-            const int nextBase = base+4*BLOCK;
-            next = make_uint2(0xffffffffu, 0xffffffffu);
-            if (nextBase+4*lane < cEnd)
-                next = *(const uint2*) (a.lists+nextBase+4*lane);
-#pragma unroll 1
+            const int nextIndex = base+4*BLOCK+4*lane;
+            const bool nextValid = (nextIndex < cEnd);
+            const uint2 next = *(const uint2*) (a.lists+(nextValid ? nextIndex : cLast));
+#pragma unroll
             for (int sub = 0; sub < 4; sub++) {
                 const unsigned word = (sub < 2 ? chunk.x : chunk.y);
                 const unsigned e = (word >> ((sub & 1)*16)) & 0xffffu;
-                bool in = (e != 0xffffu);
+                bool in = chunkValid && (e != 0xffffu);
                 if (in) {
                     const int il = (int) (e >> 6), jl = (int) (e & 63);
-                    const R4 pa = q.i.pos[il], pb = q.j.pos[jl];
+                    const R4 pa = load_pos(q.i, il), pb = load_pos(q.j, jl);
                     Real dx = pb.x-pa.x, dy = pb.y-pa.y, dz = pb.z-pa.z;
                     if (a.pairImage)
                         min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
@@ -1063,14 +1083,15 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
                 if (in)
                     q.queue[(head+count+__popcll(ballot & ((1ULL << lane)-1))) & (QUEUE-1)] = (unsigned short) e;
                 count += __popcll(ballot);
-                if (count >= BLOCK) {
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy, pairs);
-                    head = (head+BLOCK) & (QUEUE-1);
-                    count -= BLOCK;
-                }
+            }
+            while (count >= BLOCK) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy, pairs);
+                head = (head+BLOCK) & (QUEUE-1);
+                count -= BLOCK;
             }
             chunk = next;
+            chunkValid = nextValid;
         }
         if (count > 0) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
