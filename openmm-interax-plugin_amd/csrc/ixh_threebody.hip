@@ -303,10 +303,10 @@ __device__ __forceinline__ void tt3(Real x, Real& f, Real& df) {
         f = Real(1)-expX*(Real(1)+x+x*x*Real(0.5)+term3);
 }
 
-// InteraxThreeBodyForceImpl.h:69-72
+// InteraxThreeBodyForceImpl.h:69-72 (FP32: the quotient through v_rcp_f32, within an ulp or two)
 template <typename Real>
 __device__ __forceinline__ Real c9_of(Real a1, Real w1, Real a2, Real w2, Real a3, Real w3) {
-    return Real(1.5)*a1*a2*a3*w1*w2*w3*(w1+w2+w3)/((w1+w2)*(w2+w3)*(w3+w1));
+    return Real(1.5)*a1*a2*a3*w1*w2*w3*(w1+w2+w3)*fast_rcp((w1+w2)*(w2+w3)*(w3+w1));
 }
 
 // 3. One wavefront per work item: a site A and all of its neighbour pairs, or (segmented handles) a chunk of them, so
@@ -346,16 +346,23 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
         pBegin = (long long) t.itemChunk[item]*t.chunkPairs;
         pEnd = min(pEnd, pBegin+t.chunkPairs);
     }
-    for (long long p0 = pBegin; p0 < pEnd; p0 += 64) {
-        long long p = p0+lane;
-        if (p < pEnd) {
-            // p = b(b-1)/2 + a, 0 <= a < b < m
-            int b = (int) ((1.0+sqrt(1.0+8.0*(double) p))*0.5);
-            while ((long long) b*(b-1)/2 > p)
-                b--;
-            while ((long long) (b+1)*b/2 <= p)
-                b++;
-            int a = (int) (p-(long long) b*(b-1)/2);
+    // pair p = b(b-1)/2 + a, 0 <= a < b < m: decoded once per lane, then advanced by 64 per step
+    int b = 1, a = 0;
+    {
+        const long long p = pBegin+lane;
+        b = (int) ((1.0+sqrt(1.0+8.0*(double) p))*0.5);
+        while ((long long) b*(b-1)/2 > p)
+            b--;
+        while ((long long) (b+1)*b/2 <= p)
+            b++;
+        a = (int) (p-(long long) b*(b-1)/2);
+    }
+    for (long long p0 = pBegin; p0 < pEnd; p0 += 64, a += 64) {
+        while (a >= b) {
+            a -= b;
+            b++;
+        }
+        if (p0+lane < pEnd) {
             bool ok = (sMol[a] != sMol[b]);
             R4 va = sDelta[a], vb = sDelta[b];
             Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
@@ -408,14 +415,16 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
                 Real pp = A[0]*B[0]+A[1]*B[1]+A[2]*B[2];
                 Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
                 Real s = B[0]*c[0]+B[1]*c[1]+B[2]*c[2];
-                Real invP = Real(1)/(u*v*w);
-                Real invP32 = invP*sqrt(invP);
+                // FP32: v_rcp_f32 and v_rsq_f32 (an ulp or two) in place of correctly rounded division and sqrt
+                const Real uvw = u*v*w;
+                Real invP = fast_rcp(uvw);
+                Real invP32 = invP*fast_rsqrt(uvw);
                 Real invP52 = invP32*invP;
                 Real pqs = pp*q*s;
                 Real e = c9*(invP32-3*pqs*invP52);
                 Real radial = c9*(-3*invP32+15*pqs*invP52);
                 Real cross = -3*c9*invP52;
-                Real ru = radial/u, rv = radial/v, rw = radial/w;
+                Real ru = radial*fast_rcp(u), rv = radial*fast_rcp(v), rw = radial*fast_rcp(w);
                 Real dA[3], dB[3], dC[3];
                 for (int k = 0; k < 3; k++) {
                     dA[k] = A[k]*ru + (B[k]*(q*s) + c[k]*(pp*s))*cross;
@@ -566,22 +575,33 @@ __global__ void k_apply_site_forces(const long long* siteForce, int numSites, co
     }
 }
 
-__global__ void k_reduce_sites(const double* siteEnergy, const int* siteCount, int numSites, double* energy, long long* tripleCount,
-                               int addEnergy) {
-    __shared__ double sE[256];
-    __shared__ long long sC[256];
-    int per = (numSites+255)/256;
-    int begin = threadIdx.x*per, end = min(numSites, begin+per);
+// The total energy and triple count of a launch: one workgroup of 1024 threads, strided (coalesced) partial sums
+// with four loads in flight per thread, then a fixed tree, so the result does not depend on timing.
+__global__ void __launch_bounds__(1024) k_reduce_sites(const double* siteEnergy, const int* siteCount, int numSites, double* energy,
+                                                       long long* tripleCount, int addEnergy) {
+    __shared__ double sE[1024];
+    __shared__ long long sC[1024];
     double e = 0;
     long long c = 0;
-    for (int s = begin; s < end; s++) {
-        e += siteEnergy[s];
-        c += siteCount[s];
+    for (int k0 = threadIdx.x; k0 < numSites; k0 += 4*1024) {
+        double v[4] = {0, 0, 0, 0};
+        int n[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k0+u*1024 < numSites) {
+                v[u] = siteEnergy[k0+u*1024];
+                n[u] = siteCount[k0+u*1024];
+            }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            e += v[u];
+            c += n[u];
+        }
     }
     sE[threadIdx.x] = e;
     sC[threadIdx.x] = c;
     __syncthreads();
-    for (int stride = 128; stride > 0; stride >>= 1) {
+    for (int stride = 512; stride > 0; stride >>= 1) {
         if (threadIdx.x < stride) {
             sE[threadIdx.x] += sE[threadIdx.x+stride];
             sC[threadIdx.x] += sC[threadIdx.x+stride];
@@ -1051,7 +1071,7 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                 IXH_HIP_CHECK(hipGetLastError());
             }
             if (subsetSites < 0) {
-                hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, h->numItems,
+                hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(1024), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, h->numItems,
                                    b->energy, h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
                 IXH_HIP_CHECK(hipGetLastError());
             }
@@ -1119,7 +1139,7 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                                h->particlesDev.ptr, b->force, b->force_stride);
             IXH_HIP_CHECK(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(256), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
+        hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(1024), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
                            h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
         IXH_HIP_CHECK(hipGetLastError());
         IXH_HIP_CHECK(hipEventRecord(h->done, stream));
