@@ -202,6 +202,12 @@ __device__ __forceinline__ double wave_sum(double v) {
         v += __shfl_xor(v, offset, 64);
     return v;
 }
+// The same butterfly in FP32 (every lane gets the same, order-fixed sum).
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int offset = 32; offset > 0; offset >>= 1)
+        v += __shfl_xor(v, offset, 64);
+    return v;
+}
 
 }  // namespace ixh
 

@@ -24,8 +24,9 @@ using namespace ixh;
 
 namespace {
 
-// Neighbour pairs per work item of the triple kernel of segmented handles (64 steps of a wavefront).
-constexpr int TRIPLE_CHUNK = 4096;
+// Neighbour pairs per work item of the triple kernel of segmented handles (64 steps of 64 pairs for each of its 4
+// wavefronts).
+constexpr int TRIPLE_CHUNK = 16384;
 
 struct BoxT {
     double ax, bx, by, cx, cy, cz;
@@ -309,10 +310,12 @@ __device__ __forceinline__ Real c9_of(Real a1, Real w1, Real a2, Real w2, Real a
     return Real(1.5)*a1*a2*a3*w1*w2*w3*(w1+w2+w3)*fast_rcp((w1+w2)*(w2+w3)*(w3+w1));
 }
 
-// 3. One wavefront per work item: a site A and all of its neighbour pairs, or (segmented handles) a chunk of them, so
-// that the first sites of a large cluster, with O(n^2) pairs each, are shared out over several workgroups.
-template <typename Real>
-__global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
+// 3. One workgroup per work item: a site A and all of its neighbour pairs, or (segmented handles) a chunk of them, so
+// that the first sites of a large cluster, with O(n^2) pairs each, are shared out over several workgroups.  WAVES
+// wavefronts share the neighbours staged in LDS and take every WAVES-th step of 64 pairs: segmented handles, whose
+// large clusters need a lot of LDS per item, use 4 and so keep 4 times as many wavefronts per CU in flight.
+template <typename Real, int WAVES>
+__global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     typedef typename Vec4Of<Real>::type R4;
     extern __shared__ double sharedStorage[];
     R4* sDelta = (R4*) sharedStorage;                               // AB vectors of the neighbours
@@ -322,9 +325,10 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     int* sMol = sIndex+t.capacity;
     const int item = (t.itemIndex != nullptr ? t.itemIndex[blockIdx.x] : (int) blockIdx.x);
     const int i = (t.itemSite != nullptr ? t.itemSite[item] : item);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int begin = t.start[i], m = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
-    for (int a = lane; a < m; a += 64) {
+    for (int a = threadIdx.x; a < m; a += 64*WAVES) {
         int j = t.nbrIndex[begin+a];
         double4 d = t.nbrDelta[begin+a];
         R4 dr = {Real(d.x), Real(d.y), Real(d.z), Real(0)};
@@ -346,10 +350,10 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
         pBegin = (long long) t.itemChunk[item]*t.chunkPairs;
         pEnd = min(pEnd, pBegin+t.chunkPairs);
     }
-    // pair p = b(b-1)/2 + a, 0 <= a < b < m: decoded once per lane, then advanced by 64 per step
+    // pair p = b(b-1)/2 + a, 0 <= a < b < m: decoded once per lane, then advanced by 64 WAVES per step
     int b = 1, a = 0;
     {
-        const long long p = pBegin+lane;
+        const long long p = pBegin+64*wave+lane;
         b = (int) ((1.0+sqrt(1.0+8.0*(double) p))*0.5);
         while ((long long) b*(b-1)/2 > p)
             b--;
@@ -357,12 +361,18 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
             b++;
         a = (int) (p-(long long) b*(b-1)/2);
     }
-    for (long long p0 = pBegin; p0 < pEnd; p0 += 64, a += 64) {
+    for (long long p0 = pBegin+64*wave; p0 < pEnd; p0 += 64*WAVES, a += 64*WAVES) {
         while (a >= b) {
             a -= b;
             b++;
         }
-        if (p0+lane < pEnd) {
+        const bool active = (p0+lane < pEnd);
+        // In rows of 64 pairs or more every active lane of a step has the same b (lane 0 is always active): the
+        // force on b is then summed over the wavefront and added once, instead of 64 atomics on one LDS address.
+        const int bFirst = __builtin_amdgcn_readfirstlane(b);
+        const bool uniformB = __all(!active || b == bFirst);
+        Real fB[3] = {0, 0, 0};
+        if (active) {
             bool ok = (sMol[a] != sMol[b]);
             R4 va = sDelta[a], vb = sDelta[b];
             Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
@@ -471,14 +481,24 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
                 for (int k = 0; k < 3; k++) {
                     fA[k] += dA[k]+dB[k];
                     atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(dC[k]-dA[k]));
-                    atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(-(dB[k]+dC[k])));
+                    if (uniformB)
+                        fB[k] = -(dB[k]+dC[k]);
+                    else
+                        atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(-(dB[k]+dC[k])));
                 }
+            }
+        }
+        if (uniformB) {
+            for (int k = 0; k < 3; k++) {
+                const Real sum = wave_sum(fB[k]);
+                if (lane == 0)
+                    atomicAdd(&sForce[3*bFirst+k], (unsigned long long) to_fixed(sum));
             }
         }
     }
     __syncthreads();
-    // flush the neighbour forces and the force on A
-    for (int a = lane; a < m; a += 64) {
+    // flush the neighbour forces and the force on A (integer sums: the order of the atomics does not matter)
+    for (int a = threadIdx.x; a < m; a += 64*WAVES) {
         int j = sIndex[a];
         for (int k = 0; k < 3; k++)
             add_fixed(&t.siteForce[(size_t) k*t.numSites+j], (long long) sForce[3*a+k]);
@@ -490,11 +510,25 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
         if (lane == 0)
             add_fixed(&t.siteForce[(size_t) k*t.numSites+i], v);
     }
+    // energy and count: per wavefront, then summed in wavefront order
+    __shared__ double sWaveEnergy[WAVES];
+    __shared__ int sWaveCount[WAVES];
     double e = wave_sum(energy);
     int c = (int) wave_sum((double) count);
     if (lane == 0) {
-        t.itemEnergy[item] = e;
-        t.itemCount[item] = c;
+        sWaveEnergy[wave] = e;
+        sWaveCount[wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = 0;
+        int triples = 0;
+        for (int k = 0; k < WAVES; k++) {
+            total += sWaveEnergy[k];
+            triples += sWaveCount[k];
+        }
+        t.itemEnergy[item] = total;
+        t.itemCount[item] = triples;
     }
 }
 
@@ -823,8 +857,12 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
         throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_execute: a site has too many neighbours for one workgroup");
     bool timed = h->timer.start(stream);
     const int workgroups = (subsetItems >= 0 ? subsetItems : (items ? h->numItems : h->numSites));
-    if (workgroups > 0)
-        hipLaunchKernelGGL(k_triples<Real>, dim3(workgroups), dim3(64), shared, stream, t);
+    if (workgroups > 0) {
+        if (items)
+            hipLaunchKernelGGL((k_triples<Real, 4>), dim3(workgroups), dim3(256), shared, stream, t);
+        else
+            hipLaunchKernelGGL((k_triples<Real, 1>), dim3(workgroups), dim3(64), shared, stream, t);
+    }
     IXH_HIP_CHECK(hipGetLastError());
     if (timed)
         h->timer.stop(stream);
