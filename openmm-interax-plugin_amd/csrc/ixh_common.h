@@ -202,10 +202,16 @@ __device__ __forceinline__ double wave_sum(double v) {
         v += __shfl_xor(v, offset, 64);
     return v;
 }
-// The same butterfly in FP32 (every lane gets the same, order-fixed sum).
+// FP32 sum over the 64 lanes, every lane receiving the same order-fixed total (all lanes active): DPP within each
+// row of 16 lanes (quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror; each step adds two partial sums
+// that are equal up to the order of one commutative addition), then the four row sums through two lane exchanges.
 __device__ __forceinline__ float wave_sum(float v) {
-    for (int offset = 32; offset > 0; offset >>= 1)
-        v += __shfl_xor(v, offset, 64);
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x141, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x140, 0xF, 0xF, false));
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     return v;
 }
 
