@@ -304,12 +304,6 @@ __device__ __forceinline__ void tt3(Real x, Real& f, Real& df) {
         f = Real(1)-expX*(Real(1)+x+x*x*Real(0.5)+term3);
 }
 
-// InteraxThreeBodyForceImpl.h:69-72 (FP32: the quotient through v_rcp_f32, within an ulp or two)
-template <typename Real>
-__device__ __forceinline__ Real c9_of(Real a1, Real w1, Real a2, Real w2, Real a3, Real w3) {
-    return Real(1.5)*a1*a2*a3*w1*w2*w3*(w1+w2+w3)*fast_rcp((w1+w2)*(w2+w3)*(w3+w1));
-}
-
 // 3. One workgroup per work item: a site A and all of its neighbour pairs, or (segmented handles) a chunk of them, so
 // that the first sites of a large cluster, with O(n^2) pairs each, are shared out over several workgroups.  WAVES
 // wavefronts share the neighbours staged in LDS and take every WAVES-th step of 64 pairs: segmented handles, whose
@@ -318,30 +312,38 @@ template <typename Real, int WAVES>
 __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     typedef typename Vec4Of<Real>::type R4;
     extern __shared__ double sharedStorage[];
-    R4* sDelta = (R4*) sharedStorage;                               // AB vectors of the neighbours
-    R4* sParam = sDelta+t.capacity;                                 // alpha, frequency, width, alchemical
+    R4* sDelta = (R4*) sharedStorage;           // AB vector of each neighbour B and its square |AB|^2
+    R4* sParam = sDelta+t.capacity;             // alpha w, w, w_A + w and +-1/|AB|^2 (w: the frequency; negative:
+                                                // B is alchemical)
     unsigned long long* sForce = (unsigned long long*) (sParam+t.capacity);   // 3 per neighbour
     int* sIndex = (int*) (sForce+3*(size_t) t.capacity);
-    int* sMol = sIndex+t.capacity;
+    int* sMol = sIndex+t.capacity;              // molecule
+    Real* sWidth = (Real*) (sMol+t.capacity);   // damping width
     const int item = (t.itemIndex != nullptr ? t.itemIndex[blockIdx.x] : (int) blockIdx.x);
     const int i = (t.itemSite != nullptr ? t.itemSite[item] : item);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int begin = t.start[i], m = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
+    const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
+    const bool alchI = t.siteParams[4*i+3] != 0;
+    // the per-neighbour factors of the triple's C9 and geometry, computed once per work item
     for (int a = threadIdx.x; a < m; a += 64*WAVES) {
         int j = t.nbrIndex[begin+a];
         double4 d = t.nbrDelta[begin+a];
-        R4 dr = {Real(d.x), Real(d.y), Real(d.z), Real(0)};
+        const Real x = Real(d.x), y = Real(d.y), z = Real(d.z);
+        const Real u = x*x+y*y+z*z;
+        R4 dr = {x, y, z, u};
         sDelta[a] = dr;
-        R4 pr = {Real(t.siteParams[4*j]), Real(t.siteParams[4*j+1]), Real(t.siteParams[4*j+2]), Real(t.siteParams[4*j+3])};
+        const Real alpha = Real(t.siteParams[4*j]), w = Real(t.siteParams[4*j+1]);
+        R4 pr = {alpha*w, w, wI+w, (t.siteParams[4*j+3] != 0 ? -fast_rcp(u) : fast_rcp(u))};
         sParam[a] = pr;
         sIndex[a] = j;
         sMol[a] = t.molecule[j];
+        sWidth[a] = Real(t.siteParams[4*j+2]);
         sForce[3*a] = sForce[3*a+1] = sForce[3*a+2] = 0;
     }
     __syncthreads();
-    const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
-    const bool alchI = t.siteParams[4*i+3] != 0;
+    const Real cI = t.scaleFactor*Real(1.5)*aI*wI;
     Real fA[3] = {0, 0, 0};
     double energy = 0;
     int count = 0;
@@ -410,17 +412,17 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
             R4 pa = sParam[a], pb = sParam[b];
             Real c9 = 0;
             if (ok) {
-                // ReferenceInteraxThreeBodyIxn.cpp:116-120
-                c9 = t.scaleFactor*c9_of<Real>(aI, wI, pa.x, pa.y, pb.x, pb.y);
-                if (alchI || pa.w != 0 || pb.w != 0)
+                // ReferenceInteraxThreeBodyIxn.cpp:116-120 and InteraxThreeBodyForceImpl.h:69-72:
+                // C9 = 1.5 a_A a_B a_C w_A w_B w_C (w_A+w_B+w_C) / ((w_A+w_B)(w_B+w_C)(w_C+w_A))
+                c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.z*(pa.y+pb.y)*pb.z);
+                if (alchI || pa.w < 0 || pb.w < 0)
                     c9 *= t.lambda;
                 ok = (c9 != 0);
             }
             if (ok) {
                 // InteraxThreeBodyTerms.h:97-146 with a = AB, b = AC, c = BC
                 const Real A[3] = {va.x, va.y, va.z}, B[3] = {vb.x, vb.y, vb.z};
-                Real u = A[0]*A[0]+A[1]*A[1]+A[2]*A[2];
-                Real v = B[0]*B[0]+B[1]*B[1]+B[2]*B[2];
+                const Real u = va.w, v = vb.w;
                 Real w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
                 Real pp = A[0]*B[0]+A[1]*B[1]+A[2]*B[2];
                 Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
@@ -434,7 +436,7 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                 Real e = c9*(invP32-3*pqs*invP52);
                 Real radial = c9*(-3*invP32+15*pqs*invP52);
                 Real cross = -3*c9*invP52;
-                Real ru = radial*fast_rcp(u), rv = radial*fast_rcp(v), rw = radial*fast_rcp(w);
+                Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*fast_rcp(w);
                 Real dA[3], dB[3], dC[3];
                 for (int k = 0; k < 3; k++) {
                     dA[k] = A[k]*ru + (B[k]*(q*s) + c[k]*(pp*s))*cross;
@@ -443,7 +445,8 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                 }
                 if (t.useDamping || t.invDampingRadius > 0) {
                     Real len[3] = {sqrt(u), sqrt(v), sqrt(w)};
-                    Real rates[3] = {2*t.dampingScale/(sI+pa.z), 2*t.dampingScale/(sI+pb.z), 2*t.dampingScale/(pa.z+pb.z)};
+                    const Real widthA = sWidth[a], widthB = sWidth[b];
+                    Real rates[3] = {2*t.dampingScale/(sI+widthA), 2*t.dampingScale/(sI+widthB), 2*t.dampingScale/(widthA+widthB)};
                     Real f[3], dfdr[3];
                     for (int k = 0; k < 3; k++) {
                         if (t.useDamping) {
@@ -852,7 +855,7 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.siteForce = h->siteForce.ptr;
     t.itemEnergy = h->siteEnergy.ptr;
     t.itemCount = h->siteCount.ptr;
-    size_t shared = (size_t) t.capacity*(2*sizeof(R4)+3*sizeof(long long)+2*sizeof(int));
+    size_t shared = (size_t) t.capacity*(2*sizeof(R4)+3*sizeof(long long)+2*sizeof(int)+sizeof(Real));
     if (shared > 160*1024)
         throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_execute: a site has too many neighbours for one workgroup");
     bool timed = h->timer.start(stream);
