@@ -352,28 +352,42 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
         pBegin = (long long) t.itemChunk[item]*t.chunkPairs;
         pEnd = min(pEnd, pBegin+t.chunkPairs);
     }
-    // pair p = b(b-1)/2 + a, 0 <= a < b < m: decoded once per lane, then advanced by 64 WAVES per step
-    int b = 1, a = 0;
-    {
-        const long long p = pBegin+64*wave+lane;
-        b = (int) ((1.0+sqrt(1.0+8.0*(double) p))*0.5);
-        while ((long long) b*(b-1)/2 > p)
-            b--;
-        while ((long long) (b+1)*b/2 <= p)
-            b++;
-        a = (int) (p-(long long) b*(b-1)/2);
-    }
-    for (long long p0 = pBegin+64*wave; p0 < pEnd; p0 += 64*WAVES, a += 64*WAVES) {
-        while (a >= b) {
-            a -= b;
-            b++;
+    // Pair p of the site's neighbour pairs a < b < m.  The first T = M(M-1)/2 (M = min(m, 64)), which are all the pairs
+    // with b < 64, are taken along the diagonals b - a = 1, 2, ... (diagonal e+1 starts at O(e) = e(2M-1-e)/2), so that
+    // a step's lanes hold nearly distinct a and distinct b (at most two lanes per LDS force address); the rest, rows
+    // b >= 64, in row order p = b(b-1)/2 + a, where a step's lanes share one b or two consecutive ones.
+    const int M = min(m, 64);
+    const long long T = (long long) M*(M-1)/2;
+    for (long long p0 = pBegin+64*wave; p0 < pEnd; p0 += 64*WAVES) {
+        const long long p = p0+lane;
+        int a, b;
+        if (p < T) {
+            const int q = (int) p;
+            const float c = (float) (2*M-1);
+            int e = (int) ((c-__builtin_amdgcn_sqrtf(fmaxf(c*c-8.0f*(float) q, 0.0f)))*0.5f);
+            e = max(0, min(e, M-2));
+            while (e < M-2 && (e+1)*(2*M-2-e)/2 <= q)
+                e++;
+            while (e > 0 && e*(2*M-1-e)/2 > q)
+                e--;
+            a = q-e*(2*M-1-e)/2;
+            b = a+e+1;
         }
-        const bool active = (p0+lane < pEnd);
-        // In rows of 64 pairs or more every active lane of a step has the same b (lane 0 is always active): the
-        // force on b is then summed over the wavefront and added once, instead of 64 atomics on one LDS address.
+        else {
+            b = (int) ((1.0f+__builtin_amdgcn_sqrtf(1.0f+8.0f*(float) p))*0.5f);
+            while ((long long) b*(b-1)/2 > p)
+                b--;
+            while ((long long) (b+1)*b/2 <= p)
+                b++;
+            a = (int) (p-(long long) b*(b-1)/2);
+        }
+        const bool active = (p < pEnd);
+        // Rows of 64 pairs or more: a step's active lanes (lane 0 always is one) share b or b and b + 1.  The force on
+        // each of the two is then summed over the wavefront and added once, instead of up to 64 atomics on one LDS
+        // address; diagonal steps and the step that straddles the two orders add per lane.
         const int bFirst = __builtin_amdgcn_readfirstlane(b);
-        const bool uniformB = __all(!active || b == bFirst);
-        Real fB[3] = {0, 0, 0};
+        const bool rows = (p0 >= T) && __all(!active || b == bFirst || b == bFirst+1);
+        Real fB[3] = {0, 0, 0}, fB1[3] = {0, 0, 0};
         if (active) {
             bool ok = (sMol[a] != sMol[b]);
             R4 va = sDelta[a], vb = sDelta[b];
@@ -484,18 +498,28 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                 for (int k = 0; k < 3; k++) {
                     fA[k] += dA[k]+dB[k];
                     atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(dC[k]-dA[k]));
-                    if (uniformB)
+                    if (!rows)
+                        atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(-(dB[k]+dC[k])));
+                    else if (b == bFirst)
                         fB[k] = -(dB[k]+dC[k]);
                     else
-                        atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(-(dB[k]+dC[k])));
+                        fB1[k] = -(dB[k]+dC[k]);
                 }
             }
         }
-        if (uniformB) {
+        if (rows) {
+            const bool second = __any(active && b != bFirst);
             for (int k = 0; k < 3; k++) {
                 const Real sum = wave_sum(fB[k]);
                 if (lane == 0)
                     atomicAdd(&sForce[3*bFirst+k], (unsigned long long) to_fixed(sum));
+            }
+            if (second) {
+                for (int k = 0; k < 3; k++) {
+                    const Real sum = wave_sum(fB1[k]);
+                    if (lane == 0)
+                        atomicAdd(&sForce[3*(bFirst+1)+k], (unsigned long long) to_fixed(sum));
+                }
             }
         }
     }
