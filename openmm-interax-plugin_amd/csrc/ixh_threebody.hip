@@ -352,10 +352,12 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
         pBegin = (long long) t.itemChunk[item]*t.chunkPairs;
         pEnd = min(pEnd, pBegin+t.chunkPairs);
     }
-    // Pair p of the site's neighbour pairs a < b < m.  The first T = M(M-1)/2 (M = min(m, 64)), which are all the pairs
-    // with b < 64, are taken along the diagonals b - a = 1, 2, ... (diagonal e+1 starts at O(e) = e(2M-1-e)/2), so that
-    // a step's lanes hold nearly distinct a and distinct b (at most two lanes per LDS force address); the rest, rows
-    // b >= 64, in row order p = b(b-1)/2 + a, where a step's lanes share one b or two consecutive ones.
+    // Pair p of the site's neighbour pairs.  The first T = M(M-1)/2 (M = min(m, 64)), which are all the pairs within
+    // the first 64 neighbours, are taken in rotation order: rotation s holds the pairs (x, (x+s+1) mod M), x < M (the
+    // last, for even M, only x < M/2), so that a step's lanes hold each neighbour about twice (exactly twice, once on
+    // each side, when M = 64) instead of up to 63 lanes on one LDS force address.  The energy and forces of a triple
+    // do not depend on the order of its two neighbours.  The rest, rows b >= 64, in row order p = b(b-1)/2 + a,
+    // where a step's lanes share one b or two consecutive ones.
     const int M = min(m, 64);
     const long long T = (long long) M*(M-1)/2;
     for (long long p0 = pBegin+64*wave; p0 < pEnd; p0 += 64*WAVES) {
@@ -363,15 +365,15 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
         int a, b;
         if (p < T) {
             const int q = (int) p;
-            const float c = (float) (2*M-1);
-            int e = (int) ((c-__builtin_amdgcn_sqrtf(fmaxf(c*c-8.0f*(float) q, 0.0f)))*0.5f);
-            e = max(0, min(e, M-2));
-            while (e < M-2 && (e+1)*(2*M-2-e)/2 <= q)
-                e++;
-            while (e > 0 && e*(2*M-1-e)/2 > q)
-                e--;
-            a = q-e*(2*M-1-e)/2;
-            b = a+e+1;
+            int r = (int) ((float) q*(1.0f/(float) M));
+            while (r > 0 && r*M > q)
+                r--;
+            while ((r+1)*M <= q)
+                r++;
+            a = q-r*M;
+            b = a+r+1;
+            if (b >= M)
+                b -= M;
         }
         else {
             b = (int) ((1.0f+__builtin_amdgcn_sqrtf(1.0f+8.0f*(float) p))*0.5f);
