@@ -703,6 +703,9 @@ struct ixh_threebody {
     DeviceArray<int> cellValues, sortedSites, cellStart, cellEnd;
     DeviceArray<char> sortTemp;
     DeviceArray<double> bbox;
+    // list totals of the neighbour search read back by the host: the number of entries and the longest list
+    DeviceArray<int> listTotals;
+    int* hostTotals = nullptr;   // pinned, 2 ints
     // independent systems (segments) of a batch: sites [segOffsets[s], segOffsets[s+1]); neighbour lists of static
     // capacity (the later sites of the same segment), so that an evaluation needs no host synchronization
     int numSegments = 0;
@@ -1017,6 +1020,8 @@ int ixh_threebody_destroy(ixh_threebody* h) {
         }
         if (h->subsetStaging != nullptr)
             (void) hipHostFree(h->subsetStaging);
+        if (h->hostTotals != nullptr)
+            (void) hipHostFree(h->hostTotals);
         DeviceGuard guard(h->device);
         (void) hipDeviceSynchronize();
         if (h->done != nullptr)
@@ -1164,10 +1169,20 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         int bits = 1;
         while ((1LL << bits) < numCells)
             bits++;
-        size_t tempBytes = 0;
+        // every buffer of the search is sized before the first launch (a reallocation frees memory in use)
+        h->count.reserve(n+1);
+        h->start.reserve(n+1);
+        h->listTotals.reserve(2);
+        if (h->hostTotals == nullptr)
+            IXH_HIP_CHECK(hipHostMalloc((void**) &h->hostTotals, 2*sizeof(int), hipHostMallocDefault));
+        size_t tempBytes = 0, scanBytes = 0, maxBytes = 0;
         IXH_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
                                                 h->sortedSites.ptr, (size_t) n, 0, bits, stream));
-        h->sortTemp.reserve(tempBytes);
+        IXH_HIP_CHECK(rocprim::exclusive_scan(nullptr, scanBytes, h->count.ptr, h->start.ptr, 0, (size_t) n+1, rocprim::plus<int>(),
+                                              stream));
+        IXH_HIP_CHECK(rocprim::reduce(nullptr, maxBytes, h->count.ptr, h->listTotals.ptr+1, 0, (size_t) n, rocprim::maximum<int>(),
+                                      stream));
+        h->sortTemp.reserve(std::max(tempBytes, std::max(scanBytes, maxBytes)));
         IXH_HIP_CHECK(rocprim::radix_sort_pairs(h->sortTemp.ptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
                                                 h->sortedSites.ptr, (size_t) n, 0, bits, stream));
         hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellStart.ptr, numCells, 0);
@@ -1180,17 +1195,19 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                            exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) nullptr,
                            (int*) nullptr, (double4*) nullptr);
         IXH_HIP_CHECK(hipGetLastError());
-        std::vector<int> counts(n), starts(n+1, 0);
-        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), h->count.ptr, sizeof(int)*(size_t) n, hipMemcpyDeviceToHost, stream));
+        // list offsets by a scan on the device; only the total and the longest list (sizes of the lists and of the
+        // triple kernel's LDS) come back to the host
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, h->count.ptr+n, 1, 0);
+        IXH_HIP_CHECK(rocprim::exclusive_scan(h->sortTemp.ptr, scanBytes, h->count.ptr, h->start.ptr, 0, (size_t) n+1,
+                                              rocprim::plus<int>(), stream));
+        IXH_HIP_CHECK(rocprim::reduce(h->sortTemp.ptr, maxBytes, h->count.ptr, h->listTotals.ptr+1, 0, (size_t) n,
+                                      rocprim::maximum<int>(), stream));
+        IXH_HIP_CHECK(hipMemcpyAsync(h->listTotals.ptr, h->start.ptr+n, sizeof(int), hipMemcpyDeviceToDevice, stream));
+        IXH_HIP_CHECK(hipMemcpyAsync(h->hostTotals, h->listTotals.ptr, 2*sizeof(int), hipMemcpyDeviceToHost, stream));
         IXH_HIP_CHECK(hipStreamSynchronize(stream));
-        int maxCount = 0;
-        for (int i = 0; i < n; i++) {
-            starts[i+1] = starts[i]+counts[i];
-            maxCount = std::max(maxCount, counts[i]);
-        }
-        h->start.upload(starts.data(), starts.size(), stream);
-        h->nbrIndex.reserve(std::max(starts[n], 1));
-        h->nbrDelta.reserve(std::max(starts[n], 1));
+        const int total = h->hostTotals[0], maxCount = h->hostTotals[1];
+        h->nbrIndex.reserve(std::max(total, 1));
+        h->nbrDelta.reserve(std::max(total, 1));
         hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
                            exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) h->start.ptr,
                            h->nbrIndex.ptr, h->nbrDelta.ptr);
