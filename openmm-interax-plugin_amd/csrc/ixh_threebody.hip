@@ -168,10 +168,12 @@ __device__ __forceinline__ void fast_delta(const double* from, const double* to,
 
 // One wave per site i: the sites j > i of another molecule within the cutoff (ReferenceInteraxThreeBodyIxn.cpp:97-103,
 // distance with the reference's rounding).  The candidates of the (up to) 27 neighbouring cells are taken as one
-// list, 64 at a time.  Pass 0 counts, pass 1 writes, in a fixed order (cells, then site index).
+// list, 64 at a time.  The hits are written in a fixed order (cells, then site index) to [i capacity, (i+1) capacity);
+// count[i] receives their number, so that a list longer than the capacity (its excess not written) is detected and
+// the search repeated with more room.
 __global__ void k_neighbors(const double* sites, const int* molecule, int numSites, int periodic, BoxT box, double cutoff2, CellGrid g,
                             int exactImage, const int* cellStart, const int* cellEnd, const int* sortedSites, int* count,
-                            const int* start, int* nbrIndex, double4* nbrDelta) {
+                            int capacity, int* nbrIndex, double4* nbrDelta) {
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const double* si = sites+3*i;
@@ -233,15 +235,21 @@ __global__ void k_neighbors(const double* sites, const int* molecule, int numSit
             hit = ref_norm2(d) < cutoff2;
         }
         const unsigned long long ballot = __ballot(hit);
-        if (start != nullptr && hit) {
-            const int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
-            nbrIndex[pos] = j;
-            nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
+        const int pos = written+__popcll(ballot & ((1ULL << lane)-1));
+        if (hit && pos < capacity) {
+            nbrIndex[(size_t) i*capacity+pos] = j;
+            nbrDelta[(size_t) i*capacity+pos] = make_double4(d[0], d[1], d[2], 0.0);
         }
         written += __popcll(ballot);
     }
-    if (start == nullptr && lane == 0)
+    if (lane == 0)
         count[i] = written;
+}
+
+__global__ void k_strided_starts(int* start, int n, int capacity) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        start[i] = i*capacity;
 }
 
 template <typename Real>
@@ -706,6 +714,8 @@ struct ixh_threebody {
     // list totals of the neighbour search read back by the host: the number of entries and the longest list
     DeviceArray<int> listTotals;
     int* hostTotals = nullptr;   // pinned, 2 ints
+    int listCapacity = 0;        // entries per site of the unsegmented lists
+    int listSites = -1;          // sites the lists were sized for
     // independent systems (segments) of a batch: sites [segOffsets[s], segOffsets[s+1]); neighbour lists of static
     // capacity (the later sites of the same segment), so that an evaluation needs no host synchronization
     int numSegments = 0;
@@ -852,7 +862,7 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.molecule = h->moleculeDev.ptr;
     t.siteParams = h->paramsDev.ptr;
     t.start = h->start.ptr;
-    t.count = (h->numSegments > 0 ? h->count.ptr : nullptr);
+    t.count = h->count.ptr;
     const bool items = (h->numSegments > 0);
     t.itemSite = (items ? h->itemSite.ptr : nullptr);
     t.itemChunk = (items ? h->itemChunk.ptr : nullptr);
@@ -1170,19 +1180,16 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         while ((1LL << bits) < numCells)
             bits++;
         // every buffer of the search is sized before the first launch (a reallocation frees memory in use)
-        h->count.reserve(n+1);
-        h->start.reserve(n+1);
+        h->count.reserve(n);
         h->listTotals.reserve(2);
         if (h->hostTotals == nullptr)
             IXH_HIP_CHECK(hipHostMalloc((void**) &h->hostTotals, 2*sizeof(int), hipHostMallocDefault));
-        size_t tempBytes = 0, scanBytes = 0, maxBytes = 0;
+        size_t tempBytes = 0, maxBytes = 0;
         IXH_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
                                                 h->sortedSites.ptr, (size_t) n, 0, bits, stream));
-        IXH_HIP_CHECK(rocprim::exclusive_scan(nullptr, scanBytes, h->count.ptr, h->start.ptr, 0, (size_t) n+1, rocprim::plus<int>(),
-                                              stream));
-        IXH_HIP_CHECK(rocprim::reduce(nullptr, maxBytes, h->count.ptr, h->listTotals.ptr+1, 0, (size_t) n, rocprim::maximum<int>(),
+        IXH_HIP_CHECK(rocprim::reduce(nullptr, maxBytes, h->count.ptr, h->listTotals.ptr, 0, (size_t) n, rocprim::maximum<int>(),
                                       stream));
-        h->sortTemp.reserve(std::max(tempBytes, std::max(scanBytes, maxBytes)));
+        h->sortTemp.reserve(std::max(tempBytes, maxBytes));
         IXH_HIP_CHECK(rocprim::radix_sort_pairs(h->sortTemp.ptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
                                                 h->sortedSites.ptr, (size_t) n, 0, bits, stream));
         hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellStart.ptr, numCells, 0);
@@ -1191,27 +1198,38 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                            h->cellEnd.ptr);
         IXH_HIP_CHECK(hipGetLastError());
         const int exactImage = (h->periodic && 2*h->cutoff >= std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
-        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
-                           exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) nullptr,
-                           (int*) nullptr, (double4*) nullptr);
-        IXH_HIP_CHECK(hipGetLastError());
-        // list offsets by a scan on the device; only the total and the longest list (sizes of the lists and of the
-        // triple kernel's LDS) come back to the host
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, h->count.ptr+n, 1, 0);
-        IXH_HIP_CHECK(rocprim::exclusive_scan(h->sortTemp.ptr, scanBytes, h->count.ptr, h->start.ptr, 0, (size_t) n+1,
-                                              rocprim::plus<int>(), stream));
-        IXH_HIP_CHECK(rocprim::reduce(h->sortTemp.ptr, maxBytes, h->count.ptr, h->listTotals.ptr+1, 0, (size_t) n,
-                                      rocprim::maximum<int>(), stream));
-        IXH_HIP_CHECK(hipMemcpyAsync(h->listTotals.ptr, h->start.ptr+n, sizeof(int), hipMemcpyDeviceToDevice, stream));
-        IXH_HIP_CHECK(hipMemcpyAsync(h->hostTotals, h->listTotals.ptr, 2*sizeof(int), hipMemcpyDeviceToHost, stream));
-        IXH_HIP_CHECK(hipStreamSynchronize(stream));
-        const int total = h->hostTotals[0], maxCount = h->hostTotals[1];
-        h->nbrIndex.reserve(std::max(total, 1));
-        h->nbrDelta.reserve(std::max(total, 1));
-        hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
-                           exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, (const int*) h->start.ptr,
-                           h->nbrIndex.ptr, h->nbrDelta.ptr);
-        IXH_HIP_CHECK(hipGetLastError());
+        // Lists of a fixed capacity per site, written in one pass; only the longest list comes back to the host (it
+        // sizes the triple kernel's LDS), and a list longer than the capacity makes the search run again with more.
+        int maxCount = 0;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            if (h->listCapacity <= 0 || h->listSites != n) {
+                h->listCapacity = std::max(h->listCapacity, 64);
+                h->listSites = n;
+                h->start.reserve(n);
+                h->nbrIndex.reserve((size_t) n*h->listCapacity);
+                h->nbrDelta.reserve((size_t) n*h->listCapacity);
+                hipLaunchKernelGGL(k_strided_starts, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->start.ptr, n, h->listCapacity);
+                IXH_HIP_CHECK(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
+                               exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, h->listCapacity,
+                               h->nbrIndex.ptr, h->nbrDelta.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+            IXH_HIP_CHECK(rocprim::reduce(h->sortTemp.ptr, maxBytes, h->count.ptr, h->listTotals.ptr, 0, (size_t) n,
+                                          rocprim::maximum<int>(), stream));
+            IXH_HIP_CHECK(hipMemcpyAsync(h->hostTotals, h->listTotals.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+            IXH_HIP_CHECK(hipStreamSynchronize(stream));
+            maxCount = h->hostTotals[0];
+            if (maxCount <= h->listCapacity)
+                break;
+            int capacity = 64;
+            while (capacity < maxCount+maxCount/4)
+                capacity *= 2;
+            h->listCapacity = capacity;
+            h->listSites = -1;   // reallocate and repeat the search
+        }
+        if (maxCount > h->listCapacity)
+            throw Error(IXH_ERROR_HIP, "ixh_threebody_execute: neighbour list capacity not reached after a resize");
         hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
         IXH_HIP_CHECK(hipGetLastError());
         if (h->precision == IXH_DOUBLE)
