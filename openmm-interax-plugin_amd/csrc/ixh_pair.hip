@@ -1020,7 +1020,8 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
     double energy = 0;
     int pairs = 0;
 
-    for (long long t = begin; t < end; t++) {
+    for (long long item = begin; item < end; item++) {
+        const long long t = (a.tileIndex != nullptr ? a.tileIndex[item] : item);
         const int2 tile = a.tiles[t];
         PartnerFetch<Real, TABLES> f;
         fetch_partner<Real, TABLES>(a, a.tileSlots[(size_t) t*BLOCK+lane], f);
@@ -1337,12 +1338,16 @@ struct ixh_pair {
     std::vector<int> segOffsets, segBlocks, segTiles;
     DeviceArray<int> segTilesDev;
     DeviceArray<double> tileEnergy;
-    int gridLimit = 4096;   // largest number of workgroups of a tile launch
+    int gridLimit = 1 << 20;   // largest number of workgroups of a tile launch (default: one tile per wave)
     // subsets of segments: the tile list and segment list of the last call (pinned staging, device copies)
     int* subsetStaging = nullptr;
     size_t subsetCapacity = 0;
     hipEvent_t subsetCopied = nullptr;
     DeviceArray<int> subsetTiles, subsetSegments;
+    // cutoff handles without segments: tiles in decreasing order of candidate pairs, so that the workgroups
+    // dispatched last are the lightest
+    DeviceArray<int> tileOrder;
+    bool ordered = false;
     // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
     bool useTables = false;
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
@@ -1930,6 +1935,14 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
             offsets[t+1] = (int) total;
         }
         p->listOffsets.upload(offsets.data(), offsets.size(), stream);
+        p->ordered = (p->numSegments == 0);
+        if (p->ordered) {
+            std::vector<int> order(T);
+            for (int t = 0; t < T; t++)
+                order[t] = t;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return counts[x] > counts[y]; });
+            p->tileOrder.upload(order.data(), order.size(), stream);
+        }
         p->lists.reserve((size_t) std::max<long long>(total, 4));
         hipLaunchKernelGGL(k_candidate_lists, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->masks.ptr, p->tileCandidates.ptr,
                            (const int*) p->listOffsets.ptr, p->lists.ptr);
@@ -2007,7 +2020,7 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
     a.highOrder = p->highOrder;
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
-    a.tileIndex = nullptr;
+    a.tileIndex = (p->ordered && p->method != IXH_NO_CUTOFF ? (const int*) p->tileOrder.ptr : (const int*) nullptr);
     if (subsetTiles >= 0) {
         a.numTiles = subsetTiles;
         a.tileIndex = p->subsetTiles.ptr;
