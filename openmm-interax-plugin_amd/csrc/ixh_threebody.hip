@@ -727,6 +727,7 @@ struct ixh_threebody {
     int numItems = 0;
     std::vector<int> segItems;
     DeviceArray<int> itemSite, itemChunk, segItemsDev, subsetItems;
+    DeviceArray<int> itemOrder;   // all items, most pairs first (the launch order of a full evaluation)
     // subsets of segments: site list and segment list of the last call
     int* subsetStaging = nullptr;
     size_t subsetCapacity = 0;
@@ -866,7 +867,7 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     const bool items = (h->numSegments > 0);
     t.itemSite = (items ? h->itemSite.ptr : nullptr);
     t.itemChunk = (items ? h->itemChunk.ptr : nullptr);
-    t.itemIndex = (subsetItems >= 0 ? h->subsetItems.ptr : nullptr);
+    t.itemIndex = (subsetItems >= 0 ? h->subsetItems.ptr : (items ? h->itemOrder.ptr : nullptr));
     t.chunkPairs = TRIPLE_CHUNK;
     t.nbrIndex = h->nbrIndex.ptr;
     t.nbrDelta = h->nbrDelta.ptr;
@@ -999,6 +1000,21 @@ int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* d, int 
                     h->segItems[g+1] = (int) itemSite.size();
                 }
                 h->numItems = (int) itemSite.size();
+                // launch order: items with the most pairs first, so that the workgroups dispatched last are the lightest
+                {
+                    std::vector<long long> work(itemSite.size());
+                    for (size_t w = 0; w < itemSite.size(); w++) {
+                        const int i = itemSite[w];
+                        const long long cap = end[i]-i-1;
+                        const long long pairs = cap*(cap-1)/2;
+                        work[w] = std::min<long long>(TRIPLE_CHUNK, std::max(0LL, pairs-(long long) itemChunk[w]*TRIPLE_CHUNK));
+                    }
+                    std::vector<int> order(itemSite.size());
+                    for (size_t w = 0; w < order.size(); w++)
+                        order[w] = (int) w;
+                    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+                    h->itemOrder.upload(order.data(), std::max<size_t>(order.size(), 1), nullptr);
+                }
                 h->itemSite.upload(itemSite.data(), std::max<size_t>(itemSite.size(), 1), nullptr);
                 h->itemChunk.upload(itemChunk.data(), std::max<size_t>(itemChunk.size(), 1), nullptr);
                 h->segItemsDev.upload(h->segItems.data(), h->segItems.size(), nullptr);
