@@ -1344,8 +1344,8 @@ struct ixh_pair {
     size_t subsetCapacity = 0;
     hipEvent_t subsetCopied = nullptr;
     DeviceArray<int> subsetTiles, subsetSegments;
-    // cutoff handles without segments: tiles in decreasing order of candidate pairs, so that the workgroups
-    // dispatched last are the lightest
+    // launch order of the tiles: decreasing candidate pairs (cutoff lists) or particle pairs (all-pairs tiles), so
+    // that the workgroups dispatched last are the lightest
     DeviceArray<int> tileOrder;
     bool ordered = false;
     // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
@@ -1778,6 +1778,21 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         std::iota(partners.begin(), partners.end(), 0);
         p->partners.upload(partners.data(), partners.size(), stream);
         p->pairImage = 0;
+        // launch order: tiles with the most particle pairs first
+        std::vector<int> valid(p->padded/BLOCK, 0);
+        for (int slot = 0; slot < p->padded; slot++)
+            if (perm[slot] >= 0)
+                valid[slot/BLOCK]++;
+        std::vector<long long> work(tiles.size());
+        for (size_t t = 0; t < tiles.size(); t++) {
+            const long long vi = valid[tiles[t].x];
+            work[t] = (tiles[t].y < 0 ? vi*(vi-1)/2 : vi*valid[tiles[t].y/BLOCK]);
+        }
+        std::vector<int> order(tiles.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+        p->tileOrder.upload(order.data(), std::max<size_t>(order.size(), 1), stream);
+        p->ordered = !tiles.empty();
     }
     else {
         KeyArgs k;
@@ -1935,8 +1950,8 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
             offsets[t+1] = (int) total;
         }
         p->listOffsets.upload(offsets.data(), offsets.size(), stream);
-        p->ordered = (p->numSegments == 0);
-        if (p->ordered) {
+        p->ordered = true;
+        {
             std::vector<int> order(T);
             for (int t = 0; t < T; t++)
                 order[t] = t;
@@ -2020,7 +2035,7 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
     a.highOrder = p->highOrder;
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
-    a.tileIndex = (p->ordered && p->method != IXH_NO_CUTOFF ? (const int*) p->tileOrder.ptr : (const int*) nullptr);
+    a.tileIndex = (p->ordered ? (const int*) p->tileOrder.ptr : (const int*) nullptr);
     if (subsetTiles >= 0) {
         a.numTiles = subsetTiles;
         a.tileIndex = p->subsetTiles.ptr;
