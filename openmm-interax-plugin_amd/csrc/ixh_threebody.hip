@@ -452,15 +452,17 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                 Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
                 Real s = B[0]*c[0]+B[1]*c[1]+B[2]*c[2];
                 // FP32: v_rcp_f32 and v_rsq_f32 (an ulp or two) in place of correctly rounded division and sqrt
+                // one reciprocal square root for (uvw)^-1/2, (uvw)^-1, (uvw)^-3/2 and 1/w = uv/(uvw)
                 const Real uvw = u*v*w;
-                Real invP = fast_rcp(uvw);
-                Real invP32 = invP*fast_rsqrt(uvw);
+                const Real rs = fast_rsqrt(uvw);
+                Real invP = rs*rs;
+                Real invP32 = invP*rs;
                 Real invP52 = invP32*invP;
                 Real pqs = pp*q*s;
                 Real e = c9*(invP32-3*pqs*invP52);
                 Real radial = c9*(-3*invP32+15*pqs*invP52);
                 Real cross = -3*c9*invP52;
-                Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*fast_rcp(w);
+                Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*(invP*(u*v));
                 Real dA[3], dB[3], dC[3];
                 for (int k = 0; k < 3; k++) {
                     dA[k] = A[k]*ru + (B[k]*(q*s) + c[k]*(pp*s))*cross;
