@@ -463,12 +463,14 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                 Real radial = c9*(-3*invP32+15*pqs*invP52);
                 Real cross = -3*c9*invP52;
                 Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*(invP*(u*v));
-                Real dA[3], dB[3], dC[3];
-                for (int k = 0; k < 3; k++) {
-                    dA[k] = A[k]*ru + (B[k]*(q*s) + c[k]*(pp*s))*cross;
-                    dB[k] = B[k]*rv + (A[k]*(q*s) + c[k]*(pp*q))*cross;
-                    dC[k] = c[k]*rw + (A[k]*(pp*s) + B[k]*(pp*q))*cross;
-                }
+                // With X = pp s cross, Y = pp q cross, Z = q s cross the gradients with respect to A, B and C = B - A are
+                // dA = A ru + B Z + C X, dB = B rv + A Z + C Y, dC = C rw + A X + B Y; the forces on the neighbours
+                // a (dC - dA) and b (-(dB + dC)) and the sum dA + dB for A are combinations of A, B, C with these
+                // coefficients.
+                const Real X = pp*s*cross, Y = pp*q*cross, Z = q*s*cross;
+                Real aA = X-ru, aB = Y-Z, aC = rw-X;
+                Real bA = -(Z+X), bB = -(rv+Y), bC = -(Y+rw);
+                Real sA = ru+Z, sB = Z+rv, sC = X+Y;
                 if (t.useDamping || t.invDampingRadius > 0) {
                     Real len[3] = {sqrt(u), sqrt(v), sqrt(w)};
                     const Real widthA = sWidth[a], widthB = sWidth[b];
@@ -494,28 +496,34 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                             f[k] *= g;
                         }
                     }
+                    // damped gradients: dA' = dA f + A sa, dB' = dB f + B sb, dC' = dC f + C sc
                     Real damping = f[0]*f[1]*f[2];
                     Real sa = e*dfdr[0]*f[1]*f[2]/len[0];
                     Real sb = e*f[0]*dfdr[1]*f[2]/len[1];
                     Real sc = e*f[0]*f[1]*dfdr[2]/len[2];
-                    for (int k = 0; k < 3; k++) {
-                        dA[k] = dA[k]*damping + A[k]*sa;
-                        dB[k] = dB[k]*damping + B[k]*sb;
-                        dC[k] = dC[k]*damping + c[k]*sc;
-                    }
+                    aA = aA*damping-sa;
+                    aB *= damping;
+                    aC = aC*damping+sc;
+                    bA *= damping;
+                    bB = bB*damping-sb;
+                    bC = bC*damping-sc;
+                    sA = sA*damping+sa;
+                    sB = sB*damping+sb;
+                    sC *= damping;
                     e *= damping;
                 }
                 energy += (double) e;
                 count++;
                 for (int k = 0; k < 3; k++) {
-                    fA[k] += dA[k]+dB[k];
-                    atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(dC[k]-dA[k]));
+                    fA[k] += A[k]*sA+B[k]*sB+c[k]*sC;
+                    atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(A[k]*aA+B[k]*aB+c[k]*aC));
+                    const Real forceB = A[k]*bA+B[k]*bB+c[k]*bC;
                     if (!rows)
-                        atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(-(dB[k]+dC[k])));
+                        atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(forceB));
                     else if (b == bFirst)
-                        fB[k] = -(dB[k]+dC[k]);
+                        fB[k] = forceB;
                     else
-                        fB1[k] = -(dB[k]+dC[k]);
+                        fB1[k] = forceB;
                 }
             }
         }
