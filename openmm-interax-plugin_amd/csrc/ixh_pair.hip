@@ -762,9 +762,11 @@ __device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 til
 }
 
 // Energy and pair count of a wave, written once at its end: the launch's totals are then summed in wave order.
-__device__ __forceinline__ void store_wave_totals(double* waveEnergy, int* waveCount, long long wave, int lane, double energy, int pairs) {
+// pairs: this lane's count, or with waveTotal the wave's.
+__device__ __forceinline__ void store_wave_totals(double* waveEnergy, int* waveCount, long long wave, int lane, double energy, int pairs,
+                                                  bool waveTotal = false) {
     const double e = wave_sum(energy);
-    const int c = (int) wave_sum((double) pairs);
+    const int c = (waveTotal ? pairs : (int) wave_sum((double) pairs));
     if (lane == 0) {
         waveEnergy[wave] = e;
         waveCount[wave] = c;
@@ -898,7 +900,7 @@ struct QueueLds {
 
 template <typename Real, int TERMS, bool TABLES>
 __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const PairTables& tb, QueueLds<Real, TABLES>& q, int head,
-                                               int valid, int lane, double& energy, int& pairs) {
+                                               int valid, int lane, double& energy) {
     typedef typename Vec4T<Real>::type R4;
     typedef typename ParticleType<Real, TABLES>::type Particle;
     if (lane >= valid)
@@ -918,14 +920,14 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
     Real en, fs;
     evaluate_pair<Real, TERMS>(a, tb, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
     energy += (double) en;
-    pairs++;
     const long long fx = to_fixed(dx*fs), fy = to_fixed(dy*fs), fz = to_fixed(dz*fs);
     atomicAdd(&q.fI[0][il], (unsigned long long) fx);
     atomicAdd(&q.fI[1][il], (unsigned long long) fy);
     atomicAdd(&q.fI[2][il], (unsigned long long) fz);
-    atomicAdd(&q.fJ[0][jl], (unsigned long long) -fx);
-    atomicAdd(&q.fJ[1][jl], (unsigned long long) -fy);
-    atomicAdd(&q.fJ[2][jl], (unsigned long long) -fz);
+    // the j accumulators sum the forces on i: their exact negation is added at the flush
+    atomicAdd(&q.fJ[0][jl], (unsigned long long) fx);
+    atomicAdd(&q.fJ[1][jl], (unsigned long long) fy);
+    atomicAdd(&q.fJ[2][jl], (unsigned long long) fz);
 }
 
 // One lane's partner particle of a tile, loaded from global memory.
@@ -1087,7 +1089,8 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
             }
             while (count >= BLOCK) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy, pairs);
+                evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy);
+                pairs += BLOCK;
                 head = (head+BLOCK) & (QUEUE-1);
                 count -= BLOCK;
             }
@@ -1096,16 +1099,17 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
         }
         if (count > 0) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, count, lane, energy, pairs);
+            evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, count, lane, energy);
+            pairs += count;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         {
             const int flagJ = q.j.flags[lane];
             if (flagJ >= 0) {
                 int oj = flagJ & INDEX_MASK;
-                add_fixed(&a.force[oj], (long long) q.fJ[0][lane]);
-                add_fixed(&a.force[a.stride+oj], (long long) q.fJ[1][lane]);
-                add_fixed(&a.force[2*a.stride+oj], (long long) q.fJ[2][lane]);
+                add_fixed(&a.force[oj], -(long long) q.fJ[0][lane]);
+                add_fixed(&a.force[a.stride+oj], -(long long) q.fJ[1][lane]);
+                add_fixed(&a.force[2*a.stride+oj], -(long long) q.fJ[2][lane]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1116,7 +1120,7 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
         add_fixed(&a.force[a.stride+oi], (long long) q.fI[1][lane]);
         add_fixed(&a.force[2*a.stride+oi], (long long) q.fI[2][lane]);
     }
-    store_wave_totals(a.waveEnergy, a.waveCount, wave, lane, energy, pairs);
+    store_wave_totals(a.waveEnergy, a.waveCount, wave, lane, energy, pairs, true);
 }
 
 // Sum the per-wave energies and pair counts of a launch (one workgroup, a fixed assignment of partials to threads
