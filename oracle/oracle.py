@@ -252,6 +252,61 @@ def threebody_count(force, positions, box=None):
     return lib().orc_threebody_count(ctypes.byref(s), _d(pos), _d(bx))
 
 
+# ---------------------------------------------------------------------------------------------------------------
+# the multi-threaded restatement of the CPU platform (oracle/libcpuplatform.so): the timed CPU baseline
+# ---------------------------------------------------------------------------------------------------------------
+
+CPU_PLATFORM_PATH = os.path.join(_HERE, "libcpuplatform.so")
+_cpl = None
+
+
+def cpu_platform_lib() -> ctypes.CDLL:
+    global _cpl
+    if _cpl is None:
+        if not os.path.exists(CPU_PLATFORM_PATH):
+            build()
+        L = ctypes.CDLL(CPU_PLATFORM_PATH)
+        L.cpl_pair_compute.argtypes = [ctypes.POINTER(PairForce), _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                       ctypes.POINTER(ctypes.c_longlong)]
+        L.cpl_pair_compute.restype = ctypes.c_int
+        L.cpl_threebody_compute.argtypes = [ctypes.POINTER(ThreeBodyForce), ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                            _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.cpl_threebody_compute.restype = ctypes.c_int
+        L.cpl_num_threads.restype = ctypes.c_int
+        _cpl = L
+    return _cpl
+
+
+def cpu_pair_force(force, positions, box=None, lam=1.0, threads=0):
+    """Energy, forces (N x 3) and interacting pair count of one pair force by the FP32 multi-threaded restatement of
+    the CPU platform (no long range correction)."""
+    s, keep = _pair_struct(force, double_precision=False)
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
+    bx = np.ascontiguousarray(np.asarray(box if box is not None else np.eye(3)*1e6, dtype=np.float64).reshape(-1))
+    forces = np.zeros(pos.size)
+    e = ctypes.c_double()
+    pairs = ctypes.c_longlong()
+    if cpu_platform_lib().cpl_pair_compute(ctypes.byref(s), _d(pos), _d(bx), lam, threads, _d(forces), ctypes.byref(e),
+                                           ctypes.byref(pairs)) != 0:
+        raise ValueError("cpu platform: unsupported pair force (triclinic box)")
+    return e.value, forces.reshape(-1, 3), pairs.value
+
+
+def cpu_threebody_force(force, positions, box=None, lam=1.0, threads=0):
+    """Energy, forces and interacting triple count of a three-body force by the multi-threaded restatement of the
+    CPU platform."""
+    s, keep = _threebody_struct(force)
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
+    bx = np.ascontiguousarray(np.asarray(box if box is not None else np.eye(3)*1e6, dtype=np.float64).reshape(-1))
+    forces = np.zeros(pos.size)
+    e = ctypes.c_double()
+    triples = ctypes.c_longlong()
+    if cpu_platform_lib().cpl_threebody_compute(ctypes.byref(s), pos.size//3, _d(pos), _d(bx), lam, threads, _d(forces),
+                                                ctypes.byref(e), ctypes.byref(triples)) != 0:
+        raise ValueError("cpu platform: invalid three-body force")
+    return e.value, forces.reshape(-1, 3), triples.value
+
+
 def evaluate(system, positions, box=None, groups=0xFFFFFFFF, parameters=None, double_precision=True, with_scale=False):
     """Energy and forces of the force groups of a System, as Context.getState(groups) on the Reference platform.
     with_scale also returns the sum of |pair energies| of the pair forces (the rounding-error scale)."""
