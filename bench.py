@@ -503,6 +503,51 @@ def bench_serve(args, catalog_size=256, requests_per_rate=2000, rates=(1000, 200
             "rates": table}
 
 
+def bench_lambda(args, windows=16):
+    """Lambda-window fusion on the alchemical job of the served configuration (a 12-atom alchemical solute in 1000
+    water-like molecules, periodic 3.106 nm box, cutoff 0.9 nm, soft-core radius 0.3 nm): the energies of one
+    configuration at 16 values of lambda_sterics, as 16 evaluations (one getState per value) and as one fused call
+    (one evaluation plus the alchemical pairs at every value).  A unit is one window energy."""
+    import interax_hip as ih
+    import workloads
+    system, pos = workloads.alchemical_box(0.3, seed=2)
+    values = np.linspace(0.0, 1.0, windows)
+    context = ih.Context(system, {"Precision": "mixed"})
+    context.setPositions(pos)
+    for _ in range(args.warmup):
+        context.getEnergiesAtLambdas(values)
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        fused = context.getEnergiesAtLambdas(values)
+    fused_s = (time.perf_counter()-start)/args.steps
+
+    def separate():
+        out = []
+        for v in values:
+            context.setParameter("lambda_sterics", float(v))
+            out.append(context.getState(getForces=False).getPotentialEnergy())
+        return np.array(out)
+    separate()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        reference = separate()
+    separate_s = (time.perf_counter()-start)/args.steps
+    context.close()
+    log(f"[bench] {windows} window energies: fused {fused_s*1e3:.3f} ms, separate {separate_s*1e3:.3f} ms "
+        f"({separate_s/fused_s:.1f}x); max difference {np.abs(fused-reference).max():.3e} kJ/mol")
+    return {"metric": "lambda-window energies/s", "value": windows/fused_s, "unit": "window energies/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": fused_s*1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"energies of one configuration of the alchemical job (12-atom alchemical solute in "
+                                   f"1000 water-like molecules, 3012 atoms, periodic 3.106 nm, cutoff 0.9 nm, soft-core "
+                                   f"radius 0.3 nm, long range correction) at {windows} values of lambda_sterics",
+                       "atoms": system.getNumParticles(), "windows": windows},
+            "separate": {"value": windows/separate_s, "unit": "window energies/s", "ms_per_16": separate_s*1e3,
+                         "form": "one evaluation (getState, energy only) per lambda value"},
+            "speedup_vs_separate": separate_s/fused_s,
+            "max_difference_kj_mol": float(np.abs(fused-reference).max())}
+
+
 def copy_calibration(args):
     """A device-to-device copy of a known size: the HBM counters of its blit kernel calibrate FETCH_SIZE and
     WRITE_SIZE (bytes per counted unit) for the traffic figures of the other kernels."""
@@ -525,7 +570,7 @@ def main():
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=100)
     parser.add_argument("--warmup", type=int, default=5)
-    parser.add_argument("--workload", default="pair", choices=["pair", "triplet", "batch", "serve", "copy"])
+    parser.add_argument("--workload", default="pair", choices=["pair", "triplet", "batch", "serve", "lambda", "copy"])
     parser.add_argument("--skip-oracle", action="store_true", help="no accuracy check and no CPU baseline")
     args = parser.parse_args()
     if args.gpus != 1:
@@ -540,6 +585,8 @@ def main():
         result = bench_batch(args)
     elif args.workload == "serve":
         result = bench_serve(args)
+    elif args.workload == "lambda":
+        result = bench_lambda(args)
     else:
         result = bench_pair(args)
     print(json.dumps(result), flush=True)

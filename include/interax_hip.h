@@ -176,6 +176,15 @@ int ixh_pair_execute(ixh_pair* pair, const ixh_buffers* buffers, int terms, int 
 int ixh_pair_long_range_coefficients(ixh_pair* pair, double* coefficient, double* alchemical_coefficient);
 int ixh_pair_long_range_energy(ixh_pair* pair, const double* box, double lambda, double* energy);
 /*
+ * Energies of the selected terms at count (1 to 16) values of lambda_sterics, at the positions of the buffers
+ * (lambda-window fusion; no counterpart in the reference, which evaluates one lambda per Context: a free-energy
+ * analysis needs the energy of one configuration at every window).  Only pairs with exactly one alchemical particle
+ * depend on lambda (ReferenceInteraxPairIxn.h:100-125): one evaluation at buffers->lambda (into the handle's own
+ * scratch buffers: the buffers' forces and energy are not touched), then those pairs at every lambda.  The long range
+ * correction is not included (ixh_pair_long_range_energy).  Waits for the result.
+ */
+int ixh_pair_lambda_energies(ixh_pair* pair, const ixh_buffers* buffers, int terms, const double* lambdas, int count, double* energies);
+/*
  * Batches of independent systems (no counterpart in the reference; used by the batched and served forms): the
  * particles of num_segments systems are concatenated, segment s owning particles [particle_offsets[s],
  * particle_offsets[s+1]); pairs never cross segments.  The descriptor describes the concatenated system (exclusions in

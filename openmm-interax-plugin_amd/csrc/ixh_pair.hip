@@ -1300,6 +1300,110 @@ int grid_for(long long n, int threads) {
     return (int) std::max(1LL, (n+threads-1)/threads);
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// Energies at several lambda values (no counterpart in the reference)
+// ------------------------------------------------------------------------------------------------------------
+
+constexpr int MAX_LAMBDA_WINDOWS = 16;
+
+struct LambdaArgs {
+    const void* posq;
+    const float4* correction;
+    int precision;
+    int n;
+    const double* params;       // 12 per particle, in PairParticle order, original particle order
+    const int* flags;           // original index | alchemical bit
+    const int* alchList;        // the alchemical particles
+    const int* exclusionOffsets;
+    const int* exclusionPartners;
+    int method;
+    BoxD box;
+    double cutoff;
+    PairSettings settings;
+    int useSwitch;
+    double switchDistance, invSwitchWidth;
+    double softcoreR6;          // R^6, 0 without soft core
+    int count;                  // lambda values
+    double lambdas[MAX_LAMBDA_WINDOWS+1];
+};
+
+// One wave per alchemical particle a: the energy of its pairs with the non-alchemical particles (the only pairs whose
+// energy depends on lambda: ReferenceInteraxPairIxn.h:100-125 scales a pair with exactly one alchemical particle by
+// lambda and evaluates it at r_eff), for every lambda, in FP32 arithmetic with FP64 sums.  Membership is the
+// reference's FP64 cutoff decision.  Real: the arithmetic type of the handle.
+template <typename Real, int TERMS>
+__global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, double* out) {
+    const int i = a.alchList[blockIdx.x];
+    const int lane = threadIdx.x;
+    double xi, yi, zi;
+    load_position(a.posq, a.correction, a.precision, i, xi, yi, zi);
+    PairParticle<Real> P;
+    {
+        const double* q = a.params+12*(size_t) i;
+        P.m1 = Real(q[0]); P.m2 = Real(q[1]); P.m3 = Real(q[2]); P.alpha = Real(q[3]); P.wd = Real(q[4]); P.amp = Real(q[5]);
+        P.wx = Real(q[6]); P.iwx = Real(q[7]); P.k = Real(q[8]); P.l = Real(q[9]); P.ws = Real(q[10]); P.iws = Real(q[11]);
+    }
+    double acc[MAX_LAMBDA_WINDOWS+1];
+#pragma unroll
+    for (int k = 0; k <= MAX_LAMBDA_WINDOWS; k++)
+        acc[k] = 0;
+    for (int j = lane; j < a.n; j += 64) {
+        if (j == i || (a.flags[j] & ALCHEMICAL_BIT) != 0 || excluded(a.exclusionOffsets, a.exclusionPartners, i, j))
+            continue;
+        double xj, yj, zj;
+        load_position(a.posq, a.correction, a.precision, j, xj, yj, zj);
+        double dx = xj-xi, dy = yj-yi, dz = zj-zi;
+        if (a.method == IXH_CUTOFF_PERIODIC)
+            reference_min_image(dx, dy, dz, a.box);
+        if (a.method != IXH_NO_CUTOFF &&
+            !exact_within_cutoff(a.posq, a.correction, a.precision, a.method == IXH_CUTOFF_PERIODIC, a.box, a.cutoff, i, j))
+            continue;
+        const Real r = Real(sqrt(dx*dx+dy*dy+dz*dz));
+        PairParticle<Real> Q;
+        {
+            const double* q = a.params+12*(size_t) j;
+            Q.m1 = Real(q[0]); Q.m2 = Real(q[1]); Q.m3 = Real(q[2]); Q.alpha = Real(q[3]); Q.wd = Real(q[4]); Q.amp = Real(q[5]);
+            Q.wx = Real(q[6]); Q.iwx = Real(q[7]); Q.k = Real(q[8]); Q.l = Real(q[9]); Q.ws = Real(q[10]); Q.iws = Real(q[11]);
+        }
+        Real sw = 1;
+        if (a.useSwitch && r > Real(a.switchDistance)) {
+            const Real x = (r-Real(a.switchDistance))*Real(a.invSwitchWidth);
+            sw = 1+x*x*x*(Real(-10)+x*(Real(15)-x*Real(6)));
+        }
+        Real eHard = 0, dEdR;
+        if (a.softcoreR6 == 0.0)
+            pair_energy<Real, TERMS>(r, Real(1)/r, P, Q, i < j, a.settings, eHard, dEdR);
+#pragma unroll
+        for (int k = 0; k <= MAX_LAMBDA_WINDOWS; k++) {
+            if (k >= a.count)
+                break;
+            const Real lambda = Real(a.lambdas[k]);
+            Real e = eHard;
+            if (a.softcoreR6 != 0.0) {
+                const Real rEff = pow(r*r*r*r*r*r+(Real(1)-lambda)*Real(a.softcoreR6), Real(1.0/6.0));
+                pair_energy<Real, TERMS>(rEff, Real(1)/rEff, P, Q, i < j, a.settings, e, dEdR);
+            }
+            acc[k] += (double) (lambda*(e*sw));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k <= MAX_LAMBDA_WINDOWS; k++) {
+        if (k >= a.count)
+            break;
+        const double total = wave_sum(acc[k]);
+        if (lane == 0)
+            out[(size_t) blockIdx.x*a.count+k] = total;
+    }
+}
+
+__global__ void k_zero_fixed(long long* p, long long n, double* energy) {
+    long long i = (long long) blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+    if (i == 0)
+        *energy = 0;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------------------
@@ -1366,6 +1470,16 @@ struct ixh_pair {
     DeviceArray<char> sortTemp;
     DeviceArray<int> flagDev;
     DeviceArray<long long> pairCount;
+    // energies at several lambda values: generic FP32 parameters in particle order, the alchemical particles, and the
+    // scratch buffers of the handle's own evaluation
+    int paramVersion = 0, lambdaParamVersion = -1;
+    DeviceArray<double> lambdaParams;
+    DeviceArray<int> alchList;
+    int numAlchemical = 0;
+    DeviceArray<double> lambdaOut, lambdaEnergy;
+    DeviceArray<long long> lambdaForce;
+    double* lambdaHost = nullptr;   // pinned
+    size_t lambdaHostCount = 0;
     int* hostFlag = nullptr;
     hipEvent_t flagCopied = nullptr;   // deferred list checks: the flag has reached hostFlag
     bool deferredCheck = false;
@@ -1614,6 +1728,7 @@ bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls
 // Copy the per-particle parameters of a descriptor into the handle and onto the device.
 void load_parameters(ixh_pair* p, const ixh_pair_desc* d, hipStream_t stream) {
     int n = p->n;
+    p->paramVersion++;
     if (d->terms & IXH_TERM_DISPERSION) {
         p->dispersion.assign(d->dispersion, d->dispersion+5*(size_t) n);
         p->dampingScale = d->damping_scale;
@@ -2220,6 +2335,8 @@ int ixh_pair_destroy(ixh_pair* p) {
             (void) hipEventDestroy(p->flagCopied);
         if (p->subsetStaging != nullptr)
             (void) hipHostFree(p->subsetStaging);
+        if (p->lambdaHost != nullptr)
+            (void) hipHostFree(p->lambdaHost);
         delete p;
     });
 }
@@ -2392,6 +2509,122 @@ int ixh_pair_long_range_energy(ixh_pair* p, const double* box, double lambda, do
             e = (p->lrcCoefficient+lam*p->lrcAlchemical)/(box[0]*box[4]*box[8]);
         }
         *energy = e;
+    });
+}
+
+int ixh_pair_lambda_energies(ixh_pair* p, const ixh_buffers* b, int terms, const double* lambdas, int count, double* energies) {
+    return guarded([&] {
+        if (p == nullptr || b == nullptr || lambdas == nullptr || energies == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_lambda_energies: null argument");
+        if (count < 1 || count > MAX_LAMBDA_WINDOWS)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_lambda_energies: between 1 and 16 lambda values");
+        if (b->num_particles != p->n)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_lambda_energies: the buffers hold a different number of particles");
+        terms &= p->terms;
+        std::fill(energies, energies+count, 0.0);
+        if (terms == 0 || p->n == 0)
+            return;
+        DeviceGuard guard(p->device);
+        hipStream_t stream = (hipStream_t) b->stream;
+        // 1. the handle's own evaluation at the buffers' lambda, into scratch forces and energy
+        const long long forceWords = 3LL*b->force_stride;
+        p->lambdaForce.reserve(forceWords);
+        p->lambdaEnergy.reserve(1);
+        hipLaunchKernelGGL(k_zero_fixed, dim3(grid_for(forceWords, 256)), dim3(256), 0, stream, p->lambdaForce.ptr, forceWords,
+                           p->lambdaEnergy.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        ixh_buffers own = *b;
+        own.force = p->lambdaForce.ptr;
+        own.energy = p->lambdaEnergy.ptr;
+        const int status = ixh_pair_execute(p, &own, terms, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY);
+        if (status != IXH_OK)
+            throw Error(status, ixh_last_error());
+        // 2. the pairs of the alchemical particles at every lambda and at the buffers' own
+        const int values = count+1;
+        if (p->anyAlchemical) {
+            if (p->lambdaParamVersion != p->paramVersion) {
+                std::vector<double> q(12*(size_t) p->n, 0.0);
+                std::vector<int> alch;
+                for (int i = 0; i < p->n; i++) {
+                    double* r = &q[12*(size_t) i];
+                    if (p->terms & IXH_TERM_DISPERSION)
+                        for (int k = 0; k < 5; k++)
+                            r[k] = p->dispersion[5*(size_t) i+k];
+                    if (p->terms & IXH_TERM_EXCHANGE) {
+                        r[5] = p->exchange[2*(size_t) i];
+                        r[6] = p->exchange[2*(size_t) i+1];
+                        r[7] = (p->exchange[2*(size_t) i+1] != 0 ? 1.0/p->exchange[2*(size_t) i+1] : 0.0);
+                    }
+                    if (p->terms & IXH_TERM_SHORT_RANGE) {
+                        r[8] = p->shortRange[3*(size_t) i];
+                        r[9] = p->shortRange[3*(size_t) i+1];
+                        r[10] = p->shortRange[3*(size_t) i+2];
+                        r[11] = (p->shortRange[3*(size_t) i+2] != 0 ? 1.0/p->shortRange[3*(size_t) i+2] : 0.0);
+                    }
+                    if (p->alchemical[i])
+                        alch.push_back(i);
+                }
+                p->lambdaParams.upload(q.data(), q.size(), stream);
+                p->alchList.upload(alch.data(), std::max<size_t>(alch.size(), 1), stream);
+                p->numAlchemical = (int) alch.size();
+                p->lambdaParamVersion = p->paramVersion;
+            }
+            LambdaArgs a;
+            a.posq = b->posq;
+            a.correction = (const float4*) b->posq_correction;
+            a.precision = b->precision;
+            a.n = p->n;
+            a.params = p->lambdaParams.ptr;
+            a.flags = p->flags.ptr;
+            a.alchList = p->alchList.ptr;
+            a.exclusionOffsets = p->exclusionOffsets.ptr;
+            a.exclusionPartners = p->exclusionPartners.ptr;
+            a.method = p->method;
+            a.box = make_box(b->box);
+            a.cutoff = p->cutoff;
+            a.settings = p->settings;
+            a.useSwitch = (p->method != IXH_NO_CUTOFF && p->useSwitch);
+            a.switchDistance = p->switchDistance;
+            a.invSwitchWidth = (a.useSwitch ? 1.0/(p->cutoff-p->switchDistance) : 0.0);
+            a.softcoreR6 = std::pow(p->softcoreRadius, 6);
+            a.count = values;
+            for (int k = 0; k < count; k++)
+                a.lambdas[k] = lambdas[k];
+            a.lambdas[count] = b->lambda;
+            p->lambdaOut.reserve((size_t) p->numAlchemical*values);
+            switch (terms) {
+#define IXH_LAMBDA_CASE(T) case T: \
+                if (p->precision == IXH_DOUBLE) \
+                    hipLaunchKernelGGL((k_lambda_energies<double, T>), dim3(p->numAlchemical), dim3(64), 0, stream, a, p->lambdaOut.ptr); \
+                else \
+                    hipLaunchKernelGGL((k_lambda_energies<float, T>), dim3(p->numAlchemical), dim3(64), 0, stream, a, p->lambdaOut.ptr); \
+                break;
+                IXH_LAMBDA_CASE(1) IXH_LAMBDA_CASE(2) IXH_LAMBDA_CASE(3) IXH_LAMBDA_CASE(4) IXH_LAMBDA_CASE(5) IXH_LAMBDA_CASE(6)
+                IXH_LAMBDA_CASE(7)
+#undef IXH_LAMBDA_CASE
+            }
+            IXH_HIP_CHECK(hipGetLastError());
+        }
+        // 3. E(lambda_k) = E(own lambda) - E_alchemical(own lambda) + E_alchemical(lambda_k), summed in a fixed order
+        const size_t words = 1+(p->anyAlchemical ? (size_t) p->numAlchemical*values : 0);
+        if (words > p->lambdaHostCount) {
+            if (p->lambdaHost != nullptr)
+                IXH_HIP_CHECK(hipHostFree(p->lambdaHost));
+            p->lambdaHost = nullptr;
+            IXH_HIP_CHECK(hipHostMalloc((void**) &p->lambdaHost, sizeof(double)*words, hipHostMallocDefault));
+            p->lambdaHostCount = words;
+        }
+        IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost, p->lambdaEnergy.ptr, sizeof(double), hipMemcpyDeviceToHost, stream));
+        if (words > 1)
+            IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost+1, p->lambdaOut.ptr, sizeof(double)*(words-1), hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        const double ownEnergy = p->lambdaHost[0];
+        std::vector<double> alchemical(values, 0.0);
+        for (int i = 0; i < (p->anyAlchemical ? p->numAlchemical : 0); i++)
+            for (int k = 0; k < values; k++)
+                alchemical[k] += p->lambdaHost[1+(size_t) i*values+k];
+        for (int k = 0; k < count; k++)
+            energies[k] = ownEnergy-alchemical[count]+alchemical[k];
     });
 }
 

@@ -98,6 +98,7 @@ SIGNATURES = {
     "ixh_pair_execute": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int],
     "ixh_pair_long_range_coefficients": [ctypes.c_void_p, _dp, _dp],
     "ixh_pair_long_range_energy": [ctypes.c_void_p, _dp, ctypes.c_double, _dp],
+    "ixh_pair_lambda_energies": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp],
     "ixh_pair_last_pair_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
     "ixh_pair_list_statistics": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "ixh_threebody_create": [ctypes.c_int, ctypes.POINTER(ThreeBodyDesc), ctypes.POINTER(ctypes.c_void_p)],
@@ -766,6 +767,51 @@ class Context:
             forces = np.zeros((self.system.getNumParticles(), 3))
             _check(_lib.ixh_state_get_forces(self._state, _dptr(forces)))
         return State(energy.value+host_energy, forces)
+
+    def getEnergiesAtLambdas(self, values, parameter: str = "lambda_sterics", groups: int = 0xFFFFFFFF) -> np.ndarray:
+        """Potential energy of the current positions with the context parameter `parameter` set to each of `values`
+        (lambda-window fusion; an extension without a counterpart in the reference, where each value needs its own
+        getState()).  Pair forces: one evaluation plus the pairs of the alchemical particles at every value
+        (ixh_pair_lambda_energies), and the long range correction at every value; three-body forces: the energy is
+        linear in lambda, evaluated at 0 and 1.  Forces of other parameters are taken at their current values.  The
+        Context's own force and energy buffers are used as scratch."""
+        values = np.asarray(values, dtype=np.float64).reshape(-1)
+        out = np.zeros(values.size)
+        for g in self._groups:
+            terms = 0
+            for f in g.forces:
+                if groups & (1 << f.getForceGroup()):
+                    terms |= f.TERM
+            if terms == 0:
+                continue
+            name = g.forces[0].getAlchemicalParameterName()
+            lam = self._parameters.get(name, 1.0)
+            b = self._buffers(lam)
+            for start in range(0, values.size, 16):
+                chunk = values[start:start+16] if name == parameter else np.full(min(16, values.size-start), lam)
+                chunk = np.ascontiguousarray(chunk)
+                e = np.zeros(chunk.size)
+                _check(_lib.ixh_pair_lambda_energies(g.handle, ctypes.byref(b), terms, _dptr(chunk), chunk.size, _dptr(e)))
+                out[start:start+chunk.size] += e
+                if terms & IXH_TERM_DISPERSION:
+                    for k, v in enumerate(chunk):
+                        lrc = ctypes.c_double()
+                        _check(_lib.ixh_pair_long_range_energy(g.handle, _dptr(self._box), float(v), ctypes.byref(lrc)))
+                        out[start+k] += lrc.value
+        for t in self._threebody:
+            if not groups & (1 << t.force.getForceGroup()):
+                continue
+            name = t.force.getAlchemicalParameterName()
+            ends = []
+            for lam in ((0.0, 1.0) if name == parameter else (self._parameters.get(name, 1.0),)*2):
+                _check(_lib.ixh_state_clear(self._state))
+                b = self._buffers(lam)
+                _check(_lib.ixh_threebody_execute(t.handle, ctypes.byref(b), IXH_INCLUDE_ENERGY))
+                e = ctypes.c_double()
+                _check(_lib.ixh_state_get_energy(self._state, ctypes.byref(e)))
+                ends.append(e.value)
+            out += ends[0]+values*(ends[1]-ends[0]) if name == parameter else ends[0]
+        return out
 
     def updateParametersInContext(self, force) -> None:
         """Force.updateParametersInContext (InteraxPairForceImpl.h:97-100, InteraxThreeBodyForceImpl.cpp:92-95)."""
