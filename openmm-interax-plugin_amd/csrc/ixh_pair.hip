@@ -1327,16 +1327,47 @@ struct LambdaArgs {
     double lambdas[MAX_LAMBDA_WINDOWS+1];
 };
 
-// One wave per alchemical particle a: the energy of its pairs with the non-alchemical particles (the only pairs whose
-// energy depends on lambda: ReferenceInteraxPairIxn.h:100-125 scales a pair with exactly one alchemical particle by
-// lambda and evaluates it at r_eff), for every lambda, in FP32 arithmetic with FP64 sums.  Membership is the
-// reference's FP64 cutoff decision.  Real: the arithmetic type of the handle.
+constexpr int LAMBDA_CHUNK = 256;   // partners scanned by one wave
+
+// One wave per (alchemical particle a, chunk of LAMBDA_CHUNK partners): the energy of the pairs of a with the
+// non-alchemical particles of the chunk (the only pairs whose energy depends on lambda: ReferenceInteraxPairIxn.h:100-125
+// scales a pair with exactly one alchemical particle by lambda and evaluates it at r_eff), for every lambda, in Real
+// arithmetic with FP64 sums.  Membership is the reference's FP64 cutoff decision.  The members of the chunk are first
+// compacted into LDS so that every lane of an evaluation batch holds a pair.  out[wave*count+k]: the wave's sum at
+// lambda k.
 template <typename Real, int TERMS>
-__global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, double* out) {
-    const int i = a.alchList[blockIdx.x];
+__global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, int chunks, double* out) {
+    __shared__ int sIndex[LAMBDA_CHUNK];
+    __shared__ double sDist[LAMBDA_CHUNK];
+    const int i = a.alchList[blockIdx.x/chunks];
+    const int first = (blockIdx.x%chunks)*LAMBDA_CHUNK;
     const int lane = threadIdx.x;
     double xi, yi, zi;
     load_position(a.posq, a.correction, a.precision, i, xi, yi, zi);
+    int members = 0;
+    for (int j = first+lane; j < first+LAMBDA_CHUNK; j += 64) {
+        bool in = false;
+        double r2 = 0;
+        if (j < a.n && j != i && (a.flags[j] & ALCHEMICAL_BIT) == 0 &&
+            !excluded(a.exclusionOffsets, a.exclusionPartners, i, j)) {
+            double xj, yj, zj;
+            load_position(a.posq, a.correction, a.precision, j, xj, yj, zj);
+            double dx = xj-xi, dy = yj-yi, dz = zj-zi;
+            if (a.method == IXH_CUTOFF_PERIODIC)
+                reference_min_image(dx, dy, dz, a.box);
+            r2 = dx*dx+dy*dy+dz*dz;
+            in = (a.method == IXH_NO_CUTOFF ||
+                  exact_within_cutoff(a.posq, a.correction, a.precision, a.method == IXH_CUTOFF_PERIODIC, a.box, a.cutoff, i, j));
+        }
+        const unsigned long long ballot = __ballot(in);
+        if (in) {
+            const int slot = members+__popcll(ballot & ((1ULL << lane)-1));
+            sIndex[slot] = j;
+            sDist[slot] = sqrt(r2);
+        }
+        members += __popcll(ballot);
+    }
+    __syncthreads();
     PairParticle<Real> P;
     {
         const double* q = a.params+12*(size_t) i;
@@ -1347,18 +1378,9 @@ __global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, double* ou
 #pragma unroll
     for (int k = 0; k <= MAX_LAMBDA_WINDOWS; k++)
         acc[k] = 0;
-    for (int j = lane; j < a.n; j += 64) {
-        if (j == i || (a.flags[j] & ALCHEMICAL_BIT) != 0 || excluded(a.exclusionOffsets, a.exclusionPartners, i, j))
-            continue;
-        double xj, yj, zj;
-        load_position(a.posq, a.correction, a.precision, j, xj, yj, zj);
-        double dx = xj-xi, dy = yj-yi, dz = zj-zi;
-        if (a.method == IXH_CUTOFF_PERIODIC)
-            reference_min_image(dx, dy, dz, a.box);
-        if (a.method != IXH_NO_CUTOFF &&
-            !exact_within_cutoff(a.posq, a.correction, a.precision, a.method == IXH_CUTOFF_PERIODIC, a.box, a.cutoff, i, j))
-            continue;
-        const Real r = Real(sqrt(dx*dx+dy*dy+dz*dz));
+    for (int m = lane; m < members; m += 64) {
+        const int j = sIndex[m];
+        const Real r = Real(sDist[m]);
         PairParticle<Real> Q;
         {
             const double* q = a.params+12*(size_t) j;
@@ -1394,6 +1416,17 @@ __global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, double* ou
         if (lane == 0)
             out[(size_t) blockIdx.x*a.count+k] = total;
     }
+}
+
+// sums[k] = sum over the rows w of out[w*count+k], in a fixed order (one wave per column k).
+__global__ void __launch_bounds__(64) k_lambda_columns(const double* out, int rows, int count, double* sums) {
+    const int k = blockIdx.x;
+    double v = 0;
+    for (int w = threadIdx.x; w < rows; w += 64)
+        v += out[(size_t) w*count+k];
+    v = wave_sum(v);
+    if (threadIdx.x == 0)
+        sums[k] = v;
 }
 
 __global__ void k_zero_fixed(long long* p, long long n, double* energy) {
@@ -2591,22 +2624,29 @@ int ixh_pair_lambda_energies(ixh_pair* p, const ixh_buffers* b, int terms, const
             for (int k = 0; k < count; k++)
                 a.lambdas[k] = lambdas[k];
             a.lambdas[count] = b->lambda;
-            p->lambdaOut.reserve((size_t) p->numAlchemical*values);
-            switch (terms) {
+            const int chunks = (p->n+LAMBDA_CHUNK-1)/LAMBDA_CHUNK;
+            const int rows = p->numAlchemical*chunks;
+            p->lambdaOut.reserve((size_t) rows*values+values);
+            double* sums = p->lambdaOut.ptr+(size_t) rows*values;
+            if (rows > 0) {
+                switch (terms) {
 #define IXH_LAMBDA_CASE(T) case T: \
-                if (p->precision == IXH_DOUBLE) \
-                    hipLaunchKernelGGL((k_lambda_energies<double, T>), dim3(p->numAlchemical), dim3(64), 0, stream, a, p->lambdaOut.ptr); \
-                else \
-                    hipLaunchKernelGGL((k_lambda_energies<float, T>), dim3(p->numAlchemical), dim3(64), 0, stream, a, p->lambdaOut.ptr); \
-                break;
-                IXH_LAMBDA_CASE(1) IXH_LAMBDA_CASE(2) IXH_LAMBDA_CASE(3) IXH_LAMBDA_CASE(4) IXH_LAMBDA_CASE(5) IXH_LAMBDA_CASE(6)
-                IXH_LAMBDA_CASE(7)
+                    if (p->precision == IXH_DOUBLE) \
+                        hipLaunchKernelGGL((k_lambda_energies<double, T>), dim3(rows), dim3(64), 0, stream, a, chunks, p->lambdaOut.ptr); \
+                    else \
+                        hipLaunchKernelGGL((k_lambda_energies<float, T>), dim3(rows), dim3(64), 0, stream, a, chunks, p->lambdaOut.ptr); \
+                    break;
+                    IXH_LAMBDA_CASE(1) IXH_LAMBDA_CASE(2) IXH_LAMBDA_CASE(3) IXH_LAMBDA_CASE(4) IXH_LAMBDA_CASE(5) IXH_LAMBDA_CASE(6)
+                    IXH_LAMBDA_CASE(7)
 #undef IXH_LAMBDA_CASE
+                }
+                IXH_HIP_CHECK(hipGetLastError());
             }
+            hipLaunchKernelGGL(k_lambda_columns, dim3(values), dim3(64), 0, stream, p->lambdaOut.ptr, rows, values, sums);
             IXH_HIP_CHECK(hipGetLastError());
         }
-        // 3. E(lambda_k) = E(own lambda) - E_alchemical(own lambda) + E_alchemical(lambda_k), summed in a fixed order
-        const size_t words = 1+(p->anyAlchemical ? (size_t) p->numAlchemical*values : 0);
+        // 3. E(lambda_k) = E(own lambda) - E_alchemical(own lambda) + E_alchemical(lambda_k)
+        const size_t words = 1+(p->anyAlchemical ? (size_t) values : 0);
         if (words > p->lambdaHostCount) {
             if (p->lambdaHost != nullptr)
                 IXH_HIP_CHECK(hipHostFree(p->lambdaHost));
@@ -2615,16 +2655,15 @@ int ixh_pair_lambda_energies(ixh_pair* p, const ixh_buffers* b, int terms, const
             p->lambdaHostCount = words;
         }
         IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost, p->lambdaEnergy.ptr, sizeof(double), hipMemcpyDeviceToHost, stream));
-        if (words > 1)
-            IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost+1, p->lambdaOut.ptr, sizeof(double)*(words-1), hipMemcpyDeviceToHost, stream));
+        if (words > 1) {
+            const size_t rows = (size_t) p->numAlchemical*((p->n+LAMBDA_CHUNK-1)/LAMBDA_CHUNK);
+            IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost+1, p->lambdaOut.ptr+rows*values, sizeof(double)*values,
+                                         hipMemcpyDeviceToHost, stream));
+        }
         IXH_HIP_CHECK(hipStreamSynchronize(stream));
         const double ownEnergy = p->lambdaHost[0];
-        std::vector<double> alchemical(values, 0.0);
-        for (int i = 0; i < (p->anyAlchemical ? p->numAlchemical : 0); i++)
-            for (int k = 0; k < values; k++)
-                alchemical[k] += p->lambdaHost[1+(size_t) i*values+k];
         for (int k = 0; k < count; k++)
-            energies[k] = ownEnergy-alchemical[count]+alchemical[k];
+            energies[k] = ownEnergy+(words > 1 ? p->lambdaHost[1+k]-p->lambdaHost[1+count] : 0.0);
     });
 }
 
