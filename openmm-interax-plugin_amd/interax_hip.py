@@ -193,6 +193,7 @@ class InteraxPairForce:
         self._alchemical_name = "lambda_sterics"
         self._exclusions: List[Tuple[int, int]] = []
         self._group = 0
+        self._name = self.NAME
 
     # settings
     def getNonbondedMethod(self): return self._method
@@ -228,6 +229,8 @@ class InteraxPairForce:
     def usesPeriodicBoundaryConditions(self): return self._method == self.CutoffPeriodic
     def getForceGroup(self): return self._group
     def setForceGroup(self, g): self._group = int(g)
+    def getName(self): return self._name
+    def setName(self, name): self._name = str(name)
 
     # common per-particle views
     def _alchemical_array(self) -> np.ndarray:
@@ -351,6 +354,7 @@ class InteraxThreeBodyForce:
         self._periodic = False
         self._alchemical_name = "lambda_sterics"
         self._group = 0
+        self._name = self.NAME
 
     def getNumSites(self): return len(self._sites)
 
@@ -391,6 +395,8 @@ class InteraxThreeBodyForce:
     def setAlchemicalParameterName(self, name): self._alchemical_name = str(name)
     def getForceGroup(self): return self._group
     def setForceGroup(self, g): self._group = int(g)
+    def getName(self): return self._name
+    def setName(self, name): self._name = str(name)
 
     def site_arrays(self):
         """(offsets, particles, alpha, frequency, width, molecule, alchemical) as numpy arrays."""
@@ -408,15 +414,17 @@ class System:
     """The part of OpenMM's System the plugin needs: particle count, default box, forces."""
 
     def __init__(self, num_particles: int = 0, box: Optional[Sequence[float]] = None):
-        self._n = int(num_particles)
+        self._masses: List[float] = [1.0]*int(num_particles)
         self._forces: List[object] = []
         self._box = np.array(box if box is not None else [2, 0, 0, 0, 2, 0, 0, 0, 2], dtype=np.float64)
 
     def addParticle(self, mass=1.0):
-        self._n += 1
-        return self._n-1
+        self._masses.append(float(mass))
+        return len(self._masses)-1
 
-    def getNumParticles(self): return self._n
+    def getNumParticles(self): return len(self._masses)
+    def getParticleMass(self, index): return self._masses[index]
+    def setParticleMass(self, index, mass): self._masses[index] = float(mass)
 
     def addForce(self, force):
         self._forces.append(force)
@@ -440,6 +448,168 @@ def _b(node, name):
     return int(node.get(name)) != 0
 
 
+def _fmt(v) -> str:
+    """A double as text that reads back to the same value (shortest round-trip form)."""
+    return repr(float(v))
+
+
+def _set(node, **attrs):
+    for k, v in attrs.items():
+        if isinstance(v, bool):
+            node.set(k, "1" if v else "0")
+        elif isinstance(v, float):
+            node.set(k, _fmt(v))
+        else:
+            node.set(k, str(v))
+    return node
+
+
+def _serialize_force(force, tag: str = "Force") -> ET.Element:
+    """The version-1 node of one Interax force: Interax{Dispersion,Exchange,ShortRange,ThreeBody}ForceProxy.cpp
+    serialize (same attribute and child names, values as written by those proxies)."""
+    node = ET.Element(tag)
+    if isinstance(force, InteraxThreeBodyForce):
+        _set(node, type=force.NAME, version=1, forceGroup=force.getForceGroup(), name=force.getName(),
+             cutoff=force.getCutoffDistance(), scaleFactor=force.getScaleFactor(),
+             usesPeriodic=force.usesPeriodicBoundaryConditions(), alchemicalParameter=force.getAlchemicalParameterName(),
+             dampingMethod=force.getDampingMethod(), dampingScale=force.getDampingScale(),
+             centroidDampingRadius=force.getCentroidDampingRadius())
+        sites = ET.SubElement(node, "Sites")
+        for particles, alpha, frequency, width, molecule, alchemical in force._sites:
+            site = _set(ET.SubElement(sites, "Site"), alpha=alpha, frequency=frequency, width=width, molecule=molecule,
+                        alchemical=alchemical)
+            for q in particles:
+                _set(ET.SubElement(site, "Particle"), index=q)
+        return node
+    if not isinstance(force, InteraxPairForce):
+        raise InteraxError(f"XmlSerializer: no serialization proxy for {type(force).__name__}")
+    _set(node, type=force.NAME, version=1, forceGroup=force.getForceGroup(), name=force.getName(),
+         method=force.getNonbondedMethod(), cutoff=force.getCutoffDistance())
+    if isinstance(force, _OverlapForce):
+        _set(node, overlapMethod=force.getOverlapMethod())
+    _set(node, useSwitchingFunction=force.getUseSwitchingFunction(), switchingDistance=force.getSwitchingDistance(),
+         alchemicalParameter=force.getAlchemicalParameterName(), softcoreRadius=force.getSoftcoreRadius())
+    if isinstance(force, InteraxDispersionForce):
+        _set(node, dampingScale=force.getDampingScale(), useLongRangeCorrection=force.getUseLongRangeCorrection())
+    particles = ET.SubElement(node, "Particles")
+    for p in force._particles:
+        q = ET.SubElement(particles, "Particle")
+        if isinstance(force, InteraxDispersionForce):
+            _set(q, m1=p[0], m2=p[1], m3=p[2], alpha=p[3], width=p[4], alchemical=p[5])
+        elif isinstance(force, InteraxExchangeForce):
+            _set(q, amplitude=p[0], width=p[1], alchemical=p[2])
+        else:
+            _set(q, k=p[0], l=p[1], width=p[2], alchemical=p[3])
+    exclusions = ET.SubElement(node, "Exclusions")
+    for p1, p2 in force._exclusions:
+        _set(ET.SubElement(exclusions, "Exclusion"), p1=p1, p2=p2)
+    return node
+
+
+_FORCE_TYPES = ("InteraxDispersionForce", "InteraxExchangeForce", "InteraxShortRangeForce", "InteraxThreeBodyForce")
+
+
+def _deserialize_force(node):
+    """Interax{Dispersion,Exchange,ShortRange,ThreeBody}ForceProxy.cpp deserialize: the version check, then every
+    attribute the proxy writes ("forceGroup" defaults to 0 and "name" to the class name, as there)."""
+    kind = node.get("type")
+    if int(node.get("version")) != 1:
+        raise InteraxError("Unsupported version number")
+    if kind == "InteraxThreeBodyForce":
+        f = InteraxThreeBodyForce()
+        f.setForceGroup(int(node.get("forceGroup", 0)))
+        f.setName(node.get("name", f.getName()))
+        f.setCutoffDistance(float(node.get("cutoff")))
+        f.setScaleFactor(float(node.get("scaleFactor")))
+        f.setUsesPeriodicBoundaryConditions(_b(node, "usesPeriodic"))
+        f.setAlchemicalParameterName(node.get("alchemicalParameter"))
+        f.setDampingMethod(int(node.get("dampingMethod")))
+        f.setDampingScale(float(node.get("dampingScale")))
+        f.setCentroidDampingRadius(float(node.get("centroidDampingRadius")))
+        for site in node.find("Sites"):
+            f.addSite([int(p.get("index")) for p in site], float(site.get("alpha")), float(site.get("frequency")),
+                      float(site.get("width")), int(site.get("molecule")), _b(site, "alchemical"))
+        return f
+    f = {"InteraxDispersionForce": InteraxDispersionForce, "InteraxExchangeForce": InteraxExchangeForce,
+         "InteraxShortRangeForce": InteraxShortRangeForce}[kind]()
+    f.setForceGroup(int(node.get("forceGroup", 0)))
+    f.setName(node.get("name", f.getName()))
+    f.setNonbondedMethod(int(node.get("method")))
+    f.setCutoffDistance(float(node.get("cutoff")))
+    f.setUseSwitchingFunction(_b(node, "useSwitchingFunction"))
+    f.setSwitchingDistance(float(node.get("switchingDistance")))
+    f.setAlchemicalParameterName(node.get("alchemicalParameter"))
+    f.setSoftcoreRadius(float(node.get("softcoreRadius")))
+    if kind == "InteraxDispersionForce":
+        f.setDampingScale(float(node.get("dampingScale")))
+        f.setUseLongRangeCorrection(_b(node, "useLongRangeCorrection"))
+        for p in node.find("Particles"):
+            f.addParticle(float(p.get("m1")), float(p.get("m2")), float(p.get("m3")), float(p.get("alpha")),
+                          float(p.get("width")), _b(p, "alchemical"))
+    elif kind == "InteraxExchangeForce":
+        f.setOverlapMethod(int(node.get("overlapMethod")))
+        for p in node.find("Particles"):
+            f.addParticle(float(p.get("amplitude")), float(p.get("width")), _b(p, "alchemical"))
+    else:
+        f.setOverlapMethod(int(node.get("overlapMethod")))
+        for p in node.find("Particles"):
+            f.addParticle(float(p.get("k")), float(p.get("l")), float(p.get("width")), _b(p, "alchemical"))
+    for e in node.find("Exclusions"):
+        f.addExclusion(int(e.get("p1")), int(e.get("p2")))
+    return f
+
+
+def _serialize_system(system: "System") -> ET.Element:
+    """OpenMM's System node (version 1): box, particle masses, the Interax forces (other force types are not part of
+    this platform and are not written)."""
+    root = _set(ET.Element("System"), type="System", version=1)
+    box = ET.SubElement(root, "PeriodicBoxVectors")
+    b = system.getDefaultPeriodicBoxVectors()
+    for k, row in zip("ABC", b):
+        _set(ET.SubElement(box, k), x=float(row[0]), y=float(row[1]), z=float(row[2]))
+    particles = ET.SubElement(root, "Particles")
+    for m in system._masses:
+        _set(ET.SubElement(particles, "Particle"), mass=m)
+    ET.SubElement(root, "Constraints")
+    forces = ET.SubElement(root, "Forces")
+    for f in system.getForces():
+        forces.append(_serialize_force(f))
+    return root
+
+
+def _deserialize_system(root) -> "System":
+    box = root.find("PeriodicBoxVectors")
+    vec = [float(box.find(k).get(c)) for k in "ABC" for c in "xyz"] if box is not None else None
+    system = System(0, vec)
+    for p in root.find("Particles"):
+        system.addParticle(float(p.get("mass", 1.0)))
+    for node in root.find("Forces"):
+        if node.get("type") in _FORCE_TYPES:
+            system.addForce(_deserialize_force(node))
+    return system
+
+
+class XmlSerializer:
+    """OpenMM's XmlSerializer for the objects of this platform: a System or one of the four Interax forces.
+    serialize() writes the version-1 format of the reference's proxies (serialization/src/*Proxy.cpp); deserialize()
+    reads it back (and reads Systems written by OpenMM, ignoring other force types)."""
+
+    @staticmethod
+    def serialize(obj) -> str:
+        node = _serialize_system(obj) if isinstance(obj, System) else _serialize_force(obj)
+        ET.indent(node, space="\t")
+        return ET.tostring(node, encoding="unicode")+"\n"
+
+    @staticmethod
+    def deserialize(text: str):
+        root = ET.fromstring(text)
+        if root.tag == "System" or root.get("type") == "System":
+            return _deserialize_system(root)
+        if root.get("type") not in _FORCE_TYPES:
+            raise InteraxError(f"XmlSerializer: no serialization proxy for type {root.get('type')}")
+        return _deserialize_force(root)
+
+
 def load_system_xml(source) -> System:
     """Read a System written by OpenMM's XmlSerializer with the four Interax forces (others are ignored).
 
@@ -450,61 +620,14 @@ def load_system_xml(source) -> System:
             text = f.read()
     else:
         text = source
-    root = ET.fromstring(text)
-    box = root.find("PeriodicBoxVectors")
-    vec = [float(box.find(k).get(c)) for k in "ABC" for c in "xyz"] if box is not None else None
-    system = System(len(root.find("Particles")), vec)
-    for node in root.find("Forces"):
-        kind = node.get("type")
-        if kind not in ("InteraxDispersionForce", "InteraxExchangeForce", "InteraxShortRangeForce",
-                        "InteraxThreeBodyForce"):
-            continue
-        if int(node.get("version")) != 1:
-            raise InteraxError("Unsupported version number")
-        if kind == "InteraxThreeBodyForce":
-            # InteraxThreeBodyForceProxy.cpp deserialize
-            f = InteraxThreeBodyForce()
-            f.setForceGroup(int(node.get("forceGroup", 0)))
-            f.setCutoffDistance(float(node.get("cutoff")))
-            f.setScaleFactor(float(node.get("scaleFactor")))
-            f.setUsesPeriodicBoundaryConditions(_b(node, "usesPeriodic"))
-            f.setAlchemicalParameterName(node.get("alchemicalParameter"))
-            f.setDampingMethod(int(node.get("dampingMethod")))
-            f.setDampingScale(float(node.get("dampingScale")))
-            f.setCentroidDampingRadius(float(node.get("centroidDampingRadius")))
-            for site in node.find("Sites"):
-                f.addSite([int(p.get("index")) for p in site], float(site.get("alpha")), float(site.get("frequency")),
-                          float(site.get("width")), int(site.get("molecule")), _b(site, "alchemical"))
-            system.addForce(f)
-            continue
-        # Interax{Dispersion,Exchange,ShortRange}ForceProxy.cpp deserialize
-        f = {"InteraxDispersionForce": InteraxDispersionForce, "InteraxExchangeForce": InteraxExchangeForce,
-             "InteraxShortRangeForce": InteraxShortRangeForce}[kind]()
-        f.setForceGroup(int(node.get("forceGroup", 0)))
-        f.setNonbondedMethod(int(node.get("method")))
-        f.setCutoffDistance(float(node.get("cutoff")))
-        f.setUseSwitchingFunction(_b(node, "useSwitchingFunction"))
-        f.setSwitchingDistance(float(node.get("switchingDistance")))
-        f.setAlchemicalParameterName(node.get("alchemicalParameter"))
-        f.setSoftcoreRadius(float(node.get("softcoreRadius")))
-        if kind == "InteraxDispersionForce":
-            f.setDampingScale(float(node.get("dampingScale")))
-            f.setUseLongRangeCorrection(_b(node, "useLongRangeCorrection"))
-            for p in node.find("Particles"):
-                f.addParticle(float(p.get("m1")), float(p.get("m2")), float(p.get("m3")), float(p.get("alpha")),
-                              float(p.get("width")), _b(p, "alchemical"))
-        elif kind == "InteraxExchangeForce":
-            f.setOverlapMethod(int(node.get("overlapMethod")))
-            for p in node.find("Particles"):
-                f.addParticle(float(p.get("amplitude")), float(p.get("width")), _b(p, "alchemical"))
-        else:
-            f.setOverlapMethod(int(node.get("overlapMethod")))
-            for p in node.find("Particles"):
-                f.addParticle(float(p.get("k")), float(p.get("l")), float(p.get("width")), _b(p, "alchemical"))
-        for e in node.find("Exclusions"):
-            f.addExclusion(int(e.get("p1")), int(e.get("p2")))
-        system.addForce(f)
-    return system
+    return _deserialize_system(ET.fromstring(text))
+
+
+def save_system_xml(system: System, path: str) -> None:
+    """Write `system` with XmlSerializer.serialize (gzip-compressed when path ends in .gz)."""
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "wt") as f:
+        f.write(XmlSerializer.serialize(system))
 
 
 # ----------------------------------------------------------------------------------------------------------------
