@@ -173,14 +173,15 @@ __device__ __forceinline__ double wave_rotate_down(double v) {
 __device__ __forceinline__ long long to_fixed(double v) {
     return (long long) (v*FIXED_POINT_SCALE);
 }
-// FP32: the integer part is floor(v) and the fraction v - floor(v), both exact in FP64 (v has 24 significant
-// bits), so the result is floor(v 2^32) for |v| < 2^31 (to within one unit where the fraction of a tiny negative v
-// rounds), with six instructions instead of the eight of a general FP64-to-int64 conversion.
+// FP32, five instructions: the high word is floor(v) (v_cvt_flr_i32_f32, one instruction where floor and convert
+// take two), the low word the fraction v - floor(v), taken in FP64 (v_fract_f64), times 2^32.  The FP64 fraction of
+// a 24-bit v is exact unless -2^-29 < v < 0, where it is 1+v rounded to FP64 (kept below 1): the result is then
+// floor(v 2^32) to within one unit.  Valid for |v| < 2^31.  (The FP32 fraction would lose up to 2^8 units there.)
 __device__ __forceinline__ long long to_fixed(float v) {
-    const double d = (double) v;
-    const double hi = floor(d);
-    const unsigned lo = (unsigned) ((d-hi)*FIXED_POINT_SCALE);
-    return (long long) (((unsigned long long) (unsigned) (int) hi << 32) | lo);
+    int hi;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(v));
+    const unsigned lo = (unsigned) (__builtin_amdgcn_fract((double) v)*FIXED_POINT_SCALE);
+    return (long long) (((unsigned long long) (unsigned) hi << 32) | lo);
 }
 
 __device__ __forceinline__ void add_fixed(long long* p, long long v) {

@@ -469,35 +469,28 @@ __global__ void k_candidate_lists(const int2* tiles, const unsigned long long* m
         lists[start+count+lane] = 0xffff;
 }
 
-__global__ void k_set_int(int* p, int value) {
-    *p = value;
-}
-
-// Largest squared displacement since the list was built exceeds (skin/2)^2?
-__global__ void k_check_displacement(const void* posq, const float4* correction, int precision, int n, const double* buildPos,
-                                     int periodic, BoxD box, double limit2, int* flag) {
-    int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i >= n)
-        return;
-    double x, y, z;
-    load_position(posq, correction, precision, i, x, y, z);
-    double dx = x-buildPos[3*i], dy = y-buildPos[3*i+1], dz = z-buildPos[3*i+2];
-    if (periodic)
-        reference_min_image(dx, dy, dz, box);
-    if (dx*dx+dy*dy+dz*dz > limit2)
-        *flag = 1;
-}
-
 // ------------------------------------------------------------------------------------------------------------
 // Per-evaluation kernels
 // ------------------------------------------------------------------------------------------------------------
 
+// The neighbour-list check of a call, made in the same pass as its block frames (CHECK): flag is set if a particle
+// moved more than sqrt(limit2) since the last build, and the flag of the other frame is cleared for the next call.
+struct ListCheck {
+    const double* buildPos;   // particle order
+    double limit2;
+    int* flag;
+    int* otherFlag;
+};
+
 // Block frames: the centre of block b is the position of its first particle; every particle is stored relative to
-// the centre of its block (nearest image), in the arithmetic type.
-template <typename Real>
+// the centre of its block (nearest image), in the arithmetic type.  Every particle has exactly one slot, so with
+// CHECK every particle's displacement is tested once.
+template <typename Real, bool CHECK>
 __global__ void k_local_positions(const void* posq, const float4* correction, int precision, const int* perm, int padded,
-                                  int periodic, BoxD box, double* centers, typename Vec4T<Real>::type* local) {
+                                  int periodic, BoxD box, double* centers, typename Vec4T<Real>::type* local, ListCheck check) {
     int s = blockIdx.x*blockDim.x+threadIdx.x;
+    if (CHECK && s == 0)
+        *check.otherFlag = 0;
     if (s >= padded)
         return;
     int b = s/BLOCK;
@@ -518,6 +511,13 @@ __global__ void k_local_positions(const void* posq, const float4* correction, in
     }
     double x, y, z;
     load_position(posq, correction, precision, i, x, y, z);
+    if (CHECK) {
+        double ex = x-check.buildPos[3*i], ey = y-check.buildPos[3*i+1], ez = z-check.buildPos[3*i+2];
+        if (periodic)
+            reference_min_image(ex, ey, ez, box);
+        if (ex*ex+ey*ey+ez*ez > check.limit2)
+            *check.flag = 1;
+    }
     double dx = x-cx, dy = y-cy, dz = z-cz;
     if (periodic)
         reference_min_image(dx, dy, dz, box);
@@ -571,7 +571,7 @@ struct TileArgs {
     int precision;
     long long* force;
     int stride;
-    double* waveEnergy;   // per wave of the launch: energy and pair count of its tiles, in tile order
+    double* waveEnergy;   // per wave (all-pairs tiles) or per workgroup (cutoff tiles): energy and pair count, in tile order
     int* waveCount;
     double* tileEnergy;   // segmented handles: the energy of every tile, for the per-segment sums
     const int* tileIndex;  // subsets of segments: the tiles to evaluate (numTiles of them); nullptr: all tiles
@@ -675,28 +675,37 @@ __device__ __noinline__ bool exact_within_cutoff(const void* posq, const float4*
 }
 
 // Energy and force of one pair in range (ReferenceInteraxPairIxn.h:71-96): soft core, the pair terms, switching,
-// alchemical scaling.  d = r_j - r_i; returns the force on i as fs*d.
-template <typename Real, int TERMS, typename Particle>
+// alchemical scaling.  d = r_j - r_i; returns the force on i as fs*d.  ALCH: some particle of the handle is
+// alchemical; without, lambda is 1 and there is no soft core, and the variant leaves those steps out.
+template <typename Real, int TERMS, bool ALCH, typename Particle>
 __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const PairTables& tb, Real r2, const Particle& P, const Particle& Q,
                                               bool alchI, int flagJ, int origI, Real& e, Real& fs) {
-    const bool alchJ = ((flagJ >> 30) & 1) != 0;
-    const bool alchPair = (alchI != alchJ);
     const Real invR = fast_rsqrt(r2);
     const Real r = r2*invR;
     Real rEff = r, invREff = invR;
-    const bool softcore = a.useSoftcore && alchPair;
-    if (softcore) {
-        const Real x = r2*r2*r2+a.softcoreTerm;
-        if constexpr (std::is_same<Real, float>::value) {
-            // x^(1/6): v_log_f32/v_exp_f32 estimate, then one Newton step on y^6 = x (error a few ulp)
-            const float y = fast_exp2(__builtin_amdgcn_logf(x)*(1.0f/6.0f));
-            const float y2 = y*y;
-            rEff = y*(5.0f/6.0f) + x*fast_rcp(y2*y2*y)*(1.0f/6.0f);
-            invREff = fast_rcp(rEff);
-        }
-        else {
-            rEff = pow(x, Real(1.0/6.0));
-            invREff = Real(1)/rEff;
+    bool alchPair = false;
+    Real chain = 1;   // (r/r_eff)^5, the derivative of r_eff, on soft-core pairs
+    if constexpr (ALCH) {
+        const bool alchJ = ((flagJ >> 30) & 1) != 0;
+        alchPair = (alchI != alchJ);
+        if (a.useSoftcore && alchPair) {
+            const Real x = r2*r2*r2+a.softcoreTerm;
+            if constexpr (std::is_same<Real, float>::value) {
+                // x^(1/6): v_log_f32/v_exp_f32 estimate, then one Newton step on y^6 = x (error a few ulp)
+                const float y = fast_exp2(__builtin_amdgcn_logf(x)*(1.0f/6.0f));
+                const float y2 = y*y;
+                rEff = y*(5.0f/6.0f) + x*fast_rcp(y2*y2*y)*(1.0f/6.0f);
+                invREff = fast_rcp(rEff);
+            }
+            else {
+                rEff = pow(x, Real(1.0/6.0));
+                invREff = Real(1)/rEff;
+            }
+            if (rEff != r) {
+                const Real ratio = r*invREff;
+                const Real ratio2 = ratio*ratio;
+                chain = ratio2*ratio2*ratio;
+            }
         }
     }
     Real dEdR;
@@ -705,11 +714,8 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const Pai
         pair_energy_table<TERMS>(rEff, invREff, P, Q, iFirst, tb, e, dEdR);
     else
         pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
-    if (softcore && rEff != r) {
-        Real ratio = r*invREff;
-        Real ratio2 = ratio*ratio;
-        dEdR *= ratio2*ratio2*ratio;
-    }
+    if constexpr (ALCH)
+        dEdR *= chain;
     if (a.useSwitch && r > a.switchDistance) {
         Real x = (r-a.switchDistance)*a.invSwitchWidth;
         Real sw = 1+x*x*x*(Real(-10)+x*(Real(15)-x*Real(6)));
@@ -717,9 +723,13 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const Pai
         dEdR = dEdR*sw + e*dsw;
         e *= sw;
     }
-    const Real scale = (alchPair ? a.lambda : Real(1));
-    e *= scale;
-    fs = scale*dEdR*invR;
+    if constexpr (ALCH) {
+        const Real scale = (alchPair ? a.lambda : Real(1));
+        e *= scale;
+        fs = scale*dEdR*invR;
+    }
+    else
+        fs = dEdR*invR;
 }
 
 // The parameters of one slot, from global memory.
@@ -774,7 +784,7 @@ __device__ __forceinline__ void store_wave_totals(double* waveEnergy, int* waveC
 }
 
 // Tiles without a cutoff: every candidate interacts.  64 rotation steps, lane l paired with j = (l+s) % 64.
-template <typename Real, int TERMS, bool TABLES>
+template <typename Real, int TERMS, bool TABLES, bool ALCH>
 __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
     typedef typename ParticleType<Real, TABLES>::type Particle;
@@ -839,7 +849,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
                 const int flagJ = s.flags[jl];
                 fetch_particle(s, jl, Q);
                 Real e, fs;
-                evaluate_pair<Real, TERMS>(a, tb, r2, P, Q, alchI, flagJ, origI, e, fs);
+                evaluate_pair<Real, TERMS, ALCH>(a, tb, r2, P, Q, alchI, flagJ, origI, e, fs);
                 tileE += (double) e;
                 fi[0] += dx*fs;
                 fi[1] += dy*fs;
@@ -898,7 +908,7 @@ struct QueueLds {
     unsigned short queue[QUEUE];
 };
 
-template <typename Real, int TERMS, bool TABLES>
+template <typename Real, int TERMS, bool TABLES, bool ALCH>
 __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const PairTables& tb, QueueLds<Real, TABLES>& q, int head,
                                                int valid, int lane, double& energy) {
     typedef typename Vec4T<Real>::type R4;
@@ -918,7 +928,7 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
     fetch_particle(q.j, jl, Q);
     const int flagI = q.i.flags[il], flagJ = q.j.flags[jl];
     Real en, fs;
-    evaluate_pair<Real, TERMS>(a, tb, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
+    evaluate_pair<Real, TERMS, ALCH>(a, tb, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
     energy += (double) en;
     const long long fx = to_fixed(dx*fs), fy = to_fixed(dy*fs), fz = to_fixed(dz*fs);
     atomicAdd(&q.fI[0][il], (unsigned long long) fx);
@@ -997,7 +1007,7 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
 
 // With type tables the kernel is held to 5 waves per SIMD (at most 102 VGPRs; a few values spill): the pair batches
 // are latency-bound, and the extra wave hides more of it than the spills cost.
-template <typename Real, int TERMS, bool TABLES>
+template <typename Real, int TERMS, bool TABLES, bool ALCH>
 __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
     __shared__ QueueLds<Real, TABLES> lds[4];
@@ -1089,7 +1099,7 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
             }
             while (count >= BLOCK) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, BLOCK, lane, energy);
+                evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, BLOCK, lane, energy);
                 pairs += BLOCK;
                 head = (head+BLOCK) & (QUEUE-1);
                 count -= BLOCK;
@@ -1099,7 +1109,7 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
         }
         if (count > 0) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            evaluate_batch<Real, TERMS, TABLES>(a, tb, q, head, count, lane, energy);
+            evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, count, lane, energy);
             pairs += count;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1120,7 +1130,19 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
         add_fixed(&a.force[a.stride+oi], (long long) q.fI[1][lane]);
         add_fixed(&a.force[2*a.stride+oi], (long long) q.fI[2][lane]);
     }
-    store_wave_totals(a.waveEnergy, a.waveCount, wave, lane, energy, pairs, true);
+    // one energy and pair count per workgroup, its four waves summed in wave order
+    __shared__ double groupEnergy[4];
+    __shared__ int groupPairs[4];
+    const double waveTotal = wave_sum(energy);
+    if (lane == 0) {
+        groupEnergy[w] = waveTotal;
+        groupPairs[w] = pairs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.waveEnergy[blockIdx.x] = (groupEnergy[0]+groupEnergy[1])+(groupEnergy[2]+groupEnergy[3]);
+        a.waveCount[blockIdx.x] = (groupPairs[0]+groupPairs[1])+(groupPairs[2]+groupPairs[3]);
+    }
 }
 
 // Sum the per-wave energies and pair counts of a launch (one workgroup, a fixed assignment of partials to threads
@@ -1468,7 +1490,13 @@ struct ixh_pair {
     int numTiles = 0;
     int pairImage = 0;
     // parameters in particle order and in slot order
-    DeviceArray<char> paramA, paramB, paramC, sortedA, sortedB, sortedC, local;
+    DeviceArray<char> paramA, paramB, paramC, sortedA, sortedB, sortedC;
+    // block frames (local positions and block centres) of two consecutive calls: with a cutoff, a call's list check
+    // writes its frame while the tiles of the previous call still read the other one
+    DeviceArray<char> localFrame[2];
+    DeviceArray<double> centerFrame[2];
+    hipEvent_t frameDone[2] = {nullptr, nullptr};   // the last call that used each frame has finished with it
+    int frame = 0;
     DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionOffsets, exclusionPartners, blockStart, blockCount, blockList,
         partnerCount, partnerStart, partners, waveCount, tileSlots, tileCandidates, listOffsets;
     DeviceArray<unsigned short> lists;
@@ -1497,7 +1525,7 @@ struct ixh_pair {
     DeviceArray<float> tables;
     DeviceArray<unsigned> keys, keysOut;
     DeviceArray<int> values;
-    DeviceArray<double> pos3, buildPos, bounds, centers, waveEnergy, bbox;
+    DeviceArray<double> pos3, buildPos, bounds, waveEnergy, bbox;
     DeviceArray<int2> tiles;
     DeviceArray<unsigned long long> masks;
     DeviceArray<char> sortTemp;
@@ -2135,8 +2163,8 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
     a.masks = p->masks.ptr;
     a.listOffsets = p->listOffsets.ptr;
     a.lists = p->lists.ptr;
-    a.local = (const R4*) p->local.ptr;
-    a.centers = p->centers.ptr;
+    a.local = (const R4*) p->localFrame[p->frame].ptr;
+    a.centers = p->centerFrame[p->frame].ptr;
     a.pA = (const R4*) p->sortedA.ptr;
     a.pB = (const R4*) p->sortedB.ptr;
     a.pC = (const R4*) p->sortedC.ptr;
@@ -2198,28 +2226,35 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
         p->tileEnergy.reserve(std::max(p->numTiles, 1));
         a.tileEnergy = p->tileEnergy.ptr;
     }
-    const int numWaves = 4*blocks;
+    const bool compact = (p->method != IXH_NO_CUTOFF);
+    // partial sums of the launch: one per workgroup (cutoff tiles) or per wave (all-pairs tiles)
+    const int numWaves = (compact ? blocks : 4*blocks);
     p->waveEnergy.reserve(numWaves);
     p->waveCount.reserve(numWaves);
     a.waveEnergy = p->waveEnergy.ptr;
     a.waveCount = p->waveCount.ptr;
     if (a.numTiles > 0) {
         bool timed = p->timer.start(stream);
-        bool compact = (p->method != IXH_NO_CUTOFF);
         switch (terms) {
-#define IXH_LAUNCH(T, TAB) \
+#define IXH_LAUNCH(T, TAB, ALCH) \
             if (compact) \
-                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB>), dim3(blocks), dim3(256), tableBytes, stream, a); \
+                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a); \
             else \
-                hipLaunchKernelGGL((k_pair_rotate<Real, T, TAB>), dim3(blocks), dim3(256), tableBytes, stream, a);
+                hipLaunchKernelGGL((k_pair_rotate<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a);
+// the variants without alchemical steps are built for the FP32 type-table kernels only
 #define IXH_CASE(T) case T: \
             if constexpr (sizeof(Real) == 4) { \
                 if (tables) { \
-                    IXH_LAUNCH(T, true) \
+                    if (p->anyAlchemical) { \
+                        IXH_LAUNCH(T, true, true) \
+                    } \
+                    else { \
+                        IXH_LAUNCH(T, true, false) \
+                    } \
                     break; \
                 } \
             } \
-            IXH_LAUNCH(T, false) \
+            IXH_LAUNCH(T, false, true) \
             break;
             IXH_CASE(1) IXH_CASE(2) IXH_CASE(3) IXH_CASE(4) IXH_CASE(5) IXH_CASE(6) IXH_CASE(7)
 #undef IXH_CASE
@@ -2336,7 +2371,8 @@ int ixh_pair_create_segmented(int device, const ixh_pair_desc* d, int num_segmen
                 p->numBlocks = std::max(1, (p->n+BLOCK-1)/BLOCK);
             p->padded = p->numBlocks*BLOCK;
             hipStream_t stream = nullptr;
-            p->flagDev.reserve(1);
+            p->flagDev.reserve(2);   // one list-check flag per frame
+            IXH_HIP_CHECK(hipMemset(p->flagDev.ptr, 0, 2*sizeof(int)));
             p->pairCount.reserve(1);
             IXH_HIP_CHECK(hipHostMalloc((void**) &p->hostFlag, sizeof(int), hipHostMallocDefault));
             load_parameters(p, d, stream);
@@ -2366,6 +2402,9 @@ int ixh_pair_destroy(ixh_pair* p) {
             (void) hipEventDestroy(p->subsetCopied);
         if (p->flagCopied != nullptr)
             (void) hipEventDestroy(p->flagCopied);
+        for (hipEvent_t e : p->frameDone)
+            if (e != nullptr)
+                (void) hipEventDestroy(e);
         if (p->subsetStaging != nullptr)
             (void) hipHostFree(p->subsetStaging);
         if (p->lambdaHost != nullptr)
@@ -2428,18 +2467,35 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
         hipStream_t stream = (hipStream_t) b->stream;
         const float4* corr = (const float4*) b->posq_correction;
         bool rebuild = !p->built;
-        // The displacement check reads only this call's positions and the positions of the last build, so it runs
-        // before this call waits for the previous one: its answer is back while the previous tiles still run.
+        const int periodic = (p->method == IXH_CUTOFF_PERIODIC);
+        const size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
+        // The list check reads only this call's positions and the positions of the last build, so it runs before
+        // this call waits for the previous one: its answer is back while the previous tiles still run.  The same
+        // pass writes this call's block frames into the frame the previous call does not read (the call before it,
+        // which used this frame, has finished with it first).
         const bool deferredBefore = p->deferredCheck;
         p->deferredCheck = false;
+        bool framed = false;   // this call's block frames are written
         if (!rebuild && p->method != IXH_NO_CUTOFF) {
             if (deferredBefore && stream != p->lastStream)
                 IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->flagCopied, 0));   // the previous flag has been read back
-            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, stream, p->flagDev.ptr, 0);
-            hipLaunchKernelGGL(k_check_displacement, dim3(grid_for(p->n, 256)), dim3(256), 0, stream, b->posq, corr, b->precision, p->n,
-                               p->buildPos.ptr, (int) (p->method == IXH_CUTOFF_PERIODIC), make_box(b->box), 0.25*SKIN*SKIN, p->flagDev.ptr);
+            const int f = p->frame ^ 1;
+            if (p->frameDone[f] != nullptr)
+                IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[f], 0));
+            p->localFrame[f].reserve(r4*p->padded);
+            p->centerFrame[f].reserve(3*(size_t) p->numBlocks);
+            p->frame = f;
+            const ListCheck check = {p->buildPos.ptr, 0.25*SKIN*SKIN, p->flagDev.ptr+f, p->flagDev.ptr+(f ^ 1)};
+            if (p->precision == IXH_DOUBLE)
+                hipLaunchKernelGGL((k_local_positions<double, true>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
+                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[f].ptr,
+                                   (double4*) p->localFrame[f].ptr, check);
+            else
+                hipLaunchKernelGGL((k_local_positions<float, true>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
+                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[f].ptr,
+                                   (float4*) p->localFrame[f].ptr, check);
             IXH_HIP_CHECK(hipGetLastError());
-            IXH_HIP_CHECK(hipMemcpyAsync(p->hostFlag, p->flagDev.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+            IXH_HIP_CHECK(hipMemcpyAsync(p->hostFlag, p->flagDev.ptr+f, sizeof(int), hipMemcpyDeviceToHost, stream));
             if (flags & IXH_DEFER_LIST_CHECK) {
                 if (p->flagCopied == nullptr)
                     IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
@@ -2450,6 +2506,7 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
                 IXH_HIP_CHECK(hipStreamSynchronize(stream));
                 rebuild = (*p->hostFlag != 0);
             }
+            framed = !rebuild;
         }
         // the handle's buffers are reused by every call: a call on another stream waits for the previous one
         if (p->done == nullptr)
@@ -2459,10 +2516,10 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
         p->lastStream = stream;
         if (rebuild)
             build_list(p, b, stream);
-        size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
-        p->local.reserve(r4*p->padded);
-        p->centers.reserve(3*(size_t) p->numBlocks);
-        int periodic = (p->method == IXH_CUTOFF_PERIODIC);
+        // without a list check (no cutoff, a new list) the frames are written after the previous call, which may have
+        // used either frame, has finished
+        p->localFrame[p->frame].reserve(r4*p->padded);
+        p->centerFrame[p->frame].reserve(3*(size_t) p->numBlocks);
         int subsetTileCount = -1;
         if (numSelected >= 0) {
             // the tiles of the selected segments, in segment order
@@ -2501,20 +2558,28 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
             IXH_HIP_CHECK(hipEventRecord(p->subsetCopied, stream));
             subsetTileCount = count;
         }
+        const ListCheck none = {};
         if (p->precision == IXH_DOUBLE) {
-            hipLaunchKernelGGL(k_local_positions<double>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
-                               p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (double4*) p->local.ptr);
+            if (!framed)
+                hipLaunchKernelGGL((k_local_positions<double, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
+                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
+                                   (double4*) p->localFrame[p->frame].ptr, none);
             IXH_HIP_CHECK(hipGetLastError());
             launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, subsetTileCount,
                                  std::max(numSelected, 0));
         }
         else {
-            hipLaunchKernelGGL(k_local_positions<float>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq, corr, b->precision,
-                               p->perm.ptr, p->padded, periodic, make_box(b->box), p->centers.ptr, (float4*) p->local.ptr);
+            if (!framed)
+                hipLaunchKernelGGL((k_local_positions<float, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
+                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
+                                   (float4*) p->localFrame[p->frame].ptr, none);
             IXH_HIP_CHECK(hipGetLastError());
             launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, subsetTileCount,
                                 std::max(numSelected, 0));
         }
+        if (p->frameDone[p->frame] == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->frameDone[p->frame], hipEventDisableTiming));
+        IXH_HIP_CHECK(hipEventRecord(p->frameDone[p->frame], stream));
         IXH_HIP_CHECK(hipEventRecord(p->done, stream));
     });
 }
