@@ -115,6 +115,11 @@ int ixh_state_set_positions(ixh_state* state, const double* positions);
 int ixh_state_set_box(ixh_state* state, const double* box);
 /* Set the value of the alchemical context parameter used by the next calls. */
 int ixh_state_set_lambda(ixh_state* state, double lambda);
+/* Run the state's operations, and hand out in ixh_buffers, the given stream (hipStream_t; NULL: the state's own) in
+   place of the state's own, as a platform runs all kernels of a Context on one stream (OpenMM's
+   CudaContext::getCurrentStream).  Waits for the work queued on the previous stream.  The stream must outlive its
+   use by the state. */
+int ixh_state_set_stream(ixh_state* state, void* stream);
 /* Zero the force buffer and the energy accumulator (asynchronous, on the state's stream). */
 int ixh_state_clear(ixh_state* state);
 /* Wait for the state's stream and download the forces (3 doubles per particle) / the energy. */

@@ -1495,7 +1495,8 @@ struct ixh_pair {
     // writes its frame while the tiles of the previous call still read the other one
     DeviceArray<char> localFrame[2];
     DeviceArray<double> centerFrame[2];
-    hipEvent_t frameDone[2] = {nullptr, nullptr};   // the last call that used each frame has finished with it
+    hipEvent_t frameDone[2] = {nullptr, nullptr};   // the end of the last call that used each frame
+    hipStream_t frameStream[2] = {nullptr, nullptr};   // the stream of that call
     int frame = 0;
     DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionOffsets, exclusionPartners, blockStart, blockCount, blockList,
         partnerCount, partnerStart, partners, waveCount, tileSlots, tileCandidates, listOffsets;
@@ -1529,7 +1530,6 @@ struct ixh_pair {
     DeviceArray<int2> tiles;
     DeviceArray<unsigned long long> masks;
     DeviceArray<char> sortTemp;
-    DeviceArray<int> flagDev;
     DeviceArray<long long> pairCount;
     // energies at several lambda values: generic FP32 parameters in particle order, the alchemical particles, and the
     // scratch buffers of the handle's own evaluation
@@ -1541,11 +1541,12 @@ struct ixh_pair {
     DeviceArray<long long> lambdaForce;
     double* lambdaHost = nullptr;   // pinned
     size_t lambdaHostCount = 0;
+    // list-check flags of the two frames, in coherent pinned host memory written by the check kernel itself
     int* hostFlag = nullptr;
-    hipEvent_t flagCopied = nullptr;   // deferred list checks: the flag has reached hostFlag
+    int checkedFrame = 0;              // the frame of the last list check
+    hipEvent_t flagCopied = nullptr;   // deferred list checks: the check has finished and its flag is in hostFlag
     bool deferredCheck = false;
     hipStream_t lastStream = nullptr;
-    hipEvent_t done = nullptr;   // recorded at the end of every execute: calls on other streams wait for it
     KernelTimer timer;
 };
 
@@ -2151,9 +2152,22 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     p->built = true;
 }
 
+// Launch a kernel with events bound to its start and completion (hipExtLaunchKernelGGL: the dispatch itself carries
+// them, no marker packet enters the queue), or plainly when there are none (also the form a graph capture records).
+template <typename Kernel, typename... Args>
+void launch_bound(Kernel kernel, dim3 grid, dim3 block, size_t shared, hipStream_t stream, hipEvent_t start, hipEvent_t end,
+                  Args... args) {
+    if (start != nullptr || end != nullptr)
+        hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t) shared, stream, start, end, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, shared, stream, args...);
+}
+
+// The tile kernel of an evaluation and its reductions.  end: an event to bind to the last kernel (nullptr: none).
+// Returns the event that marks the end of the work (end), or nullptr if none was bound.
 template <typename Real>
-void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, double* segmentEnergies, hipStream_t stream,
-                  int subsetTiles = -1, int subsetSegments = 0) {
+hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, double* segmentEnergies, hipStream_t stream,
+                        hipEvent_t end, int subsetTiles = -1, int subsetSegments = 0) {
     typedef typename Vec4T<Real>::type R4;
     TileArgs<Real> a;
     a.tiles = p->tiles.ptr;
@@ -2233,14 +2247,19 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
     p->waveCount.reserve(numWaves);
     a.waveEnergy = p->waveEnergy.ptr;
     a.waveCount = p->waveCount.ptr;
+    const bool segments = (segmentEnergies != nullptr && p->numSegments > 0);
+    const int segmentCount = (!segments ? 0 : subsetTiles >= 0 ? subsetSegments : p->numSegments);
+    hipEvent_t marked = nullptr;
     if (a.numTiles > 0) {
-        bool timed = p->timer.start(stream);
+        // with profiling, the kernel's own dispatch carries the timing events
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        p->timer.bound(t0, t1);
         switch (terms) {
 #define IXH_LAUNCH(T, TAB, ALCH) \
             if (compact) \
-                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a); \
+                launch_bound(k_pair_queue<Real, T, TAB, ALCH>, dim3(blocks), dim3(256), tableBytes, stream, t0, t1, a); \
             else \
-                hipLaunchKernelGGL((k_pair_rotate<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a);
+                launch_bound(k_pair_rotate<Real, T, TAB, ALCH>, dim3(blocks), dim3(256), tableBytes, stream, t0, t1, a);
 // the variants without alchemical steps are built for the FP32 type-table kernels only
 #define IXH_CASE(T) case T: \
             if constexpr (sizeof(Real) == 4) { \
@@ -2263,24 +2282,20 @@ void launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, d
                 throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: illegal terms");
         }
         IXH_HIP_CHECK(hipGetLastError());
-        if (timed)
-            p->timer.stop(stream);
-        hipLaunchKernelGGL(k_reduce_waves, dim3(1), dim3(1024), 0, stream, p->waveEnergy.ptr, p->waveCount.ptr, numWaves, b->energy,
-                           p->pairCount.ptr, addEnergy);
     }
-    else
-        hipLaunchKernelGGL(k_reduce_waves, dim3(1), dim3(1024), 0, stream, p->waveEnergy.ptr, p->waveCount.ptr, 0, b->energy,
-                           p->pairCount.ptr, addEnergy);
+    launch_bound(k_reduce_waves, dim3(1), dim3(1024), 0, stream, nullptr, segmentCount > 0 ? nullptr : end,
+                 (const double*) p->waveEnergy.ptr, (const int*) p->waveCount.ptr, a.numTiles > 0 ? numWaves : 0, b->energy,
+                 p->pairCount.ptr, addEnergy);
     IXH_HIP_CHECK(hipGetLastError());
-    if (segmentEnergies != nullptr && p->numSegments > 0) {
-        const bool subset = (subsetTiles >= 0);
-        const int count = (subset ? subsetSegments : p->numSegments);
-        if (count > 0) {
-            hipLaunchKernelGGL(k_reduce_segments, dim3(count), dim3(256), 0, stream, p->tileEnergy.ptr, p->segTilesDev.ptr,
-                               subset ? (const int*) p->subsetSegments.ptr : (const int*) nullptr, segmentEnergies);
-            IXH_HIP_CHECK(hipGetLastError());
-        }
+    marked = end;
+    if (segmentCount > 0) {
+        launch_bound(k_reduce_segments, dim3(segmentCount), dim3(256), 0, stream, nullptr, end, (const double*) p->tileEnergy.ptr,
+                     (const int*) p->segTilesDev.ptr, subsetTiles >= 0 ? (const int*) p->subsetSegments.ptr : (const int*) nullptr,
+                     segmentEnergies);
+        IXH_HIP_CHECK(hipGetLastError());
+        marked = end;
     }
+    return marked;
 }
 
 }  // namespace
@@ -2371,10 +2386,9 @@ int ixh_pair_create_segmented(int device, const ixh_pair_desc* d, int num_segmen
                 p->numBlocks = std::max(1, (p->n+BLOCK-1)/BLOCK);
             p->padded = p->numBlocks*BLOCK;
             hipStream_t stream = nullptr;
-            p->flagDev.reserve(2);   // one list-check flag per frame
-            IXH_HIP_CHECK(hipMemset(p->flagDev.ptr, 0, 2*sizeof(int)));
             p->pairCount.reserve(1);
-            IXH_HIP_CHECK(hipHostMalloc((void**) &p->hostFlag, sizeof(int), hipHostMallocDefault));
+            IXH_HIP_CHECK(hipHostMalloc((void**) &p->hostFlag, 2*sizeof(int), hipHostMallocCoherent));
+            p->hostFlag[0] = p->hostFlag[1] = 0;
             load_parameters(p, d, stream);
             compute_long_range(p, stream);
         }
@@ -2396,8 +2410,6 @@ int ixh_pair_destroy(ixh_pair* p) {
         (void) hipDeviceSynchronize();
         if (p->hostFlag != nullptr)
             (void) hipHostFree(p->hostFlag);
-        if (p->done != nullptr)
-            (void) hipEventDestroy(p->done);
         if (p->subsetCopied != nullptr)
             (void) hipEventDestroy(p->subsetCopied);
         if (p->flagCopied != nullptr)
@@ -2475,44 +2487,45 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
         // which used this frame, has finished with it first).
         const bool deferredBefore = p->deferredCheck;
         p->deferredCheck = false;
+        const int previousFrame = p->frame;   // frameDone[previousFrame] marks the end of the previous call
         bool framed = false;   // this call's block frames are written
         if (!rebuild && p->method != IXH_NO_CUTOFF) {
+            // the previous check, whose flag this one clears, has finished
             if (deferredBefore && stream != p->lastStream)
-                IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->flagCopied, 0));   // the previous flag has been read back
+                IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->flagCopied, 0));
             const int f = p->frame ^ 1;
-            if (p->frameDone[f] != nullptr)
+            if (p->frameDone[f] != nullptr && p->frameStream[f] != stream)
                 IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[f], 0));
             p->localFrame[f].reserve(r4*p->padded);
             p->centerFrame[f].reserve(3*(size_t) p->numBlocks);
             p->frame = f;
-            const ListCheck check = {p->buildPos.ptr, 0.25*SKIN*SKIN, p->flagDev.ptr+f, p->flagDev.ptr+(f ^ 1)};
+            const ListCheck check = {p->buildPos.ptr, 0.25*SKIN*SKIN, p->hostFlag+f, p->hostFlag+(f ^ 1)};
+            p->checkedFrame = f;
+            // a deferred check carries its completion event on its own dispatch
+            const bool defer = (flags & IXH_DEFER_LIST_CHECK) != 0;
+            if (defer && p->flagCopied == nullptr)
+                IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
+            hipEvent_t checked = (defer ? p->flagCopied : nullptr);
             if (p->precision == IXH_DOUBLE)
-                hipLaunchKernelGGL((k_local_positions<double, true>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
-                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[f].ptr,
-                                   (double4*) p->localFrame[f].ptr, check);
+                launch_bound(k_local_positions<double, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
+                             b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
+                             p->centerFrame[f].ptr, (double4*) p->localFrame[f].ptr, check);
             else
-                hipLaunchKernelGGL((k_local_positions<float, true>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
-                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[f].ptr,
-                                   (float4*) p->localFrame[f].ptr, check);
+                launch_bound(k_local_positions<float, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
+                             b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
+                             p->centerFrame[f].ptr, (float4*) p->localFrame[f].ptr, check);
             IXH_HIP_CHECK(hipGetLastError());
-            IXH_HIP_CHECK(hipMemcpyAsync(p->hostFlag, p->flagDev.ptr+f, sizeof(int), hipMemcpyDeviceToHost, stream));
-            if (flags & IXH_DEFER_LIST_CHECK) {
-                if (p->flagCopied == nullptr)
-                    IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
-                IXH_HIP_CHECK(hipEventRecord(p->flagCopied, stream));
+            if (defer)
                 p->deferredCheck = true;
-            }
             else {
                 IXH_HIP_CHECK(hipStreamSynchronize(stream));
-                rebuild = (*p->hostFlag != 0);
+                rebuild = (p->hostFlag[f] != 0);
             }
             framed = !rebuild;
         }
         // the handle's buffers are reused by every call: a call on another stream waits for the previous one
-        if (p->done == nullptr)
-            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
-        else if (stream != p->lastStream)
-            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->done, 0));
+        if (p->frameDone[previousFrame] != nullptr && stream != p->lastStream)
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[previousFrame], 0));
         p->lastStream = stream;
         if (rebuild)
             build_list(p, b, stream);
@@ -2559,14 +2572,21 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
             subsetTileCount = count;
         }
         const ListCheck none = {};
+        // the end of the call: an event bound to its last kernel, or recorded after it in a graph capture
+        if (p->frameDone[p->frame] == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->frameDone[p->frame], hipEventDisableTiming));
+        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+        IXH_HIP_CHECK(hipStreamIsCapturing(stream, &capture));
+        hipEvent_t end = (capture == hipStreamCaptureStatusNone ? p->frameDone[p->frame] : nullptr);
+        hipEvent_t marked = nullptr;
         if (p->precision == IXH_DOUBLE) {
             if (!framed)
                 hipLaunchKernelGGL((k_local_positions<double, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
                                    corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
                                    (double4*) p->localFrame[p->frame].ptr, none);
             IXH_HIP_CHECK(hipGetLastError());
-            launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, subsetTileCount,
-                                 std::max(numSelected, 0));
+            marked = launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
+                                          subsetTileCount, std::max(numSelected, 0));
         }
         else {
             if (!framed)
@@ -2574,13 +2594,12 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
                                    corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
                                    (float4*) p->localFrame[p->frame].ptr, none);
             IXH_HIP_CHECK(hipGetLastError());
-            launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, subsetTileCount,
-                                std::max(numSelected, 0));
+            marked = launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
+                                         subsetTileCount, std::max(numSelected, 0));
         }
-        if (p->frameDone[p->frame] == nullptr)
-            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->frameDone[p->frame], hipEventDisableTiming));
-        IXH_HIP_CHECK(hipEventRecord(p->frameDone[p->frame], stream));
-        IXH_HIP_CHECK(hipEventRecord(p->done, stream));
+        if (marked == nullptr)
+            IXH_HIP_CHECK(hipEventRecord(p->frameDone[p->frame], stream));
+        p->frameStream[p->frame] = stream;
     });
 }
 
@@ -2762,7 +2781,7 @@ int ixh_pair_list_was_valid(ixh_pair* p, int* valid) {
         if (p->deferredCheck) {
             DeviceGuard guard(p->device);
             IXH_HIP_CHECK(hipEventSynchronize(p->flagCopied));
-            *valid = (*p->hostFlag == 0);
+            *valid = (p->hostFlag[p->checkedFrame] == 0);
         }
     });
 }

@@ -43,7 +43,8 @@ struct ixh_state {
     int numParticles = 0;
     int precision = IXH_MIXED;
     int stride = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream of every operation: the state's own, or one set by the caller
+    hipStream_t ownStream = nullptr;
     DeviceArray<char> posq;
     DeviceArray<float4> correction;
     DeviceArray<long long> force;
@@ -97,7 +98,8 @@ int ixh_state_create(int device, int numParticles, int precision, ixh_state** ou
             s->numParticles = numParticles;
             s->precision = precision;
             s->stride = ((numParticles+63)/64)*64;
-            IXH_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+            IXH_HIP_CHECK(hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking));
+            s->stream = s->ownStream;
             size_t posBytes = (precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
             s->posq.reserve(posBytes*(size_t) (numParticles > 0 ? numParticles : 1));
             IXH_HIP_CHECK(hipMemset(s->posq.ptr, 0, s->posq.count));
@@ -111,8 +113,8 @@ int ixh_state_create(int device, int numParticles, int precision, ixh_state** ou
             IXH_HIP_CHECK(hipMemset(s->energy.ptr, 0, sizeof(double)));
         }
         catch (...) {
-            if (s->stream != nullptr)
-                (void) hipStreamDestroy(s->stream);
+            if (s->ownStream != nullptr)
+                (void) hipStreamDestroy(s->ownStream);
             delete s;
             throw;
         }
@@ -125,9 +127,11 @@ int ixh_state_destroy(ixh_state* s) {
         if (s == nullptr)
             return;
         DeviceGuard guard(s->device);
-        if (s->stream != nullptr) {
+        if (s->stream != s->ownStream)
             (void) hipStreamSynchronize(s->stream);
-            (void) hipStreamDestroy(s->stream);
+        if (s->ownStream != nullptr) {
+            (void) hipStreamSynchronize(s->ownStream);
+            (void) hipStreamDestroy(s->ownStream);
         }
         delete s;
     });
@@ -182,6 +186,16 @@ int ixh_state_set_lambda(ixh_state* s, double lambda) {
         if (s == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_state_set_lambda: invalid argument");
         s->lambda = lambda;
+    });
+}
+
+int ixh_state_set_stream(ixh_state* s, void* stream) {
+    return guarded([&] {
+        if (s == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_state_set_stream: invalid argument");
+        DeviceGuard guard(s->device);
+        IXH_HIP_CHECK(hipStreamSynchronize(s->stream));
+        s->stream = (stream != nullptr ? (hipStream_t) stream : s->ownStream);
     });
 }
 

@@ -88,6 +88,7 @@ SIGNATURES = {
     "ixh_state_set_box": [ctypes.c_void_p, _dp],
     "ixh_state_set_lambda": [ctypes.c_void_p, ctypes.c_double],
     "ixh_state_clear": [ctypes.c_void_p],
+    "ixh_state_set_stream": [ctypes.c_void_p, ctypes.c_void_p],
     "ixh_state_get_forces": [ctypes.c_void_p, _dp],
     "ixh_state_get_energy": [ctypes.c_void_p, _dp],
     "ixh_state_synchronize": [ctypes.c_void_p],
