@@ -25,7 +25,7 @@ import numpy as np
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (packed), TFLOP/s
 PEAK_HBM_GBS = 8000.0
 FLOPS_PER_PAIR = 240         # fused dispersion + exchange + short range, FMA = 2
-TIMER_STRIDE = 10            # pair kernel timed with bound HIP events on every 10th step of the timed loop
+TIMER_STRIDE = 10            # pair kernel timed with HIP events on every 10th step of the timed loop
 FLOPS_PER_TRIPLE = 130       # undamped ATM triple
 
 
@@ -153,9 +153,9 @@ def bench_pair(args):
         ih._check(lib.ixh_state_synchronize(frames.states[k]))
     ih._check(lib.ixh_pair_reset_kernel_time(group.handle))
     reruns[0] = 0
-    # The pair kernel's duration comes from HIP events bound to its own dispatch (hipExtLaunchKernelGGL) on every
-    # TIMER_STRIDE-th step of the timed loop: an event-carrying dispatch ends with a system-scope release that leaves
-    # the queue idle for ~5 us, which timing every step would add to the throughput being measured.
+    # The pair kernel's duration comes from HIP events recorded around it on every TIMER_STRIDE-th step of the timed
+    # loop: each recorded event leaves the queue idle for a few microseconds, which timing every step would add to
+    # the throughput being measured.
     start = time.perf_counter()
     for k in range(args.warmup, total):
         timed = (k-args.warmup) % TIMER_STRIDE == 0
@@ -196,7 +196,7 @@ def bench_pair(args):
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved/PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profiles("k_pair_queue"), "kernel": "k_pair_queue",
                      "kernel_avg_us": avg_kernel_s*1e6, "launches": launches.value,
-                     "timed_launches": f"HIP events bound to every {TIMER_STRIDE}th launch of the timed loop"},
+                     "timed_launches": f"HIP events recorded around every {TIMER_STRIDE}th launch of the timed loop"},
     }
 
     # accuracy of the first timed step against the FP64 oracle (long range correction excluded: it is a host

@@ -125,19 +125,6 @@ struct KernelTimer {
         IXH_HIP_CHECK(hipEventRecord(next(), stream));
         launches++;
     }
-    // A start/stop pair for one kernel launched with hipExtLaunchKernelGGL, which binds the events to the kernel's
-    // own dispatch: no marker packets (each of which leaves the queue idle for microseconds) between the kernels of
-    // a stream.  Returns false when profiling is off.
-    bool bound(hipEvent_t& start, hipEvent_t& stop) {
-        if (!profiling_enabled())
-            return false;
-        if (used+2 > 4096)
-            collect();
-        start = next();
-        stop = next();
-        launches++;
-        return true;
-    }
     void collect() {
         for (size_t k = 0; k+1 < used; k += 2) {
             IXH_HIP_CHECK(hipEventSynchronize(pool[k+1]));

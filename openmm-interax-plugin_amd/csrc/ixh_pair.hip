@@ -2251,15 +2251,15 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
     const int segmentCount = (!segments ? 0 : subsetTiles >= 0 ? subsetSegments : p->numSegments);
     hipEvent_t marked = nullptr;
     if (a.numTiles > 0) {
-        // with profiling, the kernel's own dispatch carries the timing events
-        hipEvent_t t0 = nullptr, t1 = nullptr;
-        p->timer.bound(t0, t1);
+        // with profiling, events recorded around the kernel (events bound to its dispatch would also count the wait
+        // for the previous kernel)
+        const bool timed = p->timer.start(stream);
         switch (terms) {
 #define IXH_LAUNCH(T, TAB, ALCH) \
             if (compact) \
-                launch_bound(k_pair_queue<Real, T, TAB, ALCH>, dim3(blocks), dim3(256), tableBytes, stream, t0, t1, a); \
+                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a); \
             else \
-                launch_bound(k_pair_rotate<Real, T, TAB, ALCH>, dim3(blocks), dim3(256), tableBytes, stream, t0, t1, a);
+                hipLaunchKernelGGL((k_pair_rotate<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a);
 // the variants without alchemical steps are built for the FP32 type-table kernels only
 #define IXH_CASE(T) case T: \
             if constexpr (sizeof(Real) == 4) { \
@@ -2282,6 +2282,8 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
                 throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: illegal terms");
         }
         IXH_HIP_CHECK(hipGetLastError());
+        if (timed)
+            p->timer.stop(stream);
     }
     launch_bound(k_reduce_waves, dim3(1), dim3(1024), 0, stream, nullptr, segmentCount > 0 ? nullptr : end,
                  (const double*) p->waveEnergy.ptr, (const int*) p->waveCount.ptr, a.numTiles > 0 ? numWaves : 0, b->energy,
