@@ -444,6 +444,11 @@ __global__ void k_build_masks(MaskArgs a, unsigned long long* masks, int* tileSl
 // consecutive entries have distinct i and distinct j.  Within the diagonal tile s runs over 1..32 (s = 32 for
 // i < 32 only), which meets every unordered pair once.  Entries are i << 6 | j; each tile's list is padded with
 // 0xffff to a multiple of 4 entries.  With offsets == nullptr only the counts (before padding) are written.
+// The candidate pairs of a tile in rotation order (pairs (i, i+s mod 64) for s = 1, 2, ...), padded to a multiple
+// of 4 with 0xffff.  The tile kernel reads a list 256 entries per step, 4 consecutive u16 per lane, and takes them
+// in 4 sub-steps; each chunk of 256 (or the last, shorter one, of 4m entries) is stored transposed so that sub-step
+// s of lane l gets entry s m + l: in a sub-step consecutive lanes hold consecutive entries of the rotation order,
+// whose particles lie in distinct LDS banks.  Without offsets only the counts are written.
 __global__ void k_candidate_lists(const int2* tiles, const unsigned long long* masks, int* counts, const int* offsets,
                                   unsigned short* lists) {
     const int t = blockIdx.x;
@@ -451,22 +456,36 @@ __global__ void k_candidate_lists(const int2* tiles, const unsigned long long* m
     const bool diag = (tiles[t].y < 0);
     const unsigned long long m = masks[(size_t) t*BLOCK+lane];
     const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
-    const int start = (offsets != nullptr ? offsets[t] : 0);
+    int total = 0;
+    for (int s = sBegin; s < sEnd; s++) {
+        const int j = (lane+s) & 63;
+        const bool in = ((m >> j) & 1ULL) != 0 && !(diag && s == 32 && lane >= 32);
+        total += __popcll(__ballot(in));
+    }
+    if (offsets == nullptr) {
+        if (lane == 0)
+            counts[t] = total;
+        return;
+    }
+    const int start = offsets[t];
+    const int padded = (total+3) & ~3;
+    // entry k of the list -> its position
+    auto position = [padded](int k) {
+        const int chunk = k & ~255, kk = k & 255;
+        const int rows = min(256, padded-chunk)/4;   // lanes that read this chunk
+        return chunk+(kk % rows)*4+kk/rows;
+    };
     int count = 0;
     for (int s = sBegin; s < sEnd; s++) {
         const int j = (lane+s) & 63;
         const bool in = ((m >> j) & 1ULL) != 0 && !(diag && s == 32 && lane >= 32);
         const unsigned long long ballot = __ballot(in);
-        if (offsets != nullptr && in)
-            lists[start+count+__popcll(ballot & ((1ULL << lane)-1))] = (unsigned short) ((lane << 6) | j);
+        if (in)
+            lists[start+position(count+__popcll(ballot & ((1ULL << lane)-1)))] = (unsigned short) ((lane << 6) | j);
         count += __popcll(ballot);
     }
-    if (offsets == nullptr) {
-        if (lane == 0)
-            counts[t] = count;
-    }
-    else if (lane < ((count+3) & ~3)-count)
-        lists[start+count+lane] = 0xffff;
+    if (lane < padded-total)
+        lists[start+position(total+lane)] = 0xffff;
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -1059,7 +1078,8 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
         store_partner<Real, TABLES>(a, f, I, cIx, cIy, cIz, lane, q.j);
         q.fJ[0][lane] = q.fJ[1][lane] = q.fJ[2][lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        // The tile's candidate list, 256 entries (4 per lane) per step with the next step's entries in flight: each
+        // The tile's candidate list, 256 entries (4 per lane) per step with the next step's entries in flight (each
+        // chunk stored transposed: sub-step s of lane l is entry s m + l of the chunk's rotation order): each
         // entry in range is pushed into the queue, then every full batch of 64 queued pairs is evaluated.  The four
         // filter sub-steps are unrolled and hold no batch evaluation, so no loop inside a step makes the compiler
         // wait for the next step's entries before the filter starts.
