@@ -1025,11 +1025,14 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
 }
 
 // With type tables the kernel is held to 5 waves per SIMD (at most 102 VGPRs; a few values spill): the pair batches
-// are latency-bound, and the extra wave hides more of it than the spills cost.
+// are latency-bound, and the extra wave hides more of it than the spills cost.  A workgroup is QUEUE_GROUP_WAVES
+// wavefronts (measured on the 98 304-atom box: 1 wave, 185.5 us, but four times the partial sums to reduce, 0.209 ms
+// per step; 2 waves, 207 us; 4 waves, 186-190 us and 0.205 ms per step).
+constexpr int QUEUE_GROUP_WAVES = 4;
 template <typename Real, int TERMS, bool TABLES, bool ALCH>
-__global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Real> a) {
+__global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_queue(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
-    __shared__ QueueLds<Real, TABLES> lds[4];
+    __shared__ QueueLds<Real, TABLES> lds[QUEUE_GROUP_WAVES];
     extern __shared__ float tableLds[];   // the type tables, sized at launch
     PairTables tb = {};
     if constexpr (TABLES)
@@ -1037,8 +1040,8 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     QueueLds<Real, TABLES>& q = lds[w];
-    const long long wave = (long long) blockIdx.x*4+w;
-    const long long numWaves = (long long) gridDim.x*4;
+    const long long wave = (long long) blockIdx.x*QUEUE_GROUP_WAVES+w;
+    const long long numWaves = (long long) gridDim.x*QUEUE_GROUP_WAVES;
     const long long T = a.numTiles;
     const long long begin = T*wave/numWaves, end = T*(wave+1)/numWaves;
     const Real margin = a.band*a.cutoff2;
@@ -1150,9 +1153,9 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
         add_fixed(&a.force[a.stride+oi], (long long) q.fI[1][lane]);
         add_fixed(&a.force[2*a.stride+oi], (long long) q.fI[2][lane]);
     }
-    // one energy and pair count per workgroup, its four waves summed in wave order
-    __shared__ double groupEnergy[4];
-    __shared__ int groupPairs[4];
+    // one energy and pair count per workgroup, its waves summed in wave order
+    __shared__ double groupEnergy[QUEUE_GROUP_WAVES];
+    __shared__ int groupPairs[QUEUE_GROUP_WAVES];
     const double waveTotal = wave_sum(energy);
     if (lane == 0) {
         groupEnergy[w] = waveTotal;
@@ -1160,8 +1163,14 @@ __global__ void __launch_bounds__(256, TABLES ? 5 : 1) k_pair_queue(TileArgs<Rea
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        a.waveEnergy[blockIdx.x] = (groupEnergy[0]+groupEnergy[1])+(groupEnergy[2]+groupEnergy[3]);
-        a.waveCount[blockIdx.x] = (groupPairs[0]+groupPairs[1])+(groupPairs[2]+groupPairs[3]);
+        double e = 0;
+        int c = 0;
+        for (int k = 0; k < QUEUE_GROUP_WAVES; k++) {
+            e += groupEnergy[k];
+            c += groupPairs[k];
+        }
+        a.waveEnergy[blockIdx.x] = e;
+        a.waveCount[blockIdx.x] = c;
     }
 }
 
@@ -2254,13 +2263,14 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
         a.numTiles = subsetTiles;
         a.tileIndex = p->subsetTiles.ptr;
     }
-    const int blocks = std::max(1, std::min((a.numTiles+3)/4, p->gridLimit));
+    const bool compact = (p->method != IXH_NO_CUTOFF);
+    const int groupWaves = (compact ? QUEUE_GROUP_WAVES : 4);
+    const int blocks = std::max(1, std::min((a.numTiles+groupWaves-1)/groupWaves, p->gridLimit));
     a.tileEnergy = nullptr;
     if (p->numSegments > 0) {
         p->tileEnergy.reserve(std::max(p->numTiles, 1));
         a.tileEnergy = p->tileEnergy.ptr;
     }
-    const bool compact = (p->method != IXH_NO_CUTOFF);
     // partial sums of the launch: one per workgroup (cutoff tiles) or per wave (all-pairs tiles)
     const int numWaves = (compact ? blocks : 4*blocks);
     p->waveEnergy.reserve(numWaves);
@@ -2277,7 +2287,7 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
         switch (terms) {
 #define IXH_LAUNCH(T, TAB, ALCH) \
             if (compact) \
-                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a); \
+                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH>), dim3(blocks), dim3(64*QUEUE_GROUP_WAVES), tableBytes, stream, a); \
             else \
                 hipLaunchKernelGGL((k_pair_rotate<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a);
 // the variants without alchemical steps are built for the FP32 type-table kernels only
