@@ -368,10 +368,35 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     // where a step's lanes share one b or two consecutive ones.
     const int M = min(m, 64);
     const long long T = (long long) M*(M-1)/2;
-    for (long long p0 = pBegin+64*wave; p0 < pEnd; p0 += 64*WAVES) {
+    // Each wavefront takes a contiguous share of the item's pairs, 64 per step, so that in the rows its steps follow
+    // each other along a row: the pair 64 further on is in the same row or the next (rows have b >= 64 pairs), and
+    // the forces on the row's b are summed per lane over the steps of the row and over the wavefront only when the
+    // wavefront leaves it.
+    const long long share = (((pEnd-pBegin)+WAVES-1)/WAVES+63) & ~63LL;
+    const long long myBegin = pBegin+share*wave, myEnd = min(pEnd, myBegin+share);
+    int a = 0, b = 0;
+    int R = -1;                 // the row whose forces accR holds (accR1: row R + 1); wavefront-uniform
+    bool usedR1 = false;        // some lane has added to accR1
+    Real accR[3] = {0, 0, 0}, accR1[3] = {0, 0, 0};
+    auto flushRow = [&](int row, Real (&acc)[3]) {
+        for (int k = 0; k < 3; k++) {
+            const Real sum = wave_sum(acc[k]);
+            if (lane == 0)
+                atomicAdd(&sForce[3*row+k], (unsigned long long) to_fixed(sum));
+            acc[k] = 0;
+        }
+    };
+    for (long long p0 = myBegin; p0 < myEnd; p0 += 64) {
         const long long p = p0+lane;
-        int a, b;
-        if (p < T) {
+        if (p0 != myBegin && p0-64 >= T) {
+            // the previous step was all rows (b >= 64): advance by 64 pairs, at most into the next row
+            a += 64;
+            if (a >= b) {
+                a -= b;
+                b++;
+            }
+        }
+        else if (p < T) {
             const int q = (int) p;
             int r = (int) ((float) q*(1.0f/(float) M));
             while (r > 0 && r*M > q)
@@ -391,13 +416,29 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                 b++;
             a = (int) (p-(long long) b*(b-1)/2);
         }
-        const bool active = (p < pEnd);
-        // Rows of 64 pairs or more: a step's active lanes (lane 0 always is one) share b or b and b + 1.  The force on
-        // each of the two is then summed over the wavefront and added once, instead of up to 64 atomics on one LDS
-        // address; diagonal steps and the step that straddles the two orders add per lane.
+        const bool active = (p < myEnd);
+        // Rows of 64 pairs or more: a step's active lanes (lane 0 always is one) share b or b and b + 1, whose forces
+        // go to accR and accR1 instead of up to 64 atomics on one LDS address; the step that straddles the two
+        // orders, and the rotation order, add per lane.
         const int bFirst = __builtin_amdgcn_readfirstlane(b);
         const bool rows = (p0 >= T) && __all(!active || b == bFirst || b == bFirst+1);
-        Real fB[3] = {0, 0, 0}, fB1[3] = {0, 0, 0};
+        if (rows && bFirst != R) {
+            if (R >= 0) {
+                flushRow(R, accR);
+                if (bFirst == R+1) {
+                    for (int k = 0; k < 3; k++) {
+                        accR[k] = accR1[k];
+                        accR1[k] = 0;
+                    }
+                }
+                else if (usedR1)
+                    flushRow(R+1, accR1);
+            }
+            R = bFirst;
+            usedR1 = false;
+        }
+        if (rows)
+            usedR1 = usedR1 || __any(active && b != R);
         if (active) {
             bool ok = (sMol[a] != sMol[b]);
             R4 va = sDelta[a], vb = sDelta[b];
@@ -520,28 +561,18 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
                     const Real forceB = A[k]*bA+B[k]*bB+c[k]*bC;
                     if (!rows)
                         atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(forceB));
-                    else if (b == bFirst)
-                        fB[k] = forceB;
+                    else if (b == R)
+                        accR[k] += forceB;
                     else
-                        fB1[k] = forceB;
+                        accR1[k] += forceB;
                 }
             }
         }
-        if (rows) {
-            const bool second = __any(active && b != bFirst);
-            for (int k = 0; k < 3; k++) {
-                const Real sum = wave_sum(fB[k]);
-                if (lane == 0)
-                    atomicAdd(&sForce[3*bFirst+k], (unsigned long long) to_fixed(sum));
-            }
-            if (second) {
-                for (int k = 0; k < 3; k++) {
-                    const Real sum = wave_sum(fB1[k]);
-                    if (lane == 0)
-                        atomicAdd(&sForce[3*(bFirst+1)+k], (unsigned long long) to_fixed(sum));
-                }
-            }
-        }
+    }
+    if (R >= 0) {
+        flushRow(R, accR);
+        if (usedR1)
+            flushRow(R+1, accR1);
     }
     __syncthreads();
     // flush the neighbour forces and the force on A (integer sums: the order of the atomics does not matter)
