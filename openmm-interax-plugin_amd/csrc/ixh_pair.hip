@@ -444,6 +444,14 @@ __global__ void k_build_masks(MaskArgs a, unsigned long long* masks, int* tileSl
 // consecutive entries have distinct i and distinct j.  Within the diagonal tile s runs over 1..32 (s = 32 for
 // i < 32 only), which meets every unordered pair once.  Entries are i << 6 | j; each tile's list is padded with
 // 0xffff to a multiple of 4 entries.  With offsets == nullptr only the counts (before padding) are written.
+// Rows of the per-tile arrays (masks, partner slots) in a new tile order: row t of the output is row order[t].
+__global__ void k_gather_tile_rows(const int* order, const unsigned long long* masks, const int* slots, unsigned long long* masksOut,
+                                   int* slotsOut) {
+    const size_t from = (size_t) order[blockIdx.x]*BLOCK+threadIdx.x, to = (size_t) blockIdx.x*BLOCK+threadIdx.x;
+    masksOut[to] = masks[from];
+    slotsOut[to] = slots[from];
+}
+
 // The candidate pairs of a tile in rotation order (pairs (i, i+s mod 64) for s = 1, 2, ...), padded to a multiple
 // of 4 with 0xffff.  The tile kernel reads a list 256 entries per step, 4 consecutive u16 per lane, and takes them
 // in 4 sub-steps; each chunk of 256 (or the last, shorter one, of 4m entries) is stored transposed so that sub-step
@@ -2148,9 +2156,37 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         hipLaunchKernelGGL(k_candidate_lists, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->masks.ptr, p->tileCandidates.ptr,
                            (const int*) nullptr, (unsigned short*) nullptr);
         IXH_HIP_CHECK(hipGetLastError());
-        std::vector<int> counts(T), offsets(T+1, 0);
-        IXH_HIP_CHECK(hipMemcpyAsync(counts.data(), p->tileCandidates.ptr, sizeof(int)*T, hipMemcpyDeviceToHost, stream));
+        std::vector<int> unsortedCounts(T), counts(T), offsets(T+1, 0);
+        IXH_HIP_CHECK(hipMemcpyAsync(unsortedCounts.data(), p->tileCandidates.ptr, sizeof(int)*T, hipMemcpyDeviceToHost, stream));
         IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        {
+            // the tiles in launch order, decreasing candidate pairs (the workgroups dispatched last are the lightest),
+            // so that a wave reads its tile with no index in between
+            std::vector<int> order(T);
+            for (int t = 0; t < T; t++)
+                order[t] = t;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return unsortedCounts[x] > unsortedCounts[y]; });
+            std::vector<int2> sortedTiles(T);
+            for (int t = 0; t < T; t++) {
+                sortedTiles[t] = tiles[order[t]];
+                counts[t] = unsortedCounts[order[t]];
+            }
+            p->tileOrder.upload(order.data(), order.size(), stream);
+            DeviceArray<unsigned long long> masks;
+            DeviceArray<int> slots;
+            masks.reserve((size_t) T*BLOCK);
+            slots.reserve((size_t) T*BLOCK);
+            hipLaunchKernelGGL(k_gather_tile_rows, dim3(T), dim3(BLOCK), 0, stream, (const int*) p->tileOrder.ptr,
+                               (const unsigned long long*) p->masks.ptr, (const int*) p->tileSlots.ptr, masks.ptr, slots.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+            std::swap(p->masks.ptr, masks.ptr);
+            std::swap(p->masks.count, masks.count);
+            std::swap(p->tileSlots.ptr, slots.ptr);
+            std::swap(p->tileSlots.count, slots.count);
+            p->tiles.upload(sortedTiles.data(), sortedTiles.size(), stream);
+            IXH_HIP_CHECK(hipStreamSynchronize(stream));   // before the unsorted rows are freed
+        }
+        p->ordered = false;
         long long total = 0;
         for (int t = 0; t < T; t++) {
             p->numCandidates += counts[t];
@@ -2160,14 +2196,6 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
             offsets[t+1] = (int) total;
         }
         p->listOffsets.upload(offsets.data(), offsets.size(), stream);
-        p->ordered = true;
-        {
-            std::vector<int> order(T);
-            for (int t = 0; t < T; t++)
-                order[t] = t;
-            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return counts[x] > counts[y]; });
-            p->tileOrder.upload(order.data(), order.size(), stream);
-        }
         p->lists.reserve((size_t) std::max<long long>(total, 4));
         hipLaunchKernelGGL(k_candidate_lists, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->masks.ptr, p->tileCandidates.ptr,
                            (const int*) p->listOffsets.ptr, p->lists.ptr);
