@@ -39,6 +39,19 @@ def test_batch_against_oracle(precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_large_clusters_against_oracle(precision):
+    # sites with more than 64 later neighbours take the triple kernel's row order (pairs (a, b), b >= 64, in rows,
+    # the forces on b summed per row), and the 200-molecule cluster's first sites (more than 16 384 neighbour pairs)
+    # are split into several work items
+    systems, positions = clusters([200, 130, 66], seed=11)
+    batch = ih.Batch(systems, precision)
+    energies, forces = batch.evaluate(positions)
+    check_against_oracle(systems, positions, energies, forces, precision)
+    batch.close()
+
+
+@pytest.mark.gpu
 def test_same_result_alone_and_in_a_batch():
     systems, positions = clusters([7, 30, 2, 90, 11], seed=5)
     batch = ih.Batch(systems, "mixed")
