@@ -2,9 +2,10 @@
 //
 // No counterpart in the reference, whose kernels evaluate one System per Context.  The systems of a batch are
 // concatenated into one segmented pair handle (NoCutoff tiles within each system only) and one segmented three-body
-// handle (neighbour lists of static capacity), sharing one device state.  One evaluation is then a fixed sequence of
-// eight kernels with no host synchronization, captured once into a HIP graph and replayed; per-system energies are
-// summed in a fixed order, so a system's energy and forces are the same alone and in any batch.  A tuner times the
+// handle (neighbour lists of static capacity), sharing one device state.  One evaluation is then a fixed set of
+// kernels with no host synchronization, the pair half and the three-body half on two streams (forked and joined by
+// events), captured once into a HIP graph and replayed; per-system energies are summed in a fixed order, so a
+// system's energy and forces are the same alone and in any batch.  A tuner times the
 // launch shapes of the pair tiles (workgroups of the persistent tile launch) and keeps the fastest.
 #include "ixh_common.h"
 #include "ixh_internal.h"
@@ -20,6 +21,27 @@ __global__ void k_fill_double(double* p, int n, double value) {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i < n)
         p[i] = value;
+}
+
+// Zero the per-system energies of both halves of an evaluation and the three-body half's total.
+__global__ void k_zero_energies(double* pair, double* three, int n, double* threeTotal) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n) {
+        pair[i] = 0;
+        three[i] = 0;
+    }
+    if (i == 0)
+        *threeTotal = 0;
+}
+
+// After both halves: each system's energy is its pair energy plus its three-body energy, in that order (the sum a
+// single stream forms), and the three-body total joins the state's energy.
+__global__ void k_add_three_body(double* energies, const double* three, int n, double* total, const double* threeTotal) {
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        energies[i] += three[i];
+    if (i == 0)
+        total[0] += *threeTotal;
 }
 
 void check(int status) {
@@ -41,6 +63,11 @@ struct ixh_batch {
     ixh_state* state = nullptr;
     ixh_buffers buffers;
     DeviceArray<double> energies;
+    // The pair half of an evaluation runs on the state's stream and the three-body half beside it on `side` (forked
+    // and joined by events, also inside the captured graph), with its own energies, added at the join.
+    hipStream_t side = nullptr;
+    hipEvent_t forked = nullptr, joined = nullptr;
+    DeviceArray<double> threeEnergies, threeTotal;
     double* hostEnergies = nullptr;   // pinned
     int evaluated = 0;                // eager evaluations so far (the graph is captured after the first)
     int graphFlags = -1;
@@ -51,6 +78,17 @@ struct ixh_batch {
 };
 
 namespace {
+
+void release_side(ixh_batch* b) {
+    if (b->side != nullptr)
+        (void) hipStreamDestroy(b->side);
+    if (b->forked != nullptr)
+        (void) hipEventDestroy(b->forked);
+    if (b->joined != nullptr)
+        (void) hipEventDestroy(b->joined);
+    b->side = nullptr;
+    b->forked = b->joined = nullptr;
+}
 
 void destroy_graph(ixh_batch* b) {
     if (b->graphExec != nullptr)
@@ -65,13 +103,32 @@ void destroy_graph(ixh_batch* b) {
 // The kernels of one evaluation, on the state's stream.
 void enqueue(ixh_batch* b, int flags) {
     hipStream_t stream = (hipStream_t) b->buffers.stream;
+    const dim3 grid((b->numSystems+255)/256);
     check(ixh_state_clear(b->state));
-    hipLaunchKernelGGL(k_fill_double, dim3((b->numSystems+255)/256), dim3(256), 0, stream, b->energies.ptr, b->numSystems, 0.0);
+    if (b->pair == nullptr || b->threebody == nullptr) {
+        hipLaunchKernelGGL(k_fill_double, grid, dim3(256), 0, stream, b->energies.ptr, b->numSystems, 0.0);
+        IXH_HIP_CHECK(hipGetLastError());
+        if (b->pair != nullptr)
+            check(ixh_pair_execute_segments(b->pair, &b->buffers, b->pairTerms, flags, b->energies.ptr));
+        if (b->threebody != nullptr)
+            check(ixh_threebody_execute_segments(b->threebody, &b->buffers, flags, b->energies.ptr));
+        return;
+    }
+    hipLaunchKernelGGL(k_zero_energies, grid, dim3(256), 0, stream, b->energies.ptr, b->threeEnergies.ptr, b->numSystems,
+                       b->threeTotal.ptr);
     IXH_HIP_CHECK(hipGetLastError());
-    if (b->pair != nullptr)
-        check(ixh_pair_execute_segments(b->pair, &b->buffers, b->pairTerms, flags, b->energies.ptr));
-    if (b->threebody != nullptr)
-        check(ixh_threebody_execute_segments(b->threebody, &b->buffers, flags, b->energies.ptr));
+    IXH_HIP_CHECK(hipEventRecord(b->forked, stream));
+    IXH_HIP_CHECK(hipStreamWaitEvent(b->side, b->forked, 0));
+    check(ixh_pair_execute_segments(b->pair, &b->buffers, b->pairTerms, flags, b->energies.ptr));
+    ixh_buffers three = b->buffers;
+    three.stream = b->side;
+    three.energy = b->threeTotal.ptr;
+    check(ixh_threebody_execute_segments(b->threebody, &three, flags, b->threeEnergies.ptr));
+    IXH_HIP_CHECK(hipEventRecord(b->joined, b->side));
+    IXH_HIP_CHECK(hipStreamWaitEvent(stream, b->joined, 0));
+    hipLaunchKernelGGL(k_add_three_body, grid, dim3(256), 0, stream, b->energies.ptr, (const double*) b->threeEnergies.ptr,
+                       b->numSystems, b->buffers.energy, (const double*) b->threeTotal.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
 }
 
 void run(ixh_batch* b, int flags) {
@@ -140,6 +197,13 @@ int ixh_batch_create(int device, const ixh_batch_desc* d, ixh_batch** out) {
             check(ixh_state_create(device, b->numParticles, d->precision, &b->state));
             check(ixh_state_get_buffers(b->state, &b->buffers));
             b->energies.reserve(b->numSystems);
+            if (b->pair != nullptr && b->threebody != nullptr) {
+                b->threeEnergies.reserve(b->numSystems);
+                b->threeTotal.reserve(1);
+                IXH_HIP_CHECK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+                IXH_HIP_CHECK(hipEventCreateWithFlags(&b->forked, hipEventDisableTiming));
+                IXH_HIP_CHECK(hipEventCreateWithFlags(&b->joined, hipEventDisableTiming));
+            }
             IXH_HIP_CHECK(hipHostMalloc((void**) &b->hostEnergies, sizeof(double)*b->numSystems, hipHostMallocDefault));
         }
         catch (...) {
@@ -151,6 +215,7 @@ int ixh_batch_create(int device, const ixh_batch_desc* d, ixh_batch** out) {
                 ixh_state_destroy(b->state);
             if (b->hostEnergies != nullptr)
                 (void) hipHostFree(b->hostEnergies);
+            release_side(b);
             delete b;
             throw;
         }
@@ -173,6 +238,7 @@ int ixh_batch_destroy(ixh_batch* b) {
             ixh_state_destroy(b->state);
         if (b->hostEnergies != nullptr)
             (void) hipHostFree(b->hostEnergies);
+        release_side(b);
         delete b;
     });
 }
