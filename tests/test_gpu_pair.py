@@ -15,7 +15,7 @@ import interax_hip as ih
 import oracle
 import workloads
 
-PRECISIONS = ["double", "mixed"]
+PRECISIONS = ["double", "mixed", "single"]
 TOL = {"double": (1e-10, 1e-9), "mixed": (2e-5, 2e-4), "single": (2e-5, 2e-4)}
 RESOLUTION = 1e-8
 

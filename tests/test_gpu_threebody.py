@@ -15,8 +15,8 @@ import interax_hip as ih
 import oracle
 import workloads
 
-PRECISIONS = ["double", "mixed"]
-TOL = {"double": (1e-10, 1e-8), "mixed": (1e-5, 1e-4)}
+PRECISIONS = ["double", "mixed", "single"]
+TOL = {"double": (1e-10, 1e-8), "mixed": (1e-5, 1e-4), "single": (1e-5, 1e-4)}
 RESOLUTION = 1e-8
 # InteraxFF molecular site parameters (TestInteraxThreeBodyForce.h:172-173)
 SITE_TYPES = [(0.004087456746801944, 911.0681141195369, 0.03), (0.0016246535729804365, 1262.664320774599, 0.025),
