@@ -298,6 +298,25 @@ def test_nonperiodic_cell_grid(precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_lists_longer_than_the_first_capacity(precision):
+    # 150 sites all within the cutoff of each other: the first site's list of later neighbours (149 entries) is
+    # longer than the first capacity (64), so the search and the triple kernel, launched before the longest list is
+    # known, run again with more room before the site forces are applied; the next evaluation keeps that room
+    rng = np.random.default_rng(21)
+    n = 150
+    pos = workloads.rsa_points(rng, n, 1.6, 0.2, periodic=False)
+    system, f = single_particle_sites(n)
+    f.setCutoffDistance(5.0)
+    context = ih.Context(system, {"Precision": precision})
+    context.setPositions(pos)
+    compare(context, system, f, pos, precision)
+    moved = pos+rng.normal(0.0, 0.01, size=pos.shape)
+    context.setPositions(moved)
+    compare(context, system, f, moved, precision)
+
+
+@pytest.mark.gpu
 def test_grid_with_one_cell_along_an_axis():
     # a periodic box that is wide along x and y but thinner than three cutoffs along z: one cell along z
     rng = np.random.default_rng(3)
