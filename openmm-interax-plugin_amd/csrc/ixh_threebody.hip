@@ -127,6 +127,38 @@ __global__ void k_site_cells(const double* sites, int numSites, CellGrid g, unsi
     values[i] = i;
 }
 
+// Plain handles whose cell grid comes from the box: the centroids, their cells, the zeroed cell ranges and the
+// zeroed site forces of an evaluation in one launch (thread i: site i, cell i, force component i).
+__global__ void k_prepare_sites(const void* posq, const float4* correction, int precision, const int* offsets, const int* particles,
+                                int numSites, double* sites, CellGrid g, unsigned* keys, int* values, int* cellStart, int* cellEnd,
+                                int numCells, long long* siteForce) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < numCells) {
+        cellStart[i] = 0;
+        cellEnd[i] = 0;
+    }
+    if (i < 3*numSites)
+        siteForce[i] = 0;
+    if (i >= numSites)
+        return;
+    double s[3] = {0, 0, 0};
+    for (int k = offsets[i]; k < offsets[i+1]; k++) {
+        double p[3];
+        load_pos(posq, correction, precision, particles[k], p);
+        s[0] = __dadd_rn(s[0], p[0]);
+        s[1] = __dadd_rn(s[1], p[1]);
+        s[2] = __dadd_rn(s[2], p[2]);
+    }
+    const double scale = 1.0/(double) (offsets[i+1]-offsets[i]);
+    const double c[3] = {__dmul_rn(s[0], scale), __dmul_rn(s[1], scale), __dmul_rn(s[2], scale)};
+    sites[3*i] = c[0];
+    sites[3*i+1] = c[1];
+    sites[3*i+2] = c[2];
+    const int cx = cell_coordinate(c[0], 0, g), cy = cell_coordinate(c[1], 1, g), cz = cell_coordinate(c[2], 2, g);
+    keys[i] = (unsigned) ((cz*g.n[1]+cy)*g.n[0]+cx);
+    values[i] = i;
+}
+
 __global__ void k_fill_int(int* p, int n, int value) {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i < n)
@@ -1142,9 +1174,13 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         h->lastStream = stream;
         const float4* corr = (const float4*) b->posq_correction;
         BoxT box = {b->box[0], b->box[3], b->box[4], b->box[6], b->box[7], b->box[8]};
-        hipLaunchKernelGGL(k_sites, dim3(grid_of(n, 128)), dim3(128), 0, stream, b->posq, corr, b->precision, h->offsetsDev.ptr,
-                           h->particlesDev.ptr, n, h->sites.ptr);
-        IXH_HIP_CHECK(hipGetLastError());
+        // plain handles whose cell grid comes from the box prepare the sites in one launch, after the grid
+        const bool gridFromBox = (h->numSegments == 0 && (h->periodic || n <= 4096));
+        if (!gridFromBox) {
+            hipLaunchKernelGGL(k_sites, dim3(grid_of(n, 128)), dim3(128), 0, stream, b->posq, corr, b->precision, h->offsetsDev.ptr,
+                               h->particlesDev.ptr, n, h->sites.ptr);
+            IXH_HIP_CHECK(hipGetLastError());
+        }
         double cutoff2 = h->cutoff*h->cutoff;
         if (h->numSegments > 0) {
             // segmented: static list capacity, no host synchronization
@@ -1231,7 +1267,13 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         h->sortedSites.reserve(n);
         h->cellStart.reserve(numCells);
         h->cellEnd.reserve(numCells);
-        hipLaunchKernelGGL(k_site_cells, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->sites.ptr, n, g, h->cellKeys.ptr, h->cellValues.ptr);
+        if (gridFromBox)
+            hipLaunchKernelGGL(k_prepare_sites, dim3(grid_of(std::max(3*n, numCells), 256)), dim3(256), 0, stream, b->posq, corr,
+                               b->precision, (const int*) h->offsetsDev.ptr, (const int*) h->particlesDev.ptr, n, h->sites.ptr, g,
+                               h->cellKeys.ptr, h->cellValues.ptr, h->cellStart.ptr, h->cellEnd.ptr, numCells, h->siteForce.ptr);
+        else
+            hipLaunchKernelGGL(k_site_cells, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->sites.ptr, n, g, h->cellKeys.ptr,
+                               h->cellValues.ptr);
         IXH_HIP_CHECK(hipGetLastError());
         int bits = 1;
         while ((1LL << bits) < numCells)
@@ -1249,8 +1291,10 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         h->sortTemp.reserve(std::max(tempBytes, maxBytes));
         IXH_HIP_CHECK(rocprim::radix_sort_pairs(h->sortTemp.ptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
                                                 h->sortedSites.ptr, (size_t) n, 0, bits, stream));
-        hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellStart.ptr, numCells, 0);
-        hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellEnd.ptr, numCells, 0);
+        if (!gridFromBox) {
+            hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellStart.ptr, numCells, 0);
+            hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellEnd.ptr, numCells, 0);
+        }
         hipLaunchKernelGGL(k_cell_ranges, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->cellKeysSorted.ptr, n, h->cellStart.ptr,
                            h->cellEnd.ptr);
         IXH_HIP_CHECK(hipGetLastError());
@@ -1287,8 +1331,10 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         }
         if (maxCount > h->listCapacity)
             throw Error(IXH_ERROR_HIP, "ixh_threebody_execute: neighbour list capacity not reached after a resize");
-        hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
-        IXH_HIP_CHECK(hipGetLastError());
+        if (!gridFromBox) {
+            hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
+            IXH_HIP_CHECK(hipGetLastError());
+        }
         if (h->precision == IXH_DOUBLE)
             launch_triples<double>(h, b, box, cutoff2, maxCount, stream);
         else
