@@ -131,11 +131,11 @@ __global__ void k_site_cells(const double* sites, int numSites, CellGrid g, unsi
 // zeroed site forces of an evaluation in one launch (thread i: site i, cell i, force component i).
 __global__ void k_prepare_sites(const void* posq, const float4* correction, int precision, const int* offsets, const int* particles,
                                 int numSites, double* sites, CellGrid g, unsigned* keys, int* values, int* cellStart, int* cellEnd,
-                                int numCells, long long* siteForce) {
+                                int numCells, long long* siteForce, int* cellCount, int* identity) {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i < numCells) {
         cellStart[i] = 0;
-        cellEnd[i] = 0;
+        cellEnd[i] = (identity != nullptr ? numSites : 0);   // one cell: every site, in index order
     }
     if (i < 3*numSites)
         siteForce[i] = 0;
@@ -155,8 +155,65 @@ __global__ void k_prepare_sites(const void* posq, const float4* correction, int 
     sites[3*i+1] = c[1];
     sites[3*i+2] = c[2];
     const int cx = cell_coordinate(c[0], 0, g), cy = cell_coordinate(c[1], 1, g), cz = cell_coordinate(c[2], 2, g);
-    keys[i] = (unsigned) ((cz*g.n[1]+cy)*g.n[0]+cx);
+    const unsigned key = (unsigned) ((cz*g.n[1]+cy)*g.n[0]+cx);
+    keys[i] = key;
     values[i] = i;
+    if (cellCount != nullptr)
+        atomicAdd(&cellCount[key], 1);
+    if (identity != nullptr)
+        identity[i] = i;
+}
+
+// Counting sort of the sites by cell (grids of at most 65 536 cells): start and end of every cell from an exclusive
+// scan of the counts (one workgroup, a running total over chunks of 1024), which it leaves zero for the next
+// evaluation; then every site takes a place in its cell, and each cell's places are put in site order, so that the
+// result is the stable sort by cell.
+__global__ void __launch_bounds__(1024) k_cell_scan(int* cellCount, int numCells, int* cellStart, int* cellEnd, int* cursor) {
+    __shared__ int sum[1024];
+    int carry = 0;
+    for (int base = 0; base < numCells; base += 1024) {
+        const int c = base+(int) threadIdx.x;
+        const int v = (c < numCells ? cellCount[c] : 0);
+        sum[threadIdx.x] = v;
+        __syncthreads();
+        for (int offset = 1; offset < 1024; offset <<= 1) {
+            const int add = (threadIdx.x >= (unsigned) offset ? sum[threadIdx.x-offset] : 0);
+            __syncthreads();
+            sum[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (c < numCells) {
+            const int start = carry+sum[threadIdx.x]-v;
+            cellStart[c] = start;
+            cellEnd[c] = start+v;
+            cursor[c] = start;
+            cellCount[c] = 0;
+        }
+        carry += sum[1023];
+        __syncthreads();
+    }
+}
+
+__global__ void k_scatter_sites(const unsigned* keys, int numSites, int* cursor, int* sortedSites) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < numSites)
+        sortedSites[atomicAdd(&cursor[keys[i]], 1)] = i;
+}
+
+__global__ void k_sort_cells(const int* cellStart, const int* cellEnd, int numCells, int* sortedSites) {
+    const int c = blockIdx.x*blockDim.x+threadIdx.x;
+    if (c >= numCells)
+        return;
+    const int begin = cellStart[c], end = cellEnd[c];
+    for (int k = begin+1; k < end; k++) {
+        const int v = sortedSites[k];
+        int m = k;
+        while (m > begin && sortedSites[m-1] > v) {
+            sortedSites[m] = sortedSites[m-1];
+            m--;
+        }
+        sortedSites[m] = v;
+    }
 }
 
 __global__ void k_fill_int(int* p, int n, int value) {
@@ -782,6 +839,7 @@ struct ixh_threebody {
     // cell grid of the neighbour search
     DeviceArray<unsigned> cellKeys, cellKeysSorted;
     DeviceArray<int> cellValues, sortedSites, cellStart, cellEnd;
+    DeviceArray<int> cellCount, cellCursor;   // counting sort by cell (cellCount zero between evaluations)
     DeviceArray<char> sortTemp;
     DeviceArray<double> bbox;
     // list totals of the neighbour search read back by the host: the number of entries and the longest list
@@ -1267,10 +1325,23 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         h->sortedSites.reserve(n);
         h->cellStart.reserve(numCells);
         h->cellEnd.reserve(numCells);
+        // the sites in cell order: one cell needs no sort; grids of a few sites per cell a counting sort (three small
+        // launches); other grids the radix sort
+        const bool identity = (gridFromBox && numCells == 1);
+        const bool counting = (gridFromBox && !identity && numCells <= 65536 && (long long) n <= 64LL*numCells);
+        if (counting) {
+            const size_t had = h->cellCount.count;
+            h->cellCount.reserve(numCells);
+            h->cellCursor.reserve(numCells);
+            if (h->cellCount.count != had || had == 0)
+                hipLaunchKernelGGL(k_fill_int, dim3(grid_of((int) h->cellCount.count, 256)), dim3(256), 0, stream, h->cellCount.ptr,
+                                   (int) h->cellCount.count, 0);
+        }
         if (gridFromBox)
             hipLaunchKernelGGL(k_prepare_sites, dim3(grid_of(std::max(3*n, numCells), 256)), dim3(256), 0, stream, b->posq, corr,
                                b->precision, (const int*) h->offsetsDev.ptr, (const int*) h->particlesDev.ptr, n, h->sites.ptr, g,
-                               h->cellKeys.ptr, h->cellValues.ptr, h->cellStart.ptr, h->cellEnd.ptr, numCells, h->siteForce.ptr);
+                               h->cellKeys.ptr, h->cellValues.ptr, h->cellStart.ptr, h->cellEnd.ptr, numCells, h->siteForce.ptr,
+                               counting ? h->cellCount.ptr : (int*) nullptr, identity ? h->sortedSites.ptr : (int*) nullptr);
         else
             hipLaunchKernelGGL(k_site_cells, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->sites.ptr, n, g, h->cellKeys.ptr,
                                h->cellValues.ptr);
@@ -1289,14 +1360,24 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         IXH_HIP_CHECK(rocprim::reduce(nullptr, maxBytes, h->count.ptr, h->listTotals.ptr, 0, (size_t) n, rocprim::maximum<int>(),
                                       stream));
         h->sortTemp.reserve(std::max(tempBytes, maxBytes));
-        IXH_HIP_CHECK(rocprim::radix_sort_pairs(h->sortTemp.ptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr, h->cellValues.ptr,
-                                                h->sortedSites.ptr, (size_t) n, 0, bits, stream));
-        if (!gridFromBox) {
-            hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellStart.ptr, numCells, 0);
-            hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellEnd.ptr, numCells, 0);
+        if (counting) {
+            hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, stream, h->cellCount.ptr, numCells, h->cellStart.ptr, h->cellEnd.ptr,
+                               h->cellCursor.ptr);
+            hipLaunchKernelGGL(k_scatter_sites, dim3(grid_of(n, 256)), dim3(256), 0, stream, (const unsigned*) h->cellKeys.ptr, n,
+                               h->cellCursor.ptr, h->sortedSites.ptr);
+            hipLaunchKernelGGL(k_sort_cells, dim3(grid_of(numCells, 64)), dim3(64), 0, stream, (const int*) h->cellStart.ptr,
+                               (const int*) h->cellEnd.ptr, numCells, h->sortedSites.ptr);
         }
-        hipLaunchKernelGGL(k_cell_ranges, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->cellKeysSorted.ptr, n, h->cellStart.ptr,
-                           h->cellEnd.ptr);
+        else if (!identity) {
+            IXH_HIP_CHECK(rocprim::radix_sort_pairs(h->sortTemp.ptr, tempBytes, h->cellKeys.ptr, h->cellKeysSorted.ptr,
+                                                    h->cellValues.ptr, h->sortedSites.ptr, (size_t) n, 0, bits, stream));
+            if (!gridFromBox) {
+                hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellStart.ptr, numCells, 0);
+                hipLaunchKernelGGL(k_fill_int, dim3(grid_of(numCells, 256)), dim3(256), 0, stream, h->cellEnd.ptr, numCells, 0);
+            }
+            hipLaunchKernelGGL(k_cell_ranges, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->cellKeysSorted.ptr, n, h->cellStart.ptr,
+                               h->cellEnd.ptr);
+        }
         IXH_HIP_CHECK(hipGetLastError());
         const int exactImage = (h->periodic && 2*h->cutoff >= std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
         // Lists of a fixed capacity per site, written in one pass; only the longest list comes back to the host (it
