@@ -58,3 +58,9 @@ def test_invalid_arguments_are_rejected_before_any_device_work():
     d.method = 5
     assert lib.ixh_pair_create(0, ctypes.byref(d), ctypes.byref(handle)) == 1
     assert lib.ixh_last_error() == b"InteraxPairForce: Illegal value for nonbonded method"
+    # the state and list-check calls reject missing handles the same way
+    assert lib.ixh_state_set_stream(None, None) == 1
+    assert b"ixh_state_set_stream" in lib.ixh_last_error()
+    valid = ctypes.c_int()
+    assert lib.ixh_pair_list_was_valid(None, ctypes.byref(valid)) == 1
+    assert b"null argument" in lib.ixh_last_error()
