@@ -169,12 +169,17 @@ __global__ void k_prepare_sites(const void* posq, const float4* correction, int 
 // evaluation; then every site takes a place in its cell, and each cell's places are put in site order, so that the
 // result is the stable sort by cell.
 __global__ void __launch_bounds__(1024) k_cell_scan(int* cellCount, int numCells, int* cellStart, int* cellEnd, int* cursor) {
+    // each thread takes 4 consecutive cells of a chunk of 4096
     __shared__ int sum[1024];
     int carry = 0;
-    for (int base = 0; base < numCells; base += 1024) {
-        const int c = base+(int) threadIdx.x;
-        const int v = (c < numCells ? cellCount[c] : 0);
-        sum[threadIdx.x] = v;
+    for (int base = 0; base < numCells; base += 4096) {
+        const int first = base+4*(int) threadIdx.x;
+        int v[4], own = 0;
+        for (int k = 0; k < 4; k++) {
+            v[k] = (first+k < numCells ? cellCount[first+k] : 0);
+            own += v[k];
+        }
+        sum[threadIdx.x] = own;
         __syncthreads();
         for (int offset = 1; offset < 1024; offset <<= 1) {
             const int add = (threadIdx.x >= (unsigned) offset ? sum[threadIdx.x-offset] : 0);
@@ -182,12 +187,16 @@ __global__ void __launch_bounds__(1024) k_cell_scan(int* cellCount, int numCells
             sum[threadIdx.x] += add;
             __syncthreads();
         }
-        if (c < numCells) {
-            const int start = carry+sum[threadIdx.x]-v;
-            cellStart[c] = start;
-            cellEnd[c] = start+v;
-            cursor[c] = start;
-            cellCount[c] = 0;
+        int start = carry+sum[threadIdx.x]-own;
+        for (int k = 0; k < 4; k++) {
+            const int c = first+k;
+            if (c < numCells) {
+                cellStart[c] = start;
+                cellEnd[c] = start+v[k];
+                cursor[c] = start;
+                cellCount[c] = 0;
+            }
+            start += v[k];
         }
         carry += sum[1023];
         __syncthreads();
@@ -200,20 +209,32 @@ __global__ void k_scatter_sites(const unsigned* keys, int numSites, int* cursor,
         sortedSites[atomicAdd(&cursor[keys[i]], 1)] = i;
 }
 
-__global__ void k_sort_cells(const int* cellStart, const int* cellEnd, int numCells, int* sortedSites) {
+// Each cell's sites in index order: cells of up to SORT_CELL_LDS sites are sorted in a slice of LDS per thread,
+// longer ones in place.
+constexpr int SORT_CELL_LDS = 32;
+__global__ void __launch_bounds__(64) k_sort_cells(const int* cellStart, const int* cellEnd, int numCells, int* sortedSites) {
+    __shared__ int slice[64][SORT_CELL_LDS+1];
     const int c = blockIdx.x*blockDim.x+threadIdx.x;
     if (c >= numCells)
         return;
     const int begin = cellStart[c], end = cellEnd[c];
-    for (int k = begin+1; k < end; k++) {
-        const int v = sortedSites[k];
+    const int len = end-begin;
+    int* v = (len <= SORT_CELL_LDS ? slice[threadIdx.x] : sortedSites+begin);
+    if (len <= SORT_CELL_LDS)
+        for (int k = 0; k < len; k++)
+            v[k] = sortedSites[begin+k];
+    for (int k = 1; k < len; k++) {
+        const int x = v[k];
         int m = k;
-        while (m > begin && sortedSites[m-1] > v) {
-            sortedSites[m] = sortedSites[m-1];
+        while (m > 0 && v[m-1] > x) {
+            v[m] = v[m-1];
             m--;
         }
-        sortedSites[m] = v;
+        v[m] = x;
     }
+    if (len <= SORT_CELL_LDS)
+        for (int k = 0; k < len; k++)
+            sortedSites[begin+k] = v[k];
 }
 
 __global__ void k_fill_int(int* p, int n, int value) {
