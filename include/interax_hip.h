@@ -86,20 +86,30 @@ int ixh_device_count(int* count);
 
 /*
  * The per-Context device buffers of a platform: what OpenMM's platform data owns for a Context and hands to a
- * kernel's execute() (the posq array, the fixed-point force buffer, the energy accumulator, the box and the
- * context parameter).  All pointers are device pointers.
+ * kernel's execute() (the posq array, the fixed-point force buffer, the energy accumulator, the box, the context
+ * parameter and the atom index array).  All pointers are device pointers.
+ *
+ * Particle order.  posq and force are in the platform's current order, which may differ from the System's: OpenMM's
+ * GPU platforms reorder atoms for locality, but only ever exchange whole molecules that are identical in every
+ * parameter of every force (the ComputeForceInfo contract, CommonInteraxKernels.cpp:111-137, 386-412).  Per-particle
+ * parameters, exclusions and three-body sites given in System order therefore stay valid in the current order; the
+ * one thing that changes is the original index of the particle in each slot, which the InteraxOrder2 overlap rule
+ * uses (the width of the particle with the lower original index, CommonInteraxKernels.cpp:213-216,
+ * interaxSlaterOverlap.cc:14-17).  atom_index carries it: atom_index[i] is the System index of the particle held in
+ * slot i (OpenMM's ComputeContext::getAtomIndexArray); NULL means the identity.  It is read on every call.
  */
 typedef struct {
     int num_particles;
     int precision;                 /* IXH_SINGLE / IXH_MIXED: posq is float4; IXH_DOUBLE: posq is double4 */
-    const void* posq;              /* x, y, z, (unused) per particle, in the particle order of the System */
+    const void* posq;              /* x, y, z, (unused) per particle, in the platform's current order */
     const void* posq_correction;   /* IXH_MIXED: float4 low-order parts, position = posq + correction; else NULL */
-    long long* force;              /* component c of particle i at force[c*force_stride + i], 2^32 per kJ/mol/nm */
+    long long* force;              /* component c of slot i at force[c*force_stride + i], 2^32 per kJ/mol/nm */
     int force_stride;              /* >= num_particles */
     double* energy;                /* the energy of the call is ADDED to energy[0] */
     double box[9];                 /* periodic box vectors a, b, c (rows), in OpenMM's reduced form */
     double lambda;                 /* current value of the alchemical context parameter */
     void* stream;                  /* hipStream_t, or NULL for the default stream */
+    const int* atom_index;         /* System index of the particle in each slot, or NULL (identity) */
 } ixh_buffers;
 
 /*
@@ -208,6 +218,11 @@ int ixh_pair_execute_subset(ixh_pair* pair, const ixh_buffers* buffers, int term
 int ixh_pair_list_was_valid(ixh_pair* pair, int* valid);
 /* Number of interacting pairs of the last execute() (waits for it). */
 int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
+/* How the current neighbour list takes periodic images (no counterpart in the reference): of its `tiles` tiles,
+   `per_block_tiles` take one image shift per pair of 64-particle blocks (exact for every pair in range when the box is
+   wider than twice the cutoff plus the extents of both blocks and the skin); the others take the minimum image of
+   every pair.  0 per-block tiles when the method is not periodic.  Waits for the last execute(). */
+int ixh_pair_image_mode(ixh_pair* pair, long long* per_block_tiles, long long* tiles);
 /* Size of the current neighbour list (no counterpart in the reference): the number of 64 x 64 tiles and of candidate
    pairs in them (within the cutoff plus the list skin, exclusions removed).  Waits for the last execute(). */
 int ixh_pair_list_statistics(ixh_pair* pair, long long* tiles, long long* candidates);
@@ -289,6 +304,10 @@ int ixh_batch_get_energies(ixh_batch* batch, double* energies);          /* wait
 int ixh_batch_get_forces(ixh_batch* batch, double* forces);              /* waits; 3 x total particles */
 int ixh_batch_set_graph(ixh_batch* batch, int enable);                   /* 0: launch the kernels eagerly */
 int ixh_batch_autotune(ixh_batch* batch, int flags, int* chosen_workgroups);
+/* With profiling enabled (ixh_set_profiling), the summed durations of the batch's pair-tile and triple kernels since
+   the last reset (eager evaluations only: profiling evaluates eagerly), and their launch counts. */
+int ixh_batch_kernel_time(ixh_batch* batch, double* pair_ms, long long* pair_launches, double* triple_ms, long long* triple_launches);
+int ixh_batch_reset_kernel_time(ixh_batch* batch);
 
 /*
  * Serving (no counterpart in the reference): a request loop over device-resident systems, with its own thread.

@@ -301,6 +301,40 @@ int ixh_batch_set_graph(ixh_batch* b, int enable) {
     });
 }
 
+int ixh_batch_kernel_time(ixh_batch* b, double* pair_ms, long long* pair_launches, double* triple_ms, long long* triple_launches) {
+    return guarded([&] {
+        if (b == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_batch_kernel_time: null argument");
+        double ms = 0;
+        long long launches = 0;
+        if (b->pair != nullptr)
+            check(ixh_pair_kernel_time(b->pair, &ms, &launches));
+        if (pair_ms != nullptr)
+            *pair_ms = ms;
+        if (pair_launches != nullptr)
+            *pair_launches = launches;
+        ms = 0;
+        launches = 0;
+        if (b->threebody != nullptr)
+            check(ixh_threebody_kernel_time(b->threebody, &ms, &launches));
+        if (triple_ms != nullptr)
+            *triple_ms = ms;
+        if (triple_launches != nullptr)
+            *triple_launches = launches;
+    });
+}
+
+int ixh_batch_reset_kernel_time(ixh_batch* b) {
+    return guarded([&] {
+        if (b == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_batch_reset_kernel_time: null argument");
+        if (b->pair != nullptr)
+            check(ixh_pair_reset_kernel_time(b->pair));
+        if (b->threebody != nullptr)
+            check(ixh_threebody_reset_kernel_time(b->threebody));
+    });
+}
+
 int ixh_batch_autotune(ixh_batch* b, int flags, int* chosen) {
     return guarded([&] {
         if (b == nullptr)

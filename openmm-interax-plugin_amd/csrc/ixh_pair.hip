@@ -45,6 +45,10 @@ constexpr int BLOCK = 64;
 constexpr int INDEX_MASK = 0x3fffffff;
 constexpr int ALCHEMICAL_BIT = 0x40000000;
 constexpr double SKIN = 0.1;   // neighbour list padding (nm); rebuilt when a particle has moved skin/2
+// Tile records are (block I, partner-chunk offset or -1 for the diagonal tile); bit 30 of the first word marks a tile
+// whose pairs need their own minimum image (the box is too small for one image shift per block pair).
+constexpr int TILE_PAIR_IMAGE = 1 << 30;
+constexpr int TILE_BLOCK_MASK = TILE_PAIR_IMAGE-1;
 
 template <typename Real>
 struct Vec4T;
@@ -399,9 +403,20 @@ struct MaskArgs {
     int periodic;
     BoxD box;
     double listCutoff2;
+    const double* bounds;    // per block: centre, half extent (periodic cutoff lists)
+    double imageLimit;       // half the smallest box width minus the cutoff and twice the skin
 };
 
-__global__ void k_build_masks(MaskArgs a, unsigned long long* masks, int* tileSlots) {
+// Diameter of the bounding box of block b.
+__device__ __forceinline__ double block_extent(const double* bounds, int b) {
+    const double* h = bounds+6*b+3;
+    return 2*sqrt(h[0]*h[0]+h[1]*h[1]+h[2]*h[2]);
+}
+
+// The candidate masks and partner slots of every tile; with periodic bounds, also whether the tile's pairs need
+// their own minimum image (tileImage[t] = 1): one image shift per block pair is exact for every pair in range when
+// cutoff + (extent_I + skin) + (extent_J + skin) < half the box width (particles move at most skin/2 before a rebuild).
+__global__ void k_build_masks(MaskArgs a, unsigned long long* masks, int* tileSlots, int* tileImage) {
     __shared__ int sAtom[BLOCK];
     __shared__ double sPos[BLOCK][3];
     int t = blockIdx.x;
@@ -410,6 +425,15 @@ __global__ void k_build_masks(MaskArgs a, unsigned long long* masks, int* tileSl
     int jslot = (tile.y < 0 ? tile.x*BLOCK+lane : a.partners[tile.y+lane]);
     int ja = (jslot < 0 ? -1 : a.perm[jslot]);
     tileSlots[(size_t) t*BLOCK+lane] = jslot;
+    if (a.useCutoff && a.periodic) {
+        double extent = (jslot < 0 ? 0.0 : block_extent(a.bounds, jslot/BLOCK));
+        for (int offset = 32; offset > 0; offset >>= 1)
+            extent = fmax(extent, __shfl_xor(extent, offset, 64));
+        if (lane == 0)
+            tileImage[t] = (block_extent(a.bounds, tile.x)+extent >= a.imageLimit) ? 1 : 0;
+    }
+    else if (lane == 0)
+        tileImage[t] = 0;
     sAtom[lane] = ja;
     if (ja >= 0) {
         sPos[lane][0] = a.pos3[3*ja];
@@ -500,6 +524,14 @@ __global__ void k_candidate_lists(const int2* tiles, const unsigned long long* m
 // Per-evaluation kernels
 // ------------------------------------------------------------------------------------------------------------
 
+// The System index of a slot's particle (the platform's atom index array, or the identity) rides in the fourth
+// component of its block-frame position, refreshed with the positions on every call: the InteraxOrder2 rule needs it,
+// and a platform may reorder its particles between calls.  In FP32 the integer's bits are carried as they are.
+__device__ __forceinline__ float encode_index(int i, float) { return __int_as_float(i); }
+__device__ __forceinline__ double encode_index(int i, double) { return (double) i; }
+__device__ __forceinline__ int decode_index(float w) { return __float_as_int(w); }
+__device__ __forceinline__ int decode_index(double w) { return (int) w; }
+
 // The neighbour-list check of a call, made in the same pass as its block frames (CHECK): flag is set if a particle
 // moved more than sqrt(limit2) since the last build, and the flag of the other frame is cleared for the next call.
 struct ListCheck {
@@ -514,7 +546,8 @@ struct ListCheck {
 // CHECK every particle's displacement is tested once.
 template <typename Real, bool CHECK>
 __global__ void k_local_positions(const void* posq, const float4* correction, int precision, const int* perm, int padded,
-                                  int periodic, BoxD box, double* centers, typename Vec4T<Real>::type* local, ListCheck check) {
+                                  int periodic, BoxD box, double* centers, typename Vec4T<Real>::type* local, ListCheck check,
+                                  const int* atomIndex) {
     int s = blockIdx.x*blockDim.x+threadIdx.x;
     if (CHECK && s == 0)
         *check.otherFlag = 0;
@@ -548,7 +581,7 @@ __global__ void k_local_positions(const void* posq, const float4* correction, in
     double dx = x-cx, dy = y-cy, dz = z-cz;
     if (periodic)
         reference_min_image(dx, dy, dz, box);
-    R4 v = {Real(dx), Real(dy), Real(dz), Real(0)};
+    R4 v = {Real(dx), Real(dy), Real(dz), encode_index(atomIndex != nullptr ? atomIndex[i] : i, Real(0))};
     local[s] = v;
 }
 
@@ -562,12 +595,13 @@ struct TileArgs {
     const unsigned long long* masks;
     const int* listOffsets;  // candidate list of tile t: lists[listOffsets[t] .. listOffsets[t+1]), padded to 4 entries
     const unsigned short* lists;
+    const int* perm;         // slot -> particle (the platform's current order)
     const R4* local;
     const double* centers;
     const R4* pA;    // m1, m2, m3, alpha
     const R4* pB;    // dispersion width, exchange amplitude, exchange width, 1/exchange width
     const R4* pC;    // k, l, short-range width, 1/short-range width
-    const int* flags;   // original index | alchemical bit, -1 for padding
+    const int* flags;   // particle index (current order) | alchemical bit, -1 for padding; in LDS the System index
     // type tables (FP32 only): pA then holds wd, amp, K/2, L/2 and cls the class indices of each slot
     const int* cls;
     const float* tables;
@@ -579,7 +613,6 @@ struct TileArgs {
     // geometry
     int method;
     int periodic;
-    int pairImage;   // the minimum image of each pair must be taken (small boxes)
     BoxD box;
     Real boxR[9];   // ax, bx, by, cx, cy, cz, 1/ax, 1/by, 1/cz
     double boxInv[3];   // 1/ax, 1/by, 1/cz
@@ -661,9 +694,15 @@ struct ParticleType<float, true> {
     typedef TableParticle type;
 };
 
-// Stage the parameters of a slot whose position is already in LDS.
+// LDS flags of a slot: its System index (orig) with the alchemical bit of its flags, or -1 for padding.
+__device__ __forceinline__ int lds_flags(int flags, int orig) {
+    return (flags < 0 ? -1 : (flags & ALCHEMICAL_BIT) | orig);
+}
+
+// Stage the parameters of a slot whose position is already in LDS; orig is its System index.  Returns the slot's
+// flags (particle index in the current order, for the force buffer).
 template <typename Real, bool TABLES>
-__device__ __forceinline__ void stage_particle(SideLds<Real, TABLES>& s, int lane, const TileArgs<Real>& a, int slot) {
+__device__ __forceinline__ int stage_particle(SideLds<Real, TABLES>& s, int lane, const TileArgs<Real>& a, int slot, int orig) {
     if constexpr (TABLES) {
         s.wd[lane] = a.pA[slot].x;
         s.pw[lane] = __int_as_float(a.cls[slot]);
@@ -673,7 +712,9 @@ __device__ __forceinline__ void stage_particle(SideLds<Real, TABLES>& s, int lan
         s.B[lane] = a.pB[slot];
         s.C[lane] = a.pC[slot];
     }
-    s.flags[lane] = a.flags[slot];
+    const int flags = a.flags[slot];
+    s.flags[lane] = lds_flags(flags, orig);
+    return flags;
 }
 
 __device__ __forceinline__ void fetch_particle(const SideLds<float, true>& s, int l, TableParticle& P) {
@@ -772,17 +813,17 @@ __device__ __forceinline__ void load_particle(const TileArgs<Real>& a, int slot,
     P.k = C.x; P.l = C.y; P.ws = C.z; P.iws = C.w;
 }
 
-// Load the partner chunk of a tile into LDS, in the frame of block I.
+// Load the partner chunk of a tile into LDS, in the frame of block I.  Returns the partner's flags (-1 for none).
 template <typename Real, bool TABLES>
-__device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 tile, int lane, SideLds<Real, TABLES>& s) {
+__device__ __forceinline__ int stage_partners(const TileArgs<Real>& a, int2 tile, int lane, SideLds<Real, TABLES>& s) {
     typedef typename Vec4T<Real>::type R4;
-    const int I = tile.x;
+    const int I = tile.x & TILE_BLOCK_MASK;
     const int slot = (tile.y < 0 ? I*BLOCK+lane : a.partners[tile.y+lane]);
     if (slot < 0) {
         R4 zero = {0, 0, 0, 0};
         store_pos(s, lane, zero);
         s.flags[lane] = -1;
-        return;
+        return -1;
     }
     const int J = slot/BLOCK;
     R4 pj = a.local[slot];
@@ -795,7 +836,7 @@ __device__ __forceinline__ void stage_partners(const TileArgs<Real>& a, int2 til
         pj.z += Real(sz);
     }
     store_pos(s, lane, pj);
-    stage_particle<Real, TABLES>(s, lane, a, slot);
+    return stage_particle<Real, TABLES>(s, lane, a, slot, decode_index(pj.w));
 }
 
 // Energy and pair count of a wave, written once at its end: the launch's totals are then summed in wave order.
@@ -839,7 +880,7 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
     for (long long item = begin; item < end; item++) {
         const long long t = (a.tileIndex != nullptr ? a.tileIndex[item] : item);
         const int2 tile = a.tiles[t];
-        const int I = tile.x;
+        const int I = tile.x & TILE_BLOCK_MASK;
         if (I != curI) {
             if (curI >= 0 && flagI >= 0) {
                 int oi = flagI & INDEX_MASK;
@@ -854,13 +895,13 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
             flagI = a.flags[slot];
             load_particle(a, slot, P);
         }
-        stage_partners<Real, TABLES>(a, tile, lane, s);
+        const int flagsJ = stage_partners<Real, TABLES>(a, tile, lane, s);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const unsigned long long mrow = a.masks[(size_t) t*BLOCK+lane];
         const bool diag = (tile.y < 0);
         const int sBegin = (diag ? 1 : 0), sEnd = (diag ? 33 : 64);
         const bool alchI = (flagI >= 0 && ((flagI >> 30) & 1) != 0);
-        const int origI = flagI & INDEX_MASK;
+        const int origI = decode_index(pi.w);
         Real fi[3] = {0, 0, 0};
         Real fj[3] = {0, 0, 0};
         double tileE = 0;
@@ -890,10 +931,10 @@ __global__ void __launch_bounds__(256) k_pair_rotate(TileArgs<Real> a) {
             fj[1] = wave_rotate_down(fj[1]);
             fj[2] = wave_rotate_down(fj[2]);
         }
-        // lane l now holds the accumulator of j = (l + sEnd) % 64
+        // lane l now holds the accumulator of j = (l + sEnd) % 64, whose flags lane jl holds
         {
             const int jl = (lane+sEnd) & 63;
-            const int flagJ = s.flags[jl];
+            const int flagJ = __shfl(flagsJ, jl, 64);
             if (flagJ >= 0) {
                 int oj = flagJ & INDEX_MASK;
                 add_fixed(&a.force[oj], to_fixed(fj[0]));
@@ -937,7 +978,7 @@ struct QueueLds {
 
 template <typename Real, int TERMS, bool TABLES, bool ALCH>
 __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const PairTables& tb, QueueLds<Real, TABLES>& q, int head,
-                                               int valid, int lane, double& energy) {
+                                               int valid, int lane, bool pairImage, double& energy) {
     typedef typename Vec4T<Real>::type R4;
     typedef typename ParticleType<Real, TABLES>::type Particle;
     if (lane >= valid)
@@ -946,7 +987,7 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
     const int il = (int) (e >> 6), jl = (int) (e & 63);
     const R4 pi = load_pos(q.i, il), pj = load_pos(q.j, jl);
     Real dx = pj.x-pi.x, dy = pj.y-pi.y, dz = pj.z-pi.z;
-    if (a.pairImage)
+    if (pairImage)
         min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6], a.boxR[7],
                         a.boxR[8]);
     const Real r2 = dx*dx+dy*dy+dz*dz;
@@ -1029,7 +1070,7 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
         s.C[lane] = f.C;
     }
     store_pos(s, lane, pj);
-    s.flags[lane] = f.flags;
+    s.flags[lane] = lds_flags(f.flags, decode_index(f.pos.w));
 }
 
 // With type tables the kernel is held to 5 waves per SIMD (at most 102 VGPRs; a few values spill): the pair batches
@@ -1067,7 +1108,8 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
         const int2 tile = a.tiles[t];
         PartnerFetch<Real, TABLES> f;
         fetch_partner<Real, TABLES>(a, a.tileSlots[(size_t) t*BLOCK+lane], f);
-        const int I = tile.x;
+        const int I = tile.x & TILE_BLOCK_MASK;
+        const bool pairImage = (tile.x & TILE_PAIR_IMAGE) != 0;
         if (I != curI) {
             if (curI >= 0 && flagI >= 0) {
                 int oi = flagI & INDEX_MASK;
@@ -1083,7 +1125,7 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
             cIy = a.centers[3*I+1];
             cIz = a.centers[3*I+2];
             store_pos(q.i, lane, pi);
-            stage_particle<Real, TABLES>(q.i, lane, a, slot);
+            stage_particle<Real, TABLES>(q.i, lane, a, slot, decode_index(pi.w));
             q.fI[0][lane] = q.fI[1][lane] = q.fI[2][lane] = 0;
         }
         store_partner<Real, TABLES>(a, f, I, cIx, cIy, cIz, lane, q.j);
@@ -1115,13 +1157,13 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
                     const int il = (int) (e >> 6), jl = (int) (e & 63);
                     const R4 pa = load_pos(q.i, il), pb = load_pos(q.j, jl);
                     Real dx = pb.x-pa.x, dy = pb.y-pa.y, dz = pb.z-pa.z;
-                    if (a.pairImage)
+                    if (pairImage)
                         min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6],
                                         a.boxR[7], a.boxR[8]);
                     const Real r2 = dx*dx+dy*dy+dz*dz;
                     if (r2 >= inner)
                         in = (r2 < outer) && exact_within_cutoff(a.posq, a.correction, a.precision, a.periodic, a.box, a.cutoff,
-                                                                 q.i.flags[il] & INDEX_MASK, q.j.flags[jl] & INDEX_MASK);
+                                                                 a.perm[I*BLOCK+il], a.perm[a.tileSlots[(size_t) t*BLOCK+jl]]);
                 }
                 const unsigned long long ballot = __ballot(in);
                 if (in)
@@ -1130,7 +1172,7 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
             }
             while (count >= BLOCK) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, BLOCK, lane, energy);
+                evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, BLOCK, lane, pairImage, energy);
                 pairs += BLOCK;
                 head = (head+BLOCK) & (QUEUE-1);
                 count -= BLOCK;
@@ -1140,12 +1182,12 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
         }
         if (count > 0) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, count, lane, energy);
+            evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, count, lane, pairImage, energy);
             pairs += count;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         {
-            const int flagJ = q.j.flags[lane];
+            const int flagJ = f.flags;
             if (flagJ >= 0) {
                 int oj = flagJ & INDEX_MASK;
                 add_fixed(&a.force[oj], -(long long) q.fJ[0][lane]);
@@ -1371,7 +1413,8 @@ struct LambdaArgs {
     int precision;
     int n;
     const double* params;       // 12 per particle, in PairParticle order, original particle order
-    const int* flags;           // original index | alchemical bit
+    const int* flags;           // particle index | alchemical bit
+    const int* atomIndex;       // System index of each particle (the platform's atom index array), or nullptr
     const int* alchList;        // the alchemical particles
     const int* exclusionOffsets;
     const int* exclusionPartners;
@@ -1401,6 +1444,7 @@ __global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, int chunks
     const int i = a.alchList[blockIdx.x/chunks];
     const int first = (blockIdx.x%chunks)*LAMBDA_CHUNK;
     const int lane = threadIdx.x;
+    const int origI = (a.atomIndex != nullptr ? a.atomIndex[i] : i);
     double xi, yi, zi;
     load_position(a.posq, a.correction, a.precision, i, xi, yi, zi);
     int members = 0;
@@ -1451,9 +1495,10 @@ __global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, int chunks
             const Real x = (r-Real(a.switchDistance))*Real(a.invSwitchWidth);
             sw = 1+x*x*x*(Real(-10)+x*(Real(15)-x*Real(6)));
         }
+        const bool iFirst = origI < (a.atomIndex != nullptr ? a.atomIndex[j] : j);
         Real eHard = 0, dEdR;
         if (a.softcoreR6 == 0.0)
-            pair_energy<Real, TERMS>(r, Real(1)/r, P, Q, i < j, a.settings, eHard, dEdR);
+            pair_energy<Real, TERMS>(r, Real(1)/r, P, Q, iFirst, a.settings, eHard, dEdR);
 #pragma unroll
         for (int k = 0; k <= MAX_LAMBDA_WINDOWS; k++) {
             if (k >= a.count)
@@ -1462,7 +1507,7 @@ __global__ void __launch_bounds__(64) k_lambda_energies(LambdaArgs a, int chunks
             Real e = eHard;
             if (a.softcoreR6 != 0.0) {
                 const Real rEff = pow(r*r*r*r*r*r+(Real(1)-lambda)*Real(a.softcoreR6), Real(1.0/6.0));
-                pair_energy<Real, TERMS>(rEff, Real(1)/rEff, P, Q, i < j, a.settings, e, dEdR);
+                pair_energy<Real, TERMS>(rEff, Real(1)/rEff, P, Q, iFirst, a.settings, e, dEdR);
             }
             acc[k] += (double) (lambda*(e*sw));
         }
@@ -1525,7 +1570,7 @@ struct ixh_pair {
     int numBlocks = 0, padded = 0;
     bool built = false;
     int numTiles = 0;
-    int pairImage = 0;
+    int pairImageTiles = 0;   // tiles whose pairs take their own minimum image
     // parameters in particle order and in slot order
     DeviceArray<char> paramA, paramB, paramC, sortedA, sortedB, sortedC;
     // block frames (local positions and block centres) of two consecutive calls: with a cutoff, a call's list check
@@ -1535,6 +1580,7 @@ struct ixh_pair {
     hipEvent_t frameDone[2] = {nullptr, nullptr};   // the end of the last call that used each frame
     hipStream_t frameStream[2] = {nullptr, nullptr};   // the stream of that call
     int frame = 0;
+    DeviceArray<int> tileImage;
     DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionOffsets, exclusionPartners, blockStart, blockCount, blockList,
         partnerCount, partnerStart, partners, waveCount, tileSlots, tileCandidates, listOffsets;
     DeviceArray<unsigned short> lists;
@@ -1995,7 +2041,7 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         std::vector<int> partners(p->padded);
         std::iota(partners.begin(), partners.end(), 0);
         p->partners.upload(partners.data(), partners.size(), stream);
-        p->pairImage = 0;
+        p->pairImageTiles = 0;
         // launch order: tiles with the most particle pairs first
         std::vector<int> valid(p->padded/BLOCK, 0);
         for (int slot = 0; slot < p->padded; slot++)
@@ -2121,13 +2167,6 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
             for (int off = pstart[I]; off < pstart[I+1]; off += BLOCK)
                 tiles.push_back(make_int2(I, off));
         }
-        // Pairs need their own minimum image unless every pair in range is the nearest image of its block shift:
-        // cutoff + the block extents (which grow by at most the skin before a rebuild) below half the box.
-        double extent = 0;
-        for (int I = 0; I < Nb; I++)
-            extent = std::max(extent, 2*std::sqrt(bounds[6*I+3]*bounds[6*I+3]+bounds[6*I+4]*bounds[6*I+4]+bounds[6*I+5]*bounds[6*I+5]));
-        double width = std::min(b->box[0], std::min(b->box[4], b->box[8]));
-        p->pairImage = (periodic && p->cutoff+2*(extent+SKIN) >= 0.5*width) ? 1 : 0;
     }
     p->numTiles = (int) tiles.size();
     p->tiles.upload(tiles.data(), tiles.size(), stream);
@@ -2145,8 +2184,13 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     m.periodic = periodic;
     m.box = box;
     m.listCutoff2 = listCutoff*listCutoff;
+    m.bounds = p->bounds.ptr;
+    // Pairs need their own minimum image unless every pair in range is the nearest image of its block shift: cutoff +
+    // the extents of both blocks (each grows by at most the skin before a rebuild) below half the box (per tile)
+    m.imageLimit = 0.5*std::min(b->box[0], std::min(b->box[4], b->box[8]))-p->cutoff-2*SKIN;
+    p->tileImage.reserve(std::max(T, 1));
     if (T > 0) {
-        hipLaunchKernelGGL(k_build_masks, dim3(T), dim3(BLOCK), 0, stream, m, p->masks.ptr, p->tileSlots.ptr);
+        hipLaunchKernelGGL(k_build_masks, dim3(T), dim3(BLOCK), 0, stream, m, p->masks.ptr, p->tileSlots.ptr, p->tileImage.ptr);
         IXH_HIP_CHECK(hipGetLastError());
     }
     p->numCandidates = 0;
@@ -2156,8 +2200,9 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         hipLaunchKernelGGL(k_candidate_lists, dim3(T), dim3(BLOCK), 0, stream, p->tiles.ptr, p->masks.ptr, p->tileCandidates.ptr,
                            (const int*) nullptr, (unsigned short*) nullptr);
         IXH_HIP_CHECK(hipGetLastError());
-        std::vector<int> unsortedCounts(T), counts(T), offsets(T+1, 0);
+        std::vector<int> unsortedCounts(T), counts(T), offsets(T+1, 0), image(T);
         IXH_HIP_CHECK(hipMemcpyAsync(unsortedCounts.data(), p->tileCandidates.ptr, sizeof(int)*T, hipMemcpyDeviceToHost, stream));
+        IXH_HIP_CHECK(hipMemcpyAsync(image.data(), p->tileImage.ptr, sizeof(int)*T, hipMemcpyDeviceToHost, stream));
         IXH_HIP_CHECK(hipStreamSynchronize(stream));
         {
             // the tiles in launch order, decreasing candidate pairs (the workgroups dispatched last are the lightest),
@@ -2167,9 +2212,14 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
                 order[t] = t;
             std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return unsortedCounts[x] > unsortedCounts[y]; });
             std::vector<int2> sortedTiles(T);
+            p->pairImageTiles = 0;
             for (int t = 0; t < T; t++) {
                 sortedTiles[t] = tiles[order[t]];
                 counts[t] = unsortedCounts[order[t]];
+                if (image[order[t]]) {
+                    sortedTiles[t].x |= TILE_PAIR_IMAGE;
+                    p->pairImageTiles++;
+                }
             }
             p->tileOrder.upload(order.data(), order.size(), stream);
             DeviceArray<unsigned long long> masks;
@@ -2234,6 +2284,7 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
     a.masks = p->masks.ptr;
     a.listOffsets = p->listOffsets.ptr;
     a.lists = p->lists.ptr;
+    a.perm = p->perm.ptr;
     a.local = (const R4*) p->localFrame[p->frame].ptr;
     a.centers = p->centerFrame[p->frame].ptr;
     a.pA = (const R4*) p->sortedA.ptr;
@@ -2243,7 +2294,6 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
     a.n = p->n;
     a.method = p->method;
     a.periodic = (p->method == IXH_CUTOFF_PERIODIC);
-    a.pairImage = p->pairImage;
     a.box = make_box(b->box);
     a.boxR[0] = Real(b->box[0]);
     a.boxR[1] = Real(b->box[3]);
@@ -2569,11 +2619,11 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
             if (p->precision == IXH_DOUBLE)
                 launch_bound(k_local_positions<double, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
                              b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
-                             p->centerFrame[f].ptr, (double4*) p->localFrame[f].ptr, check);
+                             p->centerFrame[f].ptr, (double4*) p->localFrame[f].ptr, check, b->atom_index);
             else
                 launch_bound(k_local_positions<float, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
                              b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
-                             p->centerFrame[f].ptr, (float4*) p->localFrame[f].ptr, check);
+                             p->centerFrame[f].ptr, (float4*) p->localFrame[f].ptr, check, b->atom_index);
             IXH_HIP_CHECK(hipGetLastError());
             if (defer)
                 p->deferredCheck = true;
@@ -2643,7 +2693,7 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
             if (!framed)
                 hipLaunchKernelGGL((k_local_positions<double, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
                                    corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
-                                   (double4*) p->localFrame[p->frame].ptr, none);
+                                   (double4*) p->localFrame[p->frame].ptr, none, b->atom_index);
             IXH_HIP_CHECK(hipGetLastError());
             marked = launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
                                           subsetTileCount, std::max(numSelected, 0));
@@ -2652,7 +2702,7 @@ int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int fl
             if (!framed)
                 hipLaunchKernelGGL((k_local_positions<float, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
                                    corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
-                                   (float4*) p->localFrame[p->frame].ptr, none);
+                                   (float4*) p->localFrame[p->frame].ptr, none, b->atom_index);
             IXH_HIP_CHECK(hipGetLastError());
             marked = launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
                                          subsetTileCount, std::max(numSelected, 0));
@@ -2753,6 +2803,7 @@ int ixh_pair_lambda_energies(ixh_pair* p, const ixh_buffers* b, int terms, const
             a.n = p->n;
             a.params = p->lambdaParams.ptr;
             a.flags = p->flags.ptr;
+            a.atomIndex = b->atom_index;
             a.alchList = p->alchList.ptr;
             a.exclusionOffsets = p->exclusionOffsets.ptr;
             a.exclusionPartners = p->exclusionPartners.ptr;
@@ -2853,6 +2904,19 @@ int ixh_pair_last_pair_count(ixh_pair* p, long long* count) {
         DeviceGuard guard(p->device);
         IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
         IXH_HIP_CHECK(hipMemcpy(count, p->pairCount.ptr, sizeof(long long), hipMemcpyDeviceToHost));
+    });
+}
+
+int ixh_pair_image_mode(ixh_pair* p, long long* per_block_tiles, long long* tiles) {
+    return guarded([&] {
+        if (p == nullptr || per_block_tiles == nullptr || tiles == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_image_mode: null argument");
+        DeviceGuard guard(p->device);
+        if (p->lastStream != nullptr)
+            IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
+        const bool periodic = (p->built && p->method == IXH_CUTOFF_PERIODIC);
+        *tiles = p->numTiles;
+        *per_block_tiles = (periodic ? p->numTiles-p->pairImageTiles : 0);
     });
 }
 

@@ -258,6 +258,7 @@ int ixh_state_get_buffers(ixh_state* s, ixh_buffers* out) {
         std::memcpy(out->box, s->box, sizeof(s->box));
         out->lambda = s->lambda;
         out->stream = s->stream;
+        out->atom_index = nullptr;
     });
 }
 

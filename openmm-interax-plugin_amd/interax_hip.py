@@ -25,6 +25,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(_HERE, "libinterax_hip.so")
 
 IXH_OK = 0
+IXH_ERROR_INVALID_ARGUMENT, IXH_ERROR_HIP, IXH_ERROR_STATE, IXH_ERROR_UNSUPPORTED, IXH_ERROR_CAPACITY = 1, 2, 3, 4, 5
 IXH_SINGLE, IXH_MIXED, IXH_DOUBLE = 0, 1, 2
 IXH_TERM_DISPERSION, IXH_TERM_EXCHANGE, IXH_TERM_SHORT_RANGE = 1, 2, 4
 IXH_INCLUDE_FORCES, IXH_INCLUDE_ENERGY = 1, 2
@@ -49,7 +50,7 @@ class Buffers(ctypes.Structure):
     _fields_ = [("num_particles", ctypes.c_int), ("precision", ctypes.c_int), ("posq", ctypes.c_void_p),
                 ("posq_correction", ctypes.c_void_p), ("force", ctypes.c_void_p), ("force_stride", ctypes.c_int),
                 ("energy", ctypes.c_void_p), ("box", ctypes.c_double*9), ("lambda_", ctypes.c_double),
-                ("stream", ctypes.c_void_p)]
+                ("stream", ctypes.c_void_p), ("atom_index", ctypes.c_void_p)]
 
 
 class PairDesc(ctypes.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "ixh_pair_long_range_energy": [ctypes.c_void_p, _dp, ctypes.c_double, _dp],
     "ixh_pair_lambda_energies": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp],
     "ixh_pair_last_pair_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
+    "ixh_pair_image_mode": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "ixh_pair_list_statistics": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "ixh_threebody_create": [ctypes.c_int, ctypes.POINTER(ThreeBodyDesc), ctypes.POINTER(ctypes.c_void_p)],
     "ixh_threebody_destroy": [ctypes.c_void_p],
@@ -120,6 +122,8 @@ SIGNATURES = {
     "ixh_batch_create": [ctypes.c_int, ctypes.POINTER(BatchDesc), ctypes.POINTER(ctypes.c_void_p)],
     "ixh_batch_destroy": [ctypes.c_void_p],
     "ixh_batch_set_positions": [ctypes.c_void_p, _dp],
+    "ixh_batch_kernel_time": [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong), _dp, ctypes.POINTER(ctypes.c_longlong)],
+    "ixh_batch_reset_kernel_time": [ctypes.c_void_p],
     "ixh_batch_get_buffers": [ctypes.c_void_p, ctypes.POINTER(Buffers)],
     "ixh_batch_execute": [ctypes.c_void_p, ctypes.c_int],
     "ixh_batch_get_energies": [ctypes.c_void_p, _dp],
@@ -790,6 +794,7 @@ class Context:
         self.device = int(properties.get("DeviceIndex", "0"))
         n = system.getNumParticles()
         self._parameters: Dict[str, float] = {}
+        self._atom_index: Optional[int] = None
         self._groups: List[_PairGroup] = []
         self._threebody: List[_ThreeBody] = []
         self._state = ctypes.c_void_p()
@@ -859,7 +864,14 @@ class Context:
         b = Buffers()
         _check(_lib.ixh_state_get_buffers(self._state, ctypes.byref(b)))
         b.lambda_ = lam
+        b.atom_index = self._atom_index
         return b
+
+    def _set_atom_index(self, device_pointer: Optional[int]) -> None:
+        """Run as a platform that has reordered its particles: positions and forces are in that platform's current
+        order and device_pointer (an int device array, or None for the identity) holds the System index of every slot,
+        as OpenMM's ComputeContext::getAtomIndexArray.  For adapters and tests of reordered contexts."""
+        self._atom_index = device_pointer
 
     def getState(self, getEnergy: bool = True, getForces: bool = True, groups: int = 0xFFFFFFFF) -> State:
         _check(_lib.ixh_state_clear(self._state))

@@ -223,10 +223,11 @@ SOLUTE_C6, SOLUTE_QV, SOLUTE_SLATER = 1.8e-3, -4.0, 1.0/26
 
 
 def alchemical_box(softcore_radius: float = 0.0, seed: int = 0, num_waters: int = 1000, box: float = 3.106,
-                   cutoff: float = 0.9):
+                   cutoff: float = 0.9, use_lrc: bool = False):
     """An alchemical job of the served configuration: a rigid 12-atom solute, every atom alchemical (its 66
     intramolecular pairs excluded), at the centre of a periodic cube with water-like molecules (RSA-placed oxygens,
-    at least 0.25 nm apart and 0.3 nm from the solute), cutoff 0.9 nm, soft-core radius R."""
+    at least 0.25 nm apart and 0.3 nm from the solute), cutoff 0.9 nm, soft-core radius R, optionally the dispersion
+    long range correction (whose alchemical part scales with lambda)."""
     rng = np.random.default_rng(seed)
     angles = np.arange(6)*np.pi/3
     ring = np.stack([np.cos(angles), np.sin(angles), np.zeros(6)], -1)
@@ -275,4 +276,5 @@ def alchemical_box(softcore_radius: float = 0.0, seed: int = 0, num_waters: int 
             f.addExclusion(o+1, o+2)
         system.addForce(f)
     disp.setDampingScale(1.0)
+    disp.setUseLongRangeCorrection(use_lrc)
     return system, positions

@@ -92,3 +92,25 @@ def test_buffers_untouched_and_arguments_checked():
         status = ih._lib.ixh_pair_lambda_energies(group.handle, ctypes.byref(b), ih.IXH_TERM_EXCHANGE, ih._dptr(big), count, ih._dptr(big))
         assert status == 1   # IXH_ERROR_INVALID_ARGUMENT
     context.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("softcore", [0.0, 0.3])
+def test_alchemical_box_against_oracle(softcore):
+    """The alchemical job at its full size (12-atom solute in 1000 water-like molecules, 3012 atoms, 3.106 nm) with the
+    long range correction, whose alchemical part scales with lambda: every fused window energy against the FP64
+    oracle at that lambda (pair terms with the correction left out, plus the oracle's correction coefficients)."""
+    system, pos = workloads.alchemical_box(softcore, seed=2, use_lrc=True)
+    box = system.getDefaultPeriodicBoxVectors()
+    disp = system.getForce(0)
+    coef, alch = oracle.long_range_correction(disp)
+    assert alch != 0.0
+    context = ih.Context(system, {"Precision": "mixed"})
+    context.setPositions(pos)
+    fused = context.getEnergiesAtLambdas(VALUES)
+    context.close()
+    disp.setUseLongRangeCorrection(False)
+    volume = box[0, 0]*box[1, 1]*box[2, 2]
+    for v, e in zip(VALUES, fused):
+        ref = oracle.evaluate(system, pos, box, parameters={"lambda_sterics": float(v)})[0]+(coef+v*alch)/volume
+        assert abs(e-ref) <= 2e-6*abs(ref), (v, e, ref)
