@@ -186,6 +186,23 @@ int ixh_pair_set_parameters(ixh_pair* pair, const ixh_pair_desc* desc);
  * ixh_pair_long_range_energy(), as the GPU platforms return it from execute().  Asynchronous on buffers->stream.
  */
 int ixh_pair_execute(ixh_pair* pair, const ixh_buffers* buffers, int terms, int flags);
+/*
+ * ixh_pair_execute without any allocation (no counterpart in the reference; for callers that cannot allocate: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same kernels and launch as
+ * ixh_pair_execute, with the same results bit for bit; the per-workgroup energy and pair-count partials of the launch
+ * go to a static device array of the library (65 536 partials, 768 KiB per GPU, held until the process ends) instead of
+ * the handle's buffers, and the call creates no memory, event or stream.  Limits:
+ *   - the handle's neighbour list must exist: the first evaluation of a handle is made by ixh_pair_execute
+ *     (IXH_ERROR_STATE otherwise);
+ *   - a cutoff handle whose list no longer holds for the positions returns IXH_ERROR_STATE without evaluating (without
+ *     IXH_DEFER_LIST_CHECK; with it, ixh_pair_list_was_valid tells as for ixh_pair_execute); ixh_pair_execute then
+ *     rebuilds the list;
+ *   - lists that need more partials than the array holds (more than 262 144 tiles with a cutoff, 65 536 without)
+ *     return IXH_ERROR_CAPACITY; segmented handles return IXH_ERROR_UNSUPPORTED.
+ * Every call on one GPU shares that GPU's array (each launch writes all the partials it reads, so nothing needs
+ * clearing): calls that overlap on one GPU, from other threads or on other streams, race.
+ */
+int ixh_pair_execute_noalloc(ixh_pair* pair, const ixh_buffers* buffers, int terms, int flags);
 /* Dispersion long range correction: coefficients (InteraxDispersionForceImpl.cpp:86-149) and the energy
    (coefficient + lambda*alchemicalCoefficient)/V (ReferenceInteraxKernels.cpp:93-94). */
 int ixh_pair_long_range_coefficients(ixh_pair* pair, double* coefficient, double* alchemical_coefficient);

@@ -32,6 +32,7 @@
 
 #include <rocprim/rocprim.hpp>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <tuple>
 #include <vector>
@@ -1544,6 +1545,40 @@ __global__ void k_zero_fixed(long long* p, long long n, double* energy) {
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------------------
+// Static scratch of the evaluations that allocate nothing (ixh_pair_execute_noalloc): the per-workgroup energy and
+// pair-count partials of a tile launch, STATIC_PARTIALS of each.  Every GPU the library runs on holds its own copy
+// until the process ends; host code reaches it only through the address hipGetSymbolAddress gives on that device.
+// ------------------------------------------------------------------------------------------------------------
+
+namespace {
+constexpr int STATIC_PARTIALS = 65536;
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t) STATIC_PARTIALS*(sizeof(double)+sizeof(int));
+constexpr int MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+namespace {
+std::mutex scratchMutex;
+void* scratchAddress[MAX_DEVICES] = {};
+
+// The device address of the static scratch on `device`, which must be the current device (resolved once per device).
+void* static_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair: device index out of range");
+    std::lock_guard<std::mutex> lock(scratchMutex);
+    if (scratchAddress[device] == nullptr) {
+        void* ptr = nullptr;
+        IXH_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr)
+            throw Error(IXH_ERROR_HIP, "ixh_pair: the static scratch has no device address");
+        scratchAddress[device] = ptr;
+    }
+    return scratchAddress[device];
+}
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------------
 // The handle
 // ------------------------------------------------------------------------------------------------------------
 
@@ -2255,6 +2290,11 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         p->buildPos.reserve(3*(size_t) n);
         IXH_HIP_CHECK(hipMemcpyAsync(p->buildPos.ptr, p->pos3.ptr, sizeof(double)*3*(size_t) n, hipMemcpyDeviceToDevice, stream));
     }
+    // both block frames sized with the list: evaluations between builds allocate nothing
+    for (int f = 0; f < 2; f++) {
+        p->localFrame[f].reserve(r4*p->padded);
+        p->centerFrame[f].reserve(3*(size_t) p->numBlocks);
+    }
     IXH_HIP_CHECK(hipStreamSynchronize(stream));
     p->built = true;
 }
@@ -2270,11 +2310,25 @@ void launch_bound(Kernel kernel, dim3 grid, dim3 block, size_t shared, hipStream
         hipLaunchKernelGGL(kernel, grid, block, shared, stream, args...);
 }
 
+// Workgroups of a tile launch over numTiles tiles.
+int launch_workgroups(const ixh_pair* p, int numTiles) {
+    const int groupWaves = (p->method != IXH_NO_CUTOFF ? QUEUE_GROUP_WAVES : 4);
+    return std::max(1, std::min((numTiles+groupWaves-1)/groupWaves, p->gridLimit));
+}
+
+// Energy and pair-count partials of a launch over all of the handle's tiles: one per workgroup (cutoff tiles) or per
+// wave (all-pairs tiles).
+long long launch_partials(const ixh_pair* p) {
+    const long long blocks = launch_workgroups(p, p->numTiles);
+    return (p->method != IXH_NO_CUTOFF ? blocks : 4*blocks);
+}
+
 // The tile kernel of an evaluation and its reductions.  end: an event to bind to the last kernel (nullptr: none).
+// staticScratch: the partial sums go to the static scratch instead of the handle's buffers, and nothing is timed.
 // Returns the event that marks the end of the work (end), or nullptr if none was bound.
 template <typename Real>
 hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEnergy, double* segmentEnergies, hipStream_t stream,
-                        hipEvent_t end, int subsetTiles = -1, int subsetSegments = 0) {
+                        hipEvent_t end, int subsetTiles, int subsetSegments, void* staticScratch) {
     typedef typename Vec4T<Real>::type R4;
     TileArgs<Real> a;
     a.tiles = p->tiles.ptr;
@@ -2342,26 +2396,32 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
         a.tileIndex = p->subsetTiles.ptr;
     }
     const bool compact = (p->method != IXH_NO_CUTOFF);
-    const int groupWaves = (compact ? QUEUE_GROUP_WAVES : 4);
-    const int blocks = std::max(1, std::min((a.numTiles+groupWaves-1)/groupWaves, p->gridLimit));
+    const int blocks = launch_workgroups(p, a.numTiles);
     a.tileEnergy = nullptr;
     if (p->numSegments > 0) {
         p->tileEnergy.reserve(std::max(p->numTiles, 1));
         a.tileEnergy = p->tileEnergy.ptr;
     }
-    // partial sums of the launch: one per workgroup (cutoff tiles) or per wave (all-pairs tiles)
+    // partial sums of the launch: one per workgroup (cutoff tiles) or per wave (all-pairs tiles), in the handle's
+    // buffers or in the static scratch (whose capacity the caller has checked: launch_partials)
     const int numWaves = (compact ? blocks : 4*blocks);
-    p->waveEnergy.reserve(numWaves);
-    p->waveCount.reserve(numWaves);
-    a.waveEnergy = p->waveEnergy.ptr;
-    a.waveCount = p->waveCount.ptr;
+    if (staticScratch != nullptr) {
+        a.waveEnergy = (double*) staticScratch;
+        a.waveCount = (int*) ((char*) staticScratch+sizeof(double)*STATIC_PARTIALS);
+    }
+    else {
+        p->waveEnergy.reserve(numWaves);
+        p->waveCount.reserve(numWaves);
+        a.waveEnergy = p->waveEnergy.ptr;
+        a.waveCount = p->waveCount.ptr;
+    }
     const bool segments = (segmentEnergies != nullptr && p->numSegments > 0);
     const int segmentCount = (!segments ? 0 : subsetTiles >= 0 ? subsetSegments : p->numSegments);
     hipEvent_t marked = nullptr;
     if (a.numTiles > 0) {
         // with profiling, events recorded around the kernel (events bound to its dispatch would also count the wait
         // for the previous kernel)
-        const bool timed = p->timer.start(stream);
+        const bool timed = (staticScratch == nullptr && p->timer.start(stream));
         switch (terms) {
 #define IXH_LAUNCH(T, TAB, ALCH) \
             if (compact) \
@@ -2394,7 +2454,7 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
             p->timer.stop(stream);
     }
     launch_bound(k_reduce_waves, dim3(1), dim3(1024), 0, stream, nullptr, segmentCount > 0 ? nullptr : end,
-                 (const double*) p->waveEnergy.ptr, (const int*) p->waveCount.ptr, a.numTiles > 0 ? numWaves : 0, b->energy,
+                 (const double*) a.waveEnergy, (const int*) a.waveCount, a.numTiles > 0 ? numWaves : 0, b->energy,
                  p->pairCount.ptr, addEnergy);
     IXH_HIP_CHECK(hipGetLastError());
     marked = end;
@@ -2406,6 +2466,174 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
         marked = end;
     }
     return marked;
+}
+
+// One evaluation (ixh_pair_execute_subset).  staticScratch != nullptr: the no-allocation form
+// (ixh_pair_execute_noalloc): the list must exist and still hold, the launch's partial sums go to the static scratch,
+// and no device or host memory, event or timer is created.
+void execute_pair(ixh_pair* p, const ixh_buffers* b, int terms, int flags, double* segmentEnergies, const int* segments,
+                  int numSelected, void* staticScratch) {
+    if (numSelected >= 0 && (p == nullptr || p->numSegments == 0 || (numSelected > 0 && segments == nullptr)))
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute_subset: a subset needs a segmented handle and a segment list");
+    if (p == nullptr || b == nullptr)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: null argument");
+    if (b->num_particles != p->n)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: the buffers hold a different number of particles");
+    if (b->precision != p->precision && !(b->precision != IXH_DOUBLE && p->precision != IXH_DOUBLE))
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: the buffers have a different precision");
+    terms &= p->terms;
+    if (terms == 0 || p->n == 0)
+        return;
+    if (p->method == IXH_CUTOFF_PERIODIC) {
+        const double* bx = b->box;
+        if (p->cutoff > 0.5*bx[0] || p->cutoff > 0.5*bx[4] || p->cutoff > 0.5*bx[8])
+            throw Error(IXH_ERROR_INVALID_ARGUMENT,
+                        "InteraxPairForce: The cutoff distance cannot be greater than half the periodic box size.");
+    }
+    const bool noalloc = (staticScratch != nullptr);
+    if (noalloc) {
+        if (p->numSegments > 0 || numSelected >= 0)
+            throw Error(IXH_ERROR_UNSUPPORTED, "ixh_pair_execute_noalloc: segmented handles are not supported");
+        if (!p->built)
+            throw Error(IXH_ERROR_STATE, "ixh_pair_execute_noalloc: the handle has no neighbour list yet (its first "
+                                         "evaluation must be made by ixh_pair_execute)");
+        if (launch_partials(p) > STATIC_PARTIALS)
+            throw Error(IXH_ERROR_CAPACITY, "ixh_pair_execute_noalloc: the neighbour list needs " +
+                                            std::to_string(launch_partials(p)) + " partial sums; the static scratch holds " +
+                                            std::to_string(STATIC_PARTIALS));
+    }
+    DeviceGuard guard(p->device);
+    hipStream_t stream = (hipStream_t) b->stream;
+    const float4* corr = (const float4*) b->posq_correction;
+    bool rebuild = !p->built;
+    const int periodic = (p->method == IXH_CUTOFF_PERIODIC);
+    const size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
+    // The list check reads only this call's positions and the positions of the last build, so it runs before
+    // this call waits for the previous one: its answer is back while the previous tiles still run.  The same
+    // pass writes this call's block frames into the frame the previous call does not read (the call before it,
+    // which used this frame, has finished with it first).
+    const bool deferredBefore = p->deferredCheck;
+    p->deferredCheck = false;
+    const int previousFrame = p->frame;   // frameDone[previousFrame] marks the end of the previous call
+    bool framed = false;   // this call's block frames are written
+    if (!rebuild && p->method != IXH_NO_CUTOFF) {
+        // the previous check, whose flag this one clears, has finished
+        if (deferredBefore && stream != p->lastStream)
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->flagCopied, 0));
+        const int f = p->frame ^ 1;
+        if (p->frameDone[f] != nullptr && p->frameStream[f] != stream)
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[f], 0));
+        if (!noalloc) {
+            p->localFrame[f].reserve(r4*p->padded);
+            p->centerFrame[f].reserve(3*(size_t) p->numBlocks);
+        }
+        p->frame = f;
+        const ListCheck check = {p->buildPos.ptr, 0.25*SKIN*SKIN, p->hostFlag+f, p->hostFlag+(f ^ 1)};
+        p->checkedFrame = f;
+        // a deferred check carries its completion event on its own dispatch
+        const bool defer = (flags & IXH_DEFER_LIST_CHECK) != 0;
+        if (defer && p->flagCopied == nullptr)   // created with the handle; kept for handles made before
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
+        hipEvent_t checked = (defer ? p->flagCopied : nullptr);
+        if (p->precision == IXH_DOUBLE)
+            launch_bound(k_local_positions<double, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
+                         b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
+                         p->centerFrame[f].ptr, (double4*) p->localFrame[f].ptr, check, b->atom_index);
+        else
+            launch_bound(k_local_positions<float, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
+                         b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
+                         p->centerFrame[f].ptr, (float4*) p->localFrame[f].ptr, check, b->atom_index);
+        IXH_HIP_CHECK(hipGetLastError());
+        if (defer)
+            p->deferredCheck = true;
+        else {
+            IXH_HIP_CHECK(hipStreamSynchronize(stream));
+            rebuild = (p->hostFlag[f] != 0);
+        }
+        framed = !rebuild;
+    }
+    // the handle's buffers are reused by every call: a call on another stream waits for the previous one
+    if (p->frameDone[previousFrame] != nullptr && stream != p->lastStream)
+        IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[previousFrame], 0));
+    if (rebuild && noalloc)
+        throw Error(IXH_ERROR_STATE, "ixh_pair_execute_noalloc: the neighbour list no longer holds for these positions "
+                                     "(ixh_pair_execute rebuilds it)");
+    p->lastStream = stream;
+    if (rebuild)
+        build_list(p, b, stream);
+    // without a list check (no cutoff, a new list) the frames are written after the previous call, which may have
+    // used either frame, has finished (build_list sizes both frames)
+    if (!noalloc) {
+        p->localFrame[p->frame].reserve(r4*p->padded);
+        p->centerFrame[p->frame].reserve(3*(size_t) p->numBlocks);
+    }
+    int subsetTileCount = -1;
+    if (numSelected >= 0) {
+        // the tiles of the selected segments, in segment order
+        size_t needed = 0;
+        for (int k = 0; k < numSelected; k++) {
+            const int g = segments[k];
+            if (g < 0 || g >= p->numSegments)
+                throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute_subset: segment index out of range");
+            needed += p->segTiles[g+1]-p->segTiles[g];
+        }
+        needed += numSelected;
+        if (p->subsetCopied == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->subsetCopied, hipEventDisableTiming));
+        else
+            IXH_HIP_CHECK(hipEventSynchronize(p->subsetCopied));   // the staging of the previous call has been read
+        if (needed > p->subsetCapacity) {
+            if (p->subsetStaging != nullptr)
+                IXH_HIP_CHECK(hipHostFree(p->subsetStaging));
+            p->subsetStaging = nullptr;
+            p->subsetCapacity = std::max<size_t>(needed, 1024);
+            IXH_HIP_CHECK(hipHostMalloc((void**) &p->subsetStaging, sizeof(int)*p->subsetCapacity, hipHostMallocDefault));
+        }
+        int* tileList = p->subsetStaging;
+        int count = 0;
+        for (int k = 0; k < numSelected; k++)
+            for (int t = p->segTiles[segments[k]]; t < p->segTiles[segments[k]+1]; t++)
+                tileList[count++] = t;
+        std::copy(segments, segments+numSelected, tileList+count);
+        p->subsetTiles.reserve(std::max(count, 1));
+        p->subsetSegments.reserve(std::max(numSelected, 1));
+        if (count > 0)
+            IXH_HIP_CHECK(hipMemcpyAsync(p->subsetTiles.ptr, tileList, sizeof(int)*count, hipMemcpyHostToDevice, stream));
+        if (numSelected > 0)
+            IXH_HIP_CHECK(hipMemcpyAsync(p->subsetSegments.ptr, tileList+count, sizeof(int)*numSelected, hipMemcpyHostToDevice,
+                                         stream));
+        IXH_HIP_CHECK(hipEventRecord(p->subsetCopied, stream));
+        subsetTileCount = count;
+    }
+    const ListCheck none = {};
+    // the end of the call: an event bound to its last kernel, or recorded after it in a graph capture
+    if (p->frameDone[p->frame] == nullptr)
+        IXH_HIP_CHECK(hipEventCreateWithFlags(&p->frameDone[p->frame], hipEventDisableTiming));
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    IXH_HIP_CHECK(hipStreamIsCapturing(stream, &capture));
+    hipEvent_t end = (capture == hipStreamCaptureStatusNone ? p->frameDone[p->frame] : nullptr);
+    hipEvent_t marked = nullptr;
+    if (p->precision == IXH_DOUBLE) {
+        if (!framed)
+            hipLaunchKernelGGL((k_local_positions<double, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
+                               corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
+                               (double4*) p->localFrame[p->frame].ptr, none, b->atom_index);
+        IXH_HIP_CHECK(hipGetLastError());
+        marked = launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
+                                      subsetTileCount, std::max(numSelected, 0), staticScratch);
+    }
+    else {
+        if (!framed)
+            hipLaunchKernelGGL((k_local_positions<float, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
+                               corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
+                               (float4*) p->localFrame[p->frame].ptr, none, b->atom_index);
+        IXH_HIP_CHECK(hipGetLastError());
+        marked = launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
+                                     subsetTileCount, std::max(numSelected, 0), staticScratch);
+    }
+    if (marked == nullptr)
+        IXH_HIP_CHECK(hipEventRecord(p->frameDone[p->frame], stream));
+    p->frameStream[p->frame] = stream;
 }
 
 }  // namespace
@@ -2499,6 +2727,12 @@ int ixh_pair_create_segmented(int device, const ixh_pair_desc* d, int num_segmen
             p->pairCount.reserve(1);
             IXH_HIP_CHECK(hipHostMalloc((void**) &p->hostFlag, 2*sizeof(int), hipHostMallocCoherent));
             p->hostFlag[0] = p->hostFlag[1] = 0;
+            // the events of every evaluation, and the static scratch of this device, are set up here so that
+            // ixh_pair_execute_noalloc creates nothing
+            for (hipEvent_t& e : p->frameDone)
+                IXH_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
+            (void) static_scratch(device);
             load_parameters(p, d, stream);
             compute_long_range(p, stream);
         }
@@ -2567,149 +2801,19 @@ int ixh_pair_execute_segments(ixh_pair* p, const ixh_buffers* b, int terms, int 
 
 int ixh_pair_execute_subset(ixh_pair* p, const ixh_buffers* b, int terms, int flags, double* segmentEnergies, const int* segments,
                             int numSelected) {
+    return guarded([&] { execute_pair(p, b, terms, flags, segmentEnergies, segments, numSelected, nullptr); });
+}
+
+int ixh_pair_execute_noalloc(ixh_pair* p, const ixh_buffers* b, int terms, int flags) {
     return guarded([&] {
-        if (numSelected >= 0 && (p == nullptr || p->numSegments == 0 || (numSelected > 0 && segments == nullptr)))
-            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute_subset: a subset needs a segmented handle and a segment list");
-        if (p == nullptr || b == nullptr)
-            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: null argument");
-        if (b->num_particles != p->n)
-            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: the buffers hold a different number of particles");
-        if (b->precision != p->precision && !(b->precision != IXH_DOUBLE && p->precision != IXH_DOUBLE))
-            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute: the buffers have a different precision");
-        terms &= p->terms;
-        if (terms == 0 || p->n == 0)
-            return;
-        if (p->method == IXH_CUTOFF_PERIODIC) {
-            const double* bx = b->box;
-            if (p->cutoff > 0.5*bx[0] || p->cutoff > 0.5*bx[4] || p->cutoff > 0.5*bx[8])
-                throw Error(IXH_ERROR_INVALID_ARGUMENT,
-                            "InteraxPairForce: The cutoff distance cannot be greater than half the periodic box size.");
+        if (p == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute_noalloc: null argument");
+        void* address = nullptr;
+        {
+            DeviceGuard guard(p->device);
+            address = static_scratch(p->device);   // resolved when the handle was created
         }
-        DeviceGuard guard(p->device);
-        hipStream_t stream = (hipStream_t) b->stream;
-        const float4* corr = (const float4*) b->posq_correction;
-        bool rebuild = !p->built;
-        const int periodic = (p->method == IXH_CUTOFF_PERIODIC);
-        const size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
-        // The list check reads only this call's positions and the positions of the last build, so it runs before
-        // this call waits for the previous one: its answer is back while the previous tiles still run.  The same
-        // pass writes this call's block frames into the frame the previous call does not read (the call before it,
-        // which used this frame, has finished with it first).
-        const bool deferredBefore = p->deferredCheck;
-        p->deferredCheck = false;
-        const int previousFrame = p->frame;   // frameDone[previousFrame] marks the end of the previous call
-        bool framed = false;   // this call's block frames are written
-        if (!rebuild && p->method != IXH_NO_CUTOFF) {
-            // the previous check, whose flag this one clears, has finished
-            if (deferredBefore && stream != p->lastStream)
-                IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->flagCopied, 0));
-            const int f = p->frame ^ 1;
-            if (p->frameDone[f] != nullptr && p->frameStream[f] != stream)
-                IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[f], 0));
-            p->localFrame[f].reserve(r4*p->padded);
-            p->centerFrame[f].reserve(3*(size_t) p->numBlocks);
-            p->frame = f;
-            const ListCheck check = {p->buildPos.ptr, 0.25*SKIN*SKIN, p->hostFlag+f, p->hostFlag+(f ^ 1)};
-            p->checkedFrame = f;
-            // a deferred check carries its completion event on its own dispatch
-            const bool defer = (flags & IXH_DEFER_LIST_CHECK) != 0;
-            if (defer && p->flagCopied == nullptr)
-                IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
-            hipEvent_t checked = (defer ? p->flagCopied : nullptr);
-            if (p->precision == IXH_DOUBLE)
-                launch_bound(k_local_positions<double, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
-                             b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
-                             p->centerFrame[f].ptr, (double4*) p->localFrame[f].ptr, check, b->atom_index);
-            else
-                launch_bound(k_local_positions<float, true>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, nullptr, checked,
-                             b->posq, corr, b->precision, (const int*) p->perm.ptr, p->padded, periodic, make_box(b->box),
-                             p->centerFrame[f].ptr, (float4*) p->localFrame[f].ptr, check, b->atom_index);
-            IXH_HIP_CHECK(hipGetLastError());
-            if (defer)
-                p->deferredCheck = true;
-            else {
-                IXH_HIP_CHECK(hipStreamSynchronize(stream));
-                rebuild = (p->hostFlag[f] != 0);
-            }
-            framed = !rebuild;
-        }
-        // the handle's buffers are reused by every call: a call on another stream waits for the previous one
-        if (p->frameDone[previousFrame] != nullptr && stream != p->lastStream)
-            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[previousFrame], 0));
-        p->lastStream = stream;
-        if (rebuild)
-            build_list(p, b, stream);
-        // without a list check (no cutoff, a new list) the frames are written after the previous call, which may have
-        // used either frame, has finished
-        p->localFrame[p->frame].reserve(r4*p->padded);
-        p->centerFrame[p->frame].reserve(3*(size_t) p->numBlocks);
-        int subsetTileCount = -1;
-        if (numSelected >= 0) {
-            // the tiles of the selected segments, in segment order
-            size_t needed = 0;
-            for (int k = 0; k < numSelected; k++) {
-                const int g = segments[k];
-                if (g < 0 || g >= p->numSegments)
-                    throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_execute_subset: segment index out of range");
-                needed += p->segTiles[g+1]-p->segTiles[g];
-            }
-            needed += numSelected;
-            if (p->subsetCopied == nullptr)
-                IXH_HIP_CHECK(hipEventCreateWithFlags(&p->subsetCopied, hipEventDisableTiming));
-            else
-                IXH_HIP_CHECK(hipEventSynchronize(p->subsetCopied));   // the staging of the previous call has been read
-            if (needed > p->subsetCapacity) {
-                if (p->subsetStaging != nullptr)
-                    IXH_HIP_CHECK(hipHostFree(p->subsetStaging));
-                p->subsetStaging = nullptr;
-                p->subsetCapacity = std::max<size_t>(needed, 1024);
-                IXH_HIP_CHECK(hipHostMalloc((void**) &p->subsetStaging, sizeof(int)*p->subsetCapacity, hipHostMallocDefault));
-            }
-            int* tileList = p->subsetStaging;
-            int count = 0;
-            for (int k = 0; k < numSelected; k++)
-                for (int t = p->segTiles[segments[k]]; t < p->segTiles[segments[k]+1]; t++)
-                    tileList[count++] = t;
-            std::copy(segments, segments+numSelected, tileList+count);
-            p->subsetTiles.reserve(std::max(count, 1));
-            p->subsetSegments.reserve(std::max(numSelected, 1));
-            if (count > 0)
-                IXH_HIP_CHECK(hipMemcpyAsync(p->subsetTiles.ptr, tileList, sizeof(int)*count, hipMemcpyHostToDevice, stream));
-            if (numSelected > 0)
-                IXH_HIP_CHECK(hipMemcpyAsync(p->subsetSegments.ptr, tileList+count, sizeof(int)*numSelected, hipMemcpyHostToDevice,
-                                             stream));
-            IXH_HIP_CHECK(hipEventRecord(p->subsetCopied, stream));
-            subsetTileCount = count;
-        }
-        const ListCheck none = {};
-        // the end of the call: an event bound to its last kernel, or recorded after it in a graph capture
-        if (p->frameDone[p->frame] == nullptr)
-            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->frameDone[p->frame], hipEventDisableTiming));
-        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
-        IXH_HIP_CHECK(hipStreamIsCapturing(stream, &capture));
-        hipEvent_t end = (capture == hipStreamCaptureStatusNone ? p->frameDone[p->frame] : nullptr);
-        hipEvent_t marked = nullptr;
-        if (p->precision == IXH_DOUBLE) {
-            if (!framed)
-                hipLaunchKernelGGL((k_local_positions<double, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
-                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
-                                   (double4*) p->localFrame[p->frame].ptr, none, b->atom_index);
-            IXH_HIP_CHECK(hipGetLastError());
-            marked = launch_tiles<double>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
-                                          subsetTileCount, std::max(numSelected, 0));
-        }
-        else {
-            if (!framed)
-                hipLaunchKernelGGL((k_local_positions<float, false>), dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, b->posq,
-                                   corr, b->precision, p->perm.ptr, p->padded, periodic, make_box(b->box), p->centerFrame[p->frame].ptr,
-                                   (float4*) p->localFrame[p->frame].ptr, none, b->atom_index);
-            IXH_HIP_CHECK(hipGetLastError());
-            marked = launch_tiles<float>(p, b, terms, (flags & IXH_INCLUDE_ENERGY) != 0, segmentEnergies, stream, end,
-                                         subsetTileCount, std::max(numSelected, 0));
-        }
-        if (marked == nullptr)
-            IXH_HIP_CHECK(hipEventRecord(p->frameDone[p->frame], stream));
-        p->frameStream[p->frame] = stream;
+        execute_pair(p, b, terms, flags, nullptr, nullptr, -1, address);
     });
 }
 

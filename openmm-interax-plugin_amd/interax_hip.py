@@ -98,6 +98,7 @@ SIGNATURES = {
     "ixh_pair_destroy": [ctypes.c_void_p],
     "ixh_pair_set_parameters": [ctypes.c_void_p, ctypes.POINTER(PairDesc)],
     "ixh_pair_execute": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int],
+    "ixh_pair_execute_noalloc": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int],
     "ixh_pair_long_range_coefficients": [ctypes.c_void_p, _dp, _dp],
     "ixh_pair_long_range_energy": [ctypes.c_void_p, _dp, ctypes.c_double, _dp],
     "ixh_pair_lambda_energies": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp],

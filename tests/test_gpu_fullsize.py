@@ -3,8 +3,8 @@
 - The 98 304-atom water-like box (9.94 nm, cutoff 1.0 nm): the headline workload.  Its box is wide enough for most
   tiles to take one periodic image per pair of blocks, while tiles between two elongated blocks take the minimum
   image of every pair: both paths of the benchmarked kernel run (ixh_pair_image_mode).  Two perturbed coordinate
-  sets on one stream, the second with the deferred list check, in mixed precision (force RMS error <= 1e-5, largest component error <= 5e-5 of the RMS force, energy 2e-6:
-  TestInteraxExportValues.h:195-196) and in double precision (TestInteraxPairForce.h:719-725: energy 1e-10, forces
+  sets on one stream, the second with the deferred list check, in mixed precision (force RMS error <= 1e-5, largest
+  component error <= 5e-5 of the RMS force, energy 2e-6: TestInteraxExportValues.h:195-196) and in double precision (TestInteraxPairForce.h:719-725: energy 1e-10, forces
   1e-9 of the largest force); the number of interacting pairs equals the oracle's.
 - Its long range correction (98 304 dispersion classes: per-atom damping widths) against the closed form with the
   damping functions set to 1, which at r >= 1.0 nm (x >= 30) is exact to ~1e-9 (4.5e-10 on a 1 536-atom box).
