@@ -235,6 +235,11 @@ int ixh_pair_execute_subset(ixh_pair* pair, const ixh_buffers* buffers, int term
 int ixh_pair_list_was_valid(ixh_pair* pair, int* valid);
 /* Number of interacting pairs of the last execute() (waits for it). */
 int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
+/* Launch shape of the cutoff tile kernel (no counterpart in the reference): each wave takes a work item of up to
+   tiles_per_item consecutive tiles of one 64-particle block (its i side staged and its forces flushed once per item).
+   Takes effect at the next evaluation, which rebuilds the neighbour list; results do not depend on it beyond the
+   order of the FP64 energy sum. */
+int ixh_pair_set_tiles_per_item(ixh_pair* pair, int tiles_per_item);
 /* How the current neighbour list takes periodic images (no counterpart in the reference): of its `tiles` tiles,
    `per_block_tiles` take one image shift per pair of 64-particle blocks (exact for every pair in range when the box is
    wider than twice the cutoff plus the extents of both blocks and the skin); the others take the minimum image of

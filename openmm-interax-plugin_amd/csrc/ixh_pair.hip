@@ -636,6 +636,7 @@ struct TileArgs {
     int* waveCount;
     double* tileEnergy;   // segmented handles: the energy of every tile, for the per-segment sums
     const int* tileIndex;  // subsets of segments: the tiles to evaluate (numTiles of them); nullptr: all tiles
+    const int2* items;     // cutoff lists: the tile range [x, y) of every wave; nullptr: ranges split evenly
 };
 
 // Copy the type tables into LDS (all threads of the workgroup) and describe them.
@@ -1093,7 +1094,12 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
     const long long wave = (long long) blockIdx.x*QUEUE_GROUP_WAVES+w;
     const long long numWaves = (long long) gridDim.x*QUEUE_GROUP_WAVES;
     const long long T = a.numTiles;
-    const long long begin = T*wave/numWaves, end = T*(wave+1)/numWaves;
+    long long begin = T*wave/numWaves, end = T*(wave+1)/numWaves;
+    if (a.items != nullptr) {
+        const int2 item = a.items[wave];
+        begin = item.x;
+        end = item.y;
+    }
     const Real margin = a.band*a.cutoff2;
     const Real inner = a.cutoff2-margin, outer = a.cutoff2+margin;
 
@@ -1636,6 +1642,10 @@ struct ixh_pair {
     // that the workgroups dispatched last are the lightest
     DeviceArray<int> tileOrder;
     bool ordered = false;
+    // cutoff lists: work items (ranges of the tile array, one per wave, in launch order) of up to tilesPerItem tiles
+    int tilesPerItem = 3;
+    int numItems = 0;
+    DeviceArray<int2> items;
     // type tables (FP32 modes, few parameter classes): classes per particle and the rows of LDS tables
     bool useTables = false;
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
@@ -2034,6 +2044,7 @@ void compute_long_range(ixh_pair* p, hipStream_t stream) {
 // Build the slot order, the partner lists, the tiles and the candidate masks from the current positions.
 void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     int n = p->n;
+    p->numItems = 0;
     bool periodic = (p->method == IXH_CUTOFF_PERIODIC);
     bool cutoff = (p->method != IXH_NO_CUTOFF);
     BoxD box = make_box(b->box);
@@ -2240,12 +2251,42 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
         IXH_HIP_CHECK(hipMemcpyAsync(image.data(), p->tileImage.ptr, sizeof(int)*T, hipMemcpyDeviceToHost, stream));
         IXH_HIP_CHECK(hipStreamSynchronize(stream));
         {
-            // the tiles in launch order, decreasing candidate pairs (the workgroups dispatched last are the lightest),
-            // so that a wave reads its tile with no index in between
-            std::vector<int> order(T);
-            for (int t = 0; t < T; t++)
-                order[t] = t;
-            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return unsortedCounts[x] > unsortedCounts[y]; });
+            // Work items: up to tilesPerItem consecutive tiles of one block I (its diagonal tile, then its partner
+            // chunks), one item per wave, so that the i side is staged and flushed once per item; items launched
+            // heaviest first (the workgroups dispatched last are the lightest), padded with empty items to whole
+            // workgroups.  The tiles are stored in launch order, so that a wave reads its tiles with no index in
+            // between.  (Items grouped by spatial region per XCD, for L2 locality, measured 10-15 % slower on the
+            // 98 304-atom box than this global weight order.)
+            std::vector<std::pair<long long, int2>> sortedItems;   // (weight, (first tile, tiles))
+            for (int t0 = 0; t0 < T;) {
+                const int I = tiles[t0].x;
+                int t1 = t0;
+                long long weight = 0;
+                while (t1 < T && tiles[t1].x == I && t1-t0 < p->tilesPerItem)
+                    weight += unsortedCounts[t1++];
+                sortedItems.push_back(std::make_pair(weight, make_int2(t0, t1-t0)));
+                t0 = t1;
+            }
+            std::stable_sort(sortedItems.begin(), sortedItems.end(),
+                             [](const std::pair<long long, int2>& x, const std::pair<long long, int2>& y) { return x.first > y.first; });
+            std::vector<int> order;
+            order.reserve(T);
+            std::vector<int2> itemRanges;
+            const size_t numItems = (sortedItems.size()+QUEUE_GROUP_WAVES-1)/QUEUE_GROUP_WAVES*QUEUE_GROUP_WAVES;
+            itemRanges.reserve(numItems);
+            for (size_t k = 0; k < numItems; k++) {
+                const int begin = (int) order.size();
+                if (k < sortedItems.size()) {
+                    const int2 item = sortedItems[k].second;
+                    for (int t = item.x; t < item.x+item.y; t++)
+                        order.push_back(t);
+                }
+                itemRanges.push_back(make_int2(begin, (int) order.size()));
+            }
+            if (order.size() != (size_t) T)
+                throw Error(IXH_ERROR_HIP, "ixh_pair: work items do not cover the tiles");
+            p->numItems = (int) itemRanges.size();
+            p->items.upload(itemRanges.data(), std::max<size_t>(itemRanges.size(), 1), stream);
             std::vector<int2> sortedTiles(T);
             p->pairImageTiles = 0;
             for (int t = 0; t < T; t++) {
@@ -2312,6 +2353,8 @@ void launch_bound(Kernel kernel, dim3 grid, dim3 block, size_t shared, hipStream
 
 // Workgroups of a tile launch over numTiles tiles.
 int launch_workgroups(const ixh_pair* p, int numTiles) {
+    if (p->method != IXH_NO_CUTOFF && p->numItems > 0 && numTiles == p->numTiles)
+        return p->numItems/QUEUE_GROUP_WAVES;   // one wave per work item
     const int groupWaves = (p->method != IXH_NO_CUTOFF ? QUEUE_GROUP_WAVES : 4);
     return std::max(1, std::min((numTiles+groupWaves-1)/groupWaves, p->gridLimit));
 }
@@ -2391,6 +2434,7 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
     a.tileIndex = (p->ordered ? (const int*) p->tileOrder.ptr : (const int*) nullptr);
+    a.items = (p->method != IXH_NO_CUTOFF && p->numItems > 0 && subsetTiles < 0 ? (const int2*) p->items.ptr : nullptr);
     if (subsetTiles >= 0) {
         a.numTiles = subsetTiles;
         a.tileIndex = p->subsetTiles.ptr;
@@ -3008,6 +3052,20 @@ int ixh_pair_last_pair_count(ixh_pair* p, long long* count) {
         DeviceGuard guard(p->device);
         IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
         IXH_HIP_CHECK(hipMemcpy(count, p->pairCount.ptr, sizeof(long long), hipMemcpyDeviceToHost));
+    });
+}
+
+int ixh_pair_set_tiles_per_item(ixh_pair* p, int tiles_per_item) {
+    return guarded([&] {
+        if (p == nullptr || tiles_per_item < 1 || tiles_per_item > 64)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_set_tiles_per_item: between 1 and 64 tiles per work item");
+        if (p->tilesPerItem != tiles_per_item) {
+            DeviceGuard guard(p->device);
+            if (p->lastStream != nullptr)
+                IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
+            p->tilesPerItem = tiles_per_item;
+            p->built = false;   // the next evaluation lays the list out again
+        }
     });
 }
 
