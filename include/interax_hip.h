@@ -72,9 +72,10 @@ extern "C" {
 /* execute() flags */
 #define IXH_INCLUDE_FORCES 1
 #define IXH_INCLUDE_ENERGY 2
-/* Cutoff methods: check the neighbour list against the positions on the device without waiting for the answer; the
-   evaluation proceeds with the current list.  ixh_pair_list_was_valid() then tells whether it was still valid (if not,
-   the results must be discarded and the evaluation repeated without this flag, which rebuilds the list). */
+/* Pair forces with a cutoff, and three-body forces of plain handles: check the neighbour list against the positions on
+   the device without waiting for the answer; the evaluation proceeds with the current list.  ixh_pair_list_was_valid()
+   (ixh_threebody_list_was_valid()) then tells whether it was still valid (if not, the results must be discarded and the
+   evaluation repeated without this flag, which rebuilds the list). */
 #define IXH_DEFER_LIST_CHECK 4
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -295,6 +296,11 @@ int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* desc, i
 int ixh_threebody_execute_segments(ixh_threebody* threebody, const ixh_buffers* buffers, int flags, double* segment_energies);
 int ixh_threebody_execute_subset(ixh_threebody* threebody, const ixh_buffers* buffers, int flags, double* segment_energies,
                                  const int* segments, int num_selected);
+/* Plain handles keep neighbour lists within cutoff + 0.1 nm and rebuild them when a site has moved more than half of
+   that skin (interaxThreeBody.cc:11-30).  execute() checks the lists on the device; with IXH_DEFER_LIST_CHECK it does
+   not wait for the answer, and this call then tells whether the lists of the last execute() held (if not, its results
+   must be discarded and the evaluation repeated without the flag, which rebuilds them); 1 for any other execute(). */
+int ixh_threebody_list_was_valid(ixh_threebody* threebody, int* valid);
 /* Number of interacting triples of the last execute() (waits for it). */
 int ixh_threebody_last_triple_count(ixh_threebody* threebody, long long* count);
 int ixh_threebody_kernel_time(ixh_threebody* threebody, double* milliseconds, long long* launches);

@@ -609,6 +609,8 @@ struct TileArgs {
     int tableFloats;
     int numDisp, numExch, numSr, exchOffset, srOffset;
     int highOrder;   // some overlap row of the tables is of degree 8
+    int singleBranch;   // every width pair of the tables takes one overlap branch only
+    int ordered;     // the overlap of a pair depends on which particle has the lower System index (InteraxOrder2)
     float beta2;
     int n;
     // geometry
@@ -655,6 +657,7 @@ __device__ __forceinline__ PairTables load_tables(const TileArgs<Real>& a, float
     tb.beta2 = a.beta2;
     tb.shared = a.settings.shared_overlap;
     tb.highOrder = a.highOrder;
+    tb.singleBranch = a.singleBranch;
     return tb;
 }
 
@@ -996,7 +999,13 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
     Particle P, Q;
     fetch_particle(q.i, il, P);
     fetch_particle(q.j, jl, Q);
-    const int flagI = q.i.flags[il], flagJ = q.j.flags[jl];
+    // the flags hold the alchemical bits and the System indices, which only an InteraxOrder2 overlap needs (for the
+    // Exact scheme the overlap of a pair is the same either way round)
+    int flagI = 0, flagJ = 0;
+    if (ALCH || a.ordered) {
+        flagI = q.i.flags[il];
+        flagJ = q.j.flags[jl];
+    }
     Real en, fs;
     evaluate_pair<Real, TERMS, ALCH>(a, tb, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
     energy += (double) en;
@@ -1650,6 +1659,7 @@ struct ixh_pair {
     bool useTables = false;
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
     int highOrder = 0;
+    int singleBranch = 0;   // no width pair of the tables mixes the Taylor and closed-form branches
     DeviceArray<int> classes, sortedClasses;
     DeviceArray<float> tables;
     DeviceArray<unsigned> keys, keysOut;
@@ -1788,7 +1798,7 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
     taylor.n1 = taylor.n2 = (float) (1/l);
     double scale = INV_PI/(l*l*l);
     for (int k = 0; k <= s.order+2; k++, scale /= l)
-        taylor.p[k] = (float) (scale*OverlapTaylor::coefficient(s.order, k, t));
+        taylor.set_coefficient(k, (float) (scale*OverlapTaylor::coefficient(s.order, k, t)));
     const double D = b*b-a*a;
     if (D != 0) {
         const double m1 = 4*a*a*b*b/(D*D*D)*INV_8PI;
@@ -1797,7 +1807,7 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
         closed.k2 = (float) (LOG2E/b);
         closed.n2 = (float) (1/b);
         closed.m1 = (float) m1;
-        closed.p[0] = (float) (a*INV_8PI/(D*D));
+        closed.set_coefficient(0, (float) (a*INV_8PI/(D*D)));
         closed.q1 = (float) -m1;
         closed.q0 = (float) (b*INV_8PI/(D*D));
     }
@@ -1816,7 +1826,7 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
         rows[k].gate = 1;
         rows[k].high = 0;
         for (int j = 4; j < 9; j++)
-            if (rows[k].p[j] != 0.0f)
+            if (rows[k].coefficient(j) != 0.0f)
                 rows[k].high = 1;
     }
 }
@@ -1826,6 +1836,7 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
 bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls, std::vector<float>& tables) {
     const int n = p->n;
     p->highOrder = 0;
+    p->singleBranch = 0;
     typedef std::tuple<double, double, double, double> Key;
     std::map<Key, int> dispMap, exchMap, srMap;
     std::vector<Key> dispKeys, exchKeys, srKeys;    // exchange: (width, amplitude); short range: (width, K, L)
@@ -1912,6 +1923,11 @@ bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls
                 fill(srKeys, p->srOffset, p->numSr, p->settings.short_range_scheme);
         }
     }
+    // rows of a width pair that takes one branch only are equal: the kernel then reads the first without selecting
+    p->singleBranch = 1;
+    for (int offset = p->exchOffset; offset < p->tableFloats; offset += OVERLAP_PAIR_FLOATS)
+        if (std::memcmp(&tables[offset], &tables[offset+OVERLAP_PAIR_FLOATS/2], sizeof(OverlapRow)) != 0)
+            p->singleBranch = 0;
     return true;
 }
 
@@ -2431,6 +2447,9 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
     a.exchOffset = p->exchOffset;
     a.srOffset = p->srOffset;
     a.highOrder = p->highOrder;
+    a.singleBranch = p->singleBranch;
+    a.ordered = ((p->terms & IXH_TERM_EXCHANGE) && !p->settings.exchange_scheme.about_smaller) ||
+                ((p->terms & IXH_TERM_SHORT_RANGE) && !p->settings.short_range_scheme.about_smaller);
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
     a.tileIndex = (p->ordered ? (const int*) p->tileOrder.ptr : (const int*) nullptr);

@@ -317,18 +317,28 @@ constexpr int MAX_WIDTH_CLASSES = 8;
 // the closed form (:176-187) is S = [4a^2b^2/D^3 (e_a - e_b) + r/D^2 (a e_a + b e_b)]/(8 pi r), i.e.
 // m1 = 4a^2b^2/(8 pi D^3), p_0 = a/(8 pi D^2), q1 = -m1, q0 = b/(8 pi D^2).  Every pair then runs the same
 // instructions whichever branch its widths select: no divergence between Taylor and closed-form pairs of a wave.
+// The fields are laid out for 16-byte LDS reads: the first 48 bytes hold everything a row of degree 3 needs.
 struct alignas(16) OverlapRow {
-    float kTaylor, txLimit;  // the Taylor row is used where r*kTaylor < txLimit, the closed-form row elsewhere
     float k1, k2;            // log2(e)/w1, log2(e)/w2
     float n1, n2;            // 1/w1, 1/w2
     float m1;
-    float p[9];
+    float p[4];              // p_0 .. p_3
     float q1, q0;
-    float norm;              // 8 pi (a+b)^3, the short-range normalisation
-    float gate;              // 0 if a or b is zero: the pair has no such term
+    float kTaylor;           // the Taylor row is used where r*kTaylor < txLimit, the closed-form row elsewhere
+    float txLimit;
     float amplitude;         // exchange: A_i A_j
+    float norm;              // 8 pi (a+b)^3, the short-range normalisation
     float kk, ll;            // short range: (K_i+K_j)/2, (L_i+L_j)/2
-    float high;              // 1 if any of p[4..8] is non-zero (a Taylor row of unequal widths), else 0
+    float gate;              // 0 if a or b is zero: the pair has no such term
+    float high;              // 1 if any of p_4 .. p_8 is non-zero (a Taylor row of unequal widths), else 0
+    float ph[5];             // p_4 .. p_8
+    float coefficient(int k) const { return k < 4 ? p[k] : ph[k-4]; }
+    void set_coefficient(int k, float v) {
+        if (k < 4)
+            p[k] = v;
+        else
+            ph[k-4] = v;
+    }
 };
 static_assert(sizeof(OverlapRow) == 24*sizeof(float), "OverlapRow layout");
 // Per ordered class pair: the Taylor row, then the closed-form row (both identical when one branch is never taken).
@@ -348,32 +358,40 @@ struct PairTables {
     int numDisp, numExch, numSr;
     float beta2;             // 2 * damping scale
     int shared;
-    int highOrder;           // some row has non-zero p[4..8]
+    int highOrder;           // some row has non-zero p_4 .. p_8
+    int singleBranch;        // every width pair takes one branch only (its two rows are equal): no row selection
 };
 
-// S and dS/dr from the row pair of a width pair.
-__device__ __forceinline__ void overlap_from_rows(const PairTables& tb, const OverlapRow* rows, float r, float invR, float& S, float& dS) {
-    const OverlapRow& w = rows[(r*rows[0].kTaylor < rows[0].txLimit) ? 0 : 1];
+// The row of a width pair that applies at distance r (the Taylor row where r*kTaylor < txLimit, else the closed
+// form; with single-branch tables the first row, read without waiting for a selection).
+__device__ __forceinline__ const OverlapRow& overlap_row(const PairTables& tb, const OverlapRow* rows, float r) {
+    return (tb.singleBranch ? rows[0] : rows[(r*rows[0].kTaylor < rows[0].txLimit) ? 0 : 1]);
+}
+
+// S and dS/dr of a row: S = e1 (m1/r + sum_k p_k r^k) + e2 (q1/r + q0).
+__device__ __forceinline__ void overlap_from_row(const PairTables& tb, const OverlapRow& w, float r, float invR, float& S, float& dS) {
     const float e1 = fast_exp2(-r*w.k1);
     const float e2 = fast_exp2(-r*w.k2);
     float P, dP = 0;
     if (tb.highOrder && __builtin_amdgcn_ballot_w64(w.high != 0.0f) != 0) {
         // some lane of the wave holds a degree-8 row
-        P = w.p[8];
+        P = w.ph[4];
 #pragma unroll
-        for (int k = 7; k >= 0; k--) {
+        for (int k = 7; k >= 4; k--) {
             dP = dP*r + P;
-            P = P*r + w.p[k];
+            P = P*r + w.ph[k-4];
         }
+        dP = dP*r + P;
+        P = P*r + w.p[3];
     }
     else {
-        // every row of the wave is a closed form or an equal-width expansion: p[4..8] are zero
+        // every row of the wave is a closed form or an equal-width expansion: p_4 .. p_8 are zero
         P = w.p[3];
+    }
 #pragma unroll
-        for (int k = 2; k >= 0; k--) {
-            dP = dP*r + P;
-            P = P*r + w.p[k];
-        }
+    for (int k = 2; k >= 0; k--) {
+        dP = dP*r + P;
+        P = P*r + w.p[k];
     }
     const float invR2 = invR*invR;
     const float P1 = P + w.m1*invR;
@@ -414,26 +432,26 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     float sx = 0, dsx = 0;
     if (EXCH) {
         const int xi = (pi.cls >> 8) & 0xff, xj = (pj.cls >> 8) & 0xff;
-        const OverlapRow* rows = tb.exch + 2*(iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi);
-        overlap_from_rows(tb, rows, r, invR, sx, dsx);
-        const float amp = rows[0].amplitude;
+        const OverlapRow& w = overlap_row(tb, tb.exch+2*(iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi), r);
+        overlap_from_row(tb, w, r, invR, sx, dsx);
+        const float amp = w.amplitude;
         energy += amp*sx;
         dEdR += amp*dsx;
     }
     if (SR) {
         const int si = (pi.cls >> 16) & 0xff, sj = (pj.cls >> 16) & 0xff;
-        const OverlapRow* rows = tb.sr + 2*(iFirst ? __umul24(si, tb.numSr)+sj : __umul24(sj, tb.numSr)+si);
+        const OverlapRow& w = overlap_row(tb, tb.sr+2*(iFirst ? __umul24(si, tb.numSr)+sj : __umul24(sj, tb.numSr)+si), r);
         float ss, dss;
         if (EXCH && tb.shared) {
             ss = sx;
             dss = dsx;
         }
         else
-            overlap_from_rows(tb, rows, r, invR, ss, dss);
-        const float norm = rows[0].norm;
+            overlap_from_row(tb, w, r, invR, ss, dss);
+        const float norm = w.norm;
         const float st = norm*ss;
-        const float kk = rows[0].kk;
-        const float ll = rows[0].ll;
+        const float kk = w.kk;
+        const float ll = w.ll;
         energy += (kk+ll*st)*st;
         dEdR += (kk+2.0f*ll*st)*norm*dss;
     }

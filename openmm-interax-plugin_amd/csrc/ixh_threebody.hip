@@ -5,12 +5,16 @@
 // InteraxThreeBodyForceImpl.cpp:42-62 (validation).
 //
 // Pipeline per evaluation:
-//   1. site centroids in FP64, summed in the reference's order;
-//   2. half neighbour lists (sites above, other molecule, |d|^2 < rc^2 decided in FP64 with the reference's
-//      minimum image), one wavefront per site with ballot compaction, so each list is in ascending site order;
-//   3. one wavefront per site A walks the pairs (B, C) of its list, tests B-C in FP32 (FP64 near the cutoff or near
-//      a change of periodic image) and evaluates the triple in FP32; the forces on B and C go to LDS as 64-bit
-//      fixed point (integer atomics, so their order does not matter), the force on A stays in registers;
+//   1. site centroids in FP64, summed in the reference's order, and the neighbour-list check: has a site moved more
+//      than half the list skin since the lists were built (interaxThreeBody.cc:11-30 keeps 0.1 nm-skin lists the same
+//      way)?
+//   2. only when the lists are stale (or new): half neighbour lists within cutoff + skin (sites above, other
+//      molecule, decided in FP64 with the reference's minimum image), one wavefront per site with ballot compaction
+//      over a cell grid, so each list is in a fixed order;
+//   3. one wavefront per site A takes the neighbours of its list within the cutoff at this evaluation's centroids
+//      (the reference's FP64 decision), walks their pairs (B, C), tests B-C in FP32 (FP64 near the cutoff or near a
+//      change of periodic image) and evaluates the triple in FP32; the forces on B and C go to LDS as 64-bit fixed
+//      point (integer atomics, so their order does not matter), the force on A stays in registers;
 //   4. the site forces are shared equally among the particles of each site.
 #include "ixh_common.h"
 
@@ -27,6 +31,8 @@ namespace {
 // Neighbour pairs per work item of the triple kernel of segmented handles (64 steps of 64 pairs for each of its 4
 // wavefronts).
 constexpr int TRIPLE_CHUNK = 16384;
+// Padding of the neighbour lists of plain handles (nm); they are rebuilt when a site has moved skin/2.
+constexpr double THREE_BODY_SKIN = 0.1;
 
 struct BoxT {
     double ax, bx, by, cx, cy, cz;
@@ -93,6 +99,40 @@ __global__ void k_sites(const void* posq, const float4* correction, int precisio
     sites[3*i] = __dmul_rn(s[0], scale);
     sites[3*i+1] = __dmul_rn(s[1], scale);
     sites[3*i+2] = __dmul_rn(s[2], scale);
+}
+
+// 1'. Plain handles: the centroids (as k_sites), the zeroed site forces, and the neighbour-list check: flag is set
+// if a site moved more than sqrt(limit2) since the lists were built (minimum image), and the flag of the other frame is
+// cleared for the next call.  Without lists (buildSites == nullptr) there is nothing to check.
+__global__ void k_sites_check(const void* posq, const float4* correction, int precision, const int* offsets, const int* particles,
+                              int numSites, double* sites, const double* buildSites, int periodic, BoxT box, double limit2, int* flag,
+                              int* otherFlag, long long* siteForce) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i == 0)
+        *otherFlag = 0;
+    if (i < 3*numSites)
+        siteForce[i] = 0;
+    if (i >= numSites)
+        return;
+    double s[3] = {0, 0, 0};
+    for (int k = offsets[i]; k < offsets[i+1]; k++) {
+        double p[3];
+        load_pos(posq, correction, precision, particles[k], p);
+        s[0] = __dadd_rn(s[0], p[0]);
+        s[1] = __dadd_rn(s[1], p[1]);
+        s[2] = __dadd_rn(s[2], p[2]);
+    }
+    const double scale = 1.0/(double) (offsets[i+1]-offsets[i]);
+    const double c[3] = {__dmul_rn(s[0], scale), __dmul_rn(s[1], scale), __dmul_rn(s[2], scale)};
+    sites[3*i] = c[0];
+    sites[3*i+1] = c[1];
+    sites[3*i+2] = c[2];
+    if (buildSites != nullptr) {
+        double d[3];
+        ref_delta(buildSites+3*i, c, periodic, box, d);
+        if (ref_norm2(d) > limit2)
+            *flag = 1;
+    }
 }
 
 // 2. Half neighbour lists from a cell grid.  Each axis has either one cell (searched alone) or at least three cells
@@ -348,7 +388,8 @@ __global__ void k_neighbors(const double* sites, const int* molecule, int numSit
         const int pos = written+__popcll(ballot & ((1ULL << lane)-1));
         if (hit && pos < capacity) {
             nbrIndex[(size_t) i*capacity+pos] = j;
-            nbrDelta[(size_t) i*capacity+pos] = make_double4(d[0], d[1], d[2], 0.0);
+            if (nbrDelta != nullptr)
+                nbrDelta[(size_t) i*capacity+pos] = make_double4(d[0], d[1], d[2], 0.0);
         }
         written += __popcll(ballot);
     }
@@ -399,6 +440,11 @@ struct TripleArgs {
     Real dampingScale;
     Real invDampingRadius;       // 0: no centroid damping
     int capacity;                // LDS entries
+    int skinLists;               // plain handles: the lists hold the sites within cutoff + skin, the neighbours within the
+                                 // cutoff are found at each evaluation (nbrDelta is not used)
+    int exactImage;              // skin lists: the reference's minimum image with divisions (small boxes)
+    long long* contribution;     // skin lists: the force on the site of list entry e from the triples of the list's
+                                 // owner, component k at [3 e + k] (summed by k_gather_site_forces)
     long long* siteForce;        // 3 x numSites fixed point
     double* itemEnergy;          // energy and triple count of every work item
     int* itemCount;
@@ -437,18 +483,17 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     int* sIndex = (int*) (sForce+3*(size_t) t.capacity);
     int* sMol = sIndex+t.capacity;              // molecule
     Real* sWidth = (Real*) (sMol+t.capacity);   // damping width
+    int* sSlot = (int*) (sWidth+t.capacity);    // skin lists: the LDS slot of each list entry, -1 if beyond the cutoff
     const int item = (t.itemIndex != nullptr ? t.itemIndex[blockIdx.x] : (int) blockIdx.x);
     const int i = (t.itemSite != nullptr ? t.itemSite[item] : item);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int begin = t.start[i], m = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
+    const int begin = t.start[i], listLength = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
     const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
     const bool alchI = t.siteParams[4*i+3] != 0;
     // the per-neighbour factors of the triple's C9 and geometry, computed once per work item
-    for (int a = threadIdx.x; a < m; a += 64*WAVES) {
-        int j = t.nbrIndex[begin+a];
-        double4 d = t.nbrDelta[begin+a];
-        const Real x = Real(d.x), y = Real(d.y), z = Real(d.z);
+    auto stage = [&](int a, int j, const double d[3]) {
+        const Real x = Real(d[0]), y = Real(d[1]), z = Real(d[2]);
         const Real u = x*x+y*y+z*z;
         R4 dr = {x, y, z, u};
         sDelta[a] = dr;
@@ -459,7 +504,43 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
         sMol[a] = t.molecule[j];
         sWidth[a] = Real(t.siteParams[4*j+2]);
         sForce[3*a] = sForce[3*a+1] = sForce[3*a+2] = 0;
+    };
+    int m = listLength;
+    if (t.skinLists) {
+        // one wavefront (WAVES == 1): the list's sites within the cutoff now, in list order (ReferenceInteraxThreeBodyIxn.cpp:
+        // 97-103, the reference's FP64 rounding), compacted by ballot
+        const double* si = t.sites+3*i;
+        const double inv[3] = {1.0/t.box.ax, 1.0/t.box.by, 1.0/t.box.cz};
+        int kept = 0;
+        for (int a0 = 0; a0 < listLength; a0 += 64) {
+            const int a = a0+lane;
+            int j = -1;
+            double d[3] = {0, 0, 0};
+            bool hit = false;
+            if (a < listLength) {
+                j = t.nbrIndex[begin+a];
+                if (t.exactImage)
+                    ref_delta(si, t.sites+3*j, t.periodic, t.box, d);
+                else
+                    fast_delta(si, t.sites+3*j, t.periodic, t.box, inv, d);
+                hit = ref_norm2(d) < t.cutoff2;
+            }
+            const unsigned long long ballot = __ballot(hit);
+            const int slot = kept+__popcll(ballot & ((1ULL << lane)-1));
+            if (hit)
+                stage(slot, j, d);
+            if (a < listLength)
+                sSlot[a] = (hit ? slot : -1);
+            kept += __popcll(ballot);
+        }
+        m = kept;
     }
+    else
+        for (int a = threadIdx.x; a < m; a += 64*WAVES) {
+            const double4 d4 = t.nbrDelta[begin+a];
+            const double d[3] = {d4.x, d4.y, d4.z};
+            stage(a, t.nbrIndex[begin+a], d);
+        }
     __syncthreads();
     const Real cI = t.scaleFactor*Real(1.5)*aI*wI;
     Real fA[3] = {0, 0, 0};
@@ -685,18 +766,36 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
             flushRow(R+1, accR1);
     }
     __syncthreads();
-    // flush the neighbour forces and the force on A (integer sums: the order of the atomics does not matter)
-    for (int a = threadIdx.x; a < m; a += 64*WAVES) {
-        int j = sIndex[a];
-        for (int k = 0; k < 3; k++)
-            add_fixed(&t.siteForce[(size_t) k*t.numSites+j], (long long) sForce[3*a+k]);
+    if (t.skinLists) {
+        // plain handles: the neighbour forces as plain stores, one per list entry (zero beyond the cutoff), summed per
+        // site by k_gather_site_forces; the force on A is this workgroup's alone (integer sums: exact in any order)
+        for (int a = lane; a < listLength; a += 64) {
+            const int slot = sSlot[a];
+            for (int k = 0; k < 3; k++)
+                t.contribution[3*((long long) begin+a)+k] = (slot >= 0 ? (long long) sForce[3*slot+k] : 0LL);
+        }
+        for (int k = 0; k < 3; k++) {
+            long long v = to_fixed(fA[k]);
+            for (int offset = 32; offset > 0; offset >>= 1)
+                v += __shfl_xor(v, offset, 64);
+            if (lane == 0)
+                t.siteForce[(size_t) k*t.numSites+i] = v;
+        }
     }
-    for (int k = 0; k < 3; k++) {
-        long long v = to_fixed(fA[k]);
-        for (int offset = 32; offset > 0; offset >>= 1)
-            v += __shfl_xor(v, offset, 64);
-        if (lane == 0)
-            add_fixed(&t.siteForce[(size_t) k*t.numSites+i], v);
+    else {
+        // flush the neighbour forces and the force on A (integer sums: the order of the atomics does not matter)
+        for (int a = threadIdx.x; a < m; a += 64*WAVES) {
+            int j = sIndex[a];
+            for (int k = 0; k < 3; k++)
+                add_fixed(&t.siteForce[(size_t) k*t.numSites+j], (long long) sForce[3*a+k]);
+        }
+        for (int k = 0; k < 3; k++) {
+            long long v = to_fixed(fA[k]);
+            for (int offset = 32; offset > 0; offset >>= 1)
+                v += __shfl_xor(v, offset, 64);
+            if (lane == 0)
+                add_fixed(&t.siteForce[(size_t) k*t.numSites+i], v);
+        }
     }
     // energy and count: per wavefront, then summed in wavefront order
     __shared__ double sWaveEnergy[WAVES];
@@ -797,6 +896,90 @@ __global__ void k_apply_site_forces(const long long* siteForce, int numSites, co
     }
 }
 
+// Plain handles: the force on every site, the sum of its own part (the triples it owns) and of the list entries that
+// name it (its reverse list, built with the lists), shared equally among its particles as k_apply_site_forces does.
+// GATHER_LANES lanes per site take every GATHER_LANES-th entry, so that the loads of a site are in flight together;
+// their integer sums are combined by lane exchanges (exact in any order).  Each workgroup also sums the energies and
+// triple counts of its sites in site order into one partial (groupEnergy, groupCount), for k_reduce_sites.
+constexpr int GATHER_LANES = 16;
+constexpr int GATHER_THREADS = 256;
+__global__ void __launch_bounds__(GATHER_THREADS) k_gather_site_forces(const long long* own, const long long* contribution,
+                                                                       const int* revStart, const int* revEnd,
+                                                                       const int* revList, int numSites, const int* offsets,
+                                                                       const int* particles, long long* force, int stride,
+                                                                       int applyForces, const double* siteEnergy,
+                                                                       const int* siteCount, double* groupEnergy,
+                                                                       int* groupCount) {
+    const int i = (int) ((blockIdx.x*blockDim.x+threadIdx.x)/GATHER_LANES);
+    const int sub = threadIdx.x % GATHER_LANES;
+    {
+        constexpr int SITES = GATHER_THREADS/GATHER_LANES;
+        __shared__ double sE[SITES];
+        __shared__ int sC[SITES];
+        if (threadIdx.x < SITES) {
+            const int site = blockIdx.x*SITES+threadIdx.x;
+            sE[threadIdx.x] = (site < numSites ? siteEnergy[site] : 0.0);
+            sC[threadIdx.x] = (site < numSites ? siteCount[site] : 0);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double e = 0;
+            int c = 0;
+            for (int k = 0; k < SITES; k++) {
+                e += sE[k];
+                c += sC[k];
+            }
+            groupEnergy[blockIdx.x] = e;
+            groupCount[blockIdx.x] = c;
+        }
+    }
+    if (!applyForces)
+        return;   // uniform over the launch: the lane exchanges below need every lane of the wave
+    long long total[3] = {0, 0, 0};
+    if (i < numSites) {
+        if (sub == 0)
+            for (int k = 0; k < 3; k++)
+                total[k] = own[(size_t) k*numSites+i];
+        for (int r = revStart[i]+sub; r < revEnd[i]; r += GATHER_LANES) {
+            const long long e = revList[r];
+            for (int k = 0; k < 3; k++)
+                total[k] += contribution[3*e+k];
+        }
+    }
+    for (int offset = GATHER_LANES/2; offset > 0; offset >>= 1)
+        for (int k = 0; k < 3; k++)
+            total[k] += __shfl_xor(total[k], offset, 64);
+    if (i >= numSites || sub != 0)
+        return;
+    const double scale = 1.0/(double) (offsets[i+1]-offsets[i]);
+    long long f[3];
+    for (int k = 0; k < 3; k++)
+        f[k] = (long long) ((double) total[k]/FIXED_POINT_SCALE*scale*FIXED_POINT_SCALE);
+    for (int p = offsets[i]; p < offsets[i+1]; p++) {
+        const int particle = particles[p];
+        for (int k = 0; k < 3; k++)
+            add_fixed(&force[(size_t) k*stride+particle], f[k]);
+    }
+}
+
+// Reverse lists of the neighbour lists (list entry e = i capacity + a of site i names site j): count per site (pass 0),
+// then, after a scan of the counts (k_cell_scan), the entries of each site in any order (pass 1; the sums over them
+// are integer, so their order does not matter).
+__global__ void k_reverse_lists(const int* nbrIndex, const int* count, int numSites, int capacity, int* revCount, int* revCursor,
+                                int* revList) {
+    const long long e = (long long) blockIdx.x*blockDim.x+threadIdx.x;
+    if (e >= (long long) numSites*capacity)
+        return;
+    const int i = (int) (e/capacity), a = (int) (e-(long long) i*capacity);
+    if (a >= count[i])
+        return;
+    const int j = nbrIndex[e];
+    if (revList == nullptr)
+        atomicAdd(&revCount[j], 1);
+    else
+        revList[atomicAdd(&revCursor[j], 1)] = (int) e;
+}
+
 // The total energy and triple count of a launch: one workgroup of 1024 threads, strided (coalesced) partial sums
 // with four loads in flight per thread, then a fixed tree, so the result does not depend on timing.
 __global__ void __launch_bounds__(1024) k_reduce_sites(const double* siteEnergy, const int* siteCount, int numSites, double* energy,
@@ -868,6 +1051,19 @@ struct ixh_threebody {
     int* hostTotals = nullptr;   // pinned, 2 ints
     int listCapacity = 0;        // entries per site of the unsegmented lists
     int listSites = -1;          // sites the lists were sized for
+    // plain handles: lists within cutoff + skin, rebuilt when a site has moved half the skin since buildSites
+    bool listBuilt = false;
+    int listMax = 0;             // the longest list of the last build (sizes the triple kernel's LDS)
+    DeviceArray<double> buildSites;
+    int* hostFlag = nullptr;     // pinned, coherent: the list-check flags of the two frames, written by the check kernel
+    int frame = 0;               // the frame of the last check
+    bool deferredCheck = false;  // the last call checked without waiting (IXH_DEFER_LIST_CHECK)
+    hipEvent_t checked = nullptr;   // the end of the last deferred check
+    // per-entry neighbour forces and the reverse lists that sum them per site (plain handles)
+    DeviceArray<long long> contribution;
+    DeviceArray<int> revCount, revStart, revEnd, revCursor, revList;
+    DeviceArray<double> groupEnergy;   // energies and triple counts per workgroup of k_gather_site_forces
+    DeviceArray<int> groupCount;
     // independent systems (segments) of a batch: sites [segOffsets[s], segOffsets[s+1]); neighbour lists of static
     // capacity (the later sites of the same segment), so that an evaluation needs no host synchronization
     int numSegments = 0;
@@ -967,7 +1163,7 @@ __global__ void k_site_bounds(const double* sites, int numSites, double* out) {
 // The cell grid of one evaluation.  Periodic boxes: rectangular boxes with at least three cells of the cutoff per
 // axis; other boxes, and axes with fewer cells, get one cell along that axis.  Nonperiodic systems of more than
 // 4096 sites: a grid of cutoff-sized cells over the sites' bounding box (at most 64 per axis).
-CellGrid make_grid(ixh_threebody* h, const double* box, hipStream_t stream) {
+CellGrid make_grid(ixh_threebody* h, const double* box, double cellSize, hipStream_t stream) {
     CellGrid g;
     g.periodic = h->periodic;
     for (int k = 0; k < 3; k++) {
@@ -975,7 +1171,7 @@ CellGrid make_grid(ixh_threebody* h, const double* box, hipStream_t stream) {
         g.origin[k] = 0;
         g.scale[k] = 0;
     }
-    const double cell = h->cutoff*(1+1e-12);
+    const double cell = cellSize*(1+1e-12);
     if (h->periodic) {
         bool rectangular = (box[3] == 0 && box[6] == 0 && box[7] == 0);
         if (rectangular)
@@ -1044,10 +1240,13 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.dampingScale = Real(h->dampingScale);
     t.invDampingRadius = Real(h->centroidDampingRadius > 0 ? 1.0/h->centroidDampingRadius : 0.0);
     t.capacity = std::max(maxCount, 1);
+    t.skinLists = (h->numSegments == 0);
+    t.contribution = h->contribution.ptr;
+    t.exactImage = (h->periodic && 2*h->cutoff >= std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
     t.siteForce = h->siteForce.ptr;
     t.itemEnergy = h->siteEnergy.ptr;
     t.itemCount = h->siteCount.ptr;
-    size_t shared = (size_t) t.capacity*(2*sizeof(R4)+3*sizeof(long long)+2*sizeof(int)+sizeof(Real));
+    size_t shared = (size_t) t.capacity*(2*sizeof(R4)+3*sizeof(long long)+3*sizeof(int)+sizeof(Real));
     if (shared > 160*1024)
         throw Error(IXH_ERROR_UNSUPPORTED, "ixh_threebody_execute: a site has too many neighbours for one workgroup");
     bool timed = h->timer.start(stream);
@@ -1061,6 +1260,28 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     IXH_HIP_CHECK(hipGetLastError());
     if (timed)
         h->timer.stop(stream);
+}
+
+// The rest of a plain evaluation once its lists hold: the triples, the site forces onto the particles, the energy.
+void finish_plain(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, double cutoff2, int flags, hipStream_t stream) {
+    const int n = h->numSites;
+    if (h->precision == IXH_DOUBLE)
+        launch_triples<double>(h, b, box, cutoff2, h->listMax, stream);
+    else
+        launch_triples<float>(h, b, box, cutoff2, h->listMax, stream);
+    const int groups = grid_of((long long) n*GATHER_LANES, GATHER_THREADS);
+    h->groupEnergy.reserve(groups);
+    h->groupCount.reserve(groups);
+    hipLaunchKernelGGL(k_gather_site_forces, dim3(groups), dim3(GATHER_THREADS), 0, stream, (const long long*) h->siteForce.ptr,
+                       (const long long*) h->contribution.ptr, (const int*) h->revStart.ptr, (const int*) h->revEnd.ptr,
+                       (const int*) h->revList.ptr, n, (const int*) h->offsetsDev.ptr, (const int*) h->particlesDev.ptr, b->force,
+                       b->force_stride, (flags & IXH_INCLUDE_FORCES) != 0 ? 1 : 0, (const double*) h->siteEnergy.ptr,
+                       (const int*) h->siteCount.ptr, h->groupEnergy.ptr, h->groupCount.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(1024), 0, stream, h->groupEnergy.ptr, h->groupCount.ptr, groups, b->energy,
+                       h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
+    IXH_HIP_CHECK(hipGetLastError());
+    IXH_HIP_CHECK(hipEventRecord(h->done, stream));
 }
 
 }  // namespace
@@ -1200,10 +1421,14 @@ int ixh_threebody_destroy(ixh_threebody* h) {
             (void) hipHostFree(h->subsetStaging);
         if (h->hostTotals != nullptr)
             (void) hipHostFree(h->hostTotals);
+        if (h->hostFlag != nullptr)
+            (void) hipHostFree(h->hostFlag);
         DeviceGuard guard(h->device);
         (void) hipDeviceSynchronize();
         if (h->done != nullptr)
             (void) hipEventDestroy(h->done);
+        if (h->checked != nullptr)
+            (void) hipEventDestroy(h->checked);
         delete h;
     });
 }
@@ -1253,9 +1478,10 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         h->lastStream = stream;
         const float4* corr = (const float4*) b->posq_correction;
         BoxT box = {b->box[0], b->box[3], b->box[4], b->box[6], b->box[7], b->box[8]};
-        // plain handles whose cell grid comes from the box prepare the sites in one launch, after the grid
+        // plain handles whose cell grid comes from the box prepare the sites of a list build in one launch, after the
+        // grid
         const bool gridFromBox = (h->numSegments == 0 && (h->periodic || n <= 4096));
-        if (!gridFromBox) {
+        if (h->numSegments > 0) {
             hipLaunchKernelGGL(k_sites, dim3(grid_of(n, 128)), dim3(128), 0, stream, b->posq, corr, b->precision, h->offsetsDev.ptr,
                                h->particlesDev.ptr, n, h->sites.ptr);
             IXH_HIP_CHECK(hipGetLastError());
@@ -1338,7 +1564,38 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
             IXH_HIP_CHECK(hipEventRecord(h->done, stream));
             return;
         }
-        CellGrid g = make_grid(h, b->box, stream);
+        // Plain handles: every call computes the centroids and checks the lists (k_sites_check); the lists (within cutoff
+        // + skin) are rebuilt only when a site has moved more than half the skin since they were built.  With
+        // IXH_DEFER_LIST_CHECK the call does not wait for the check: ixh_threebody_list_was_valid tells afterwards.
+        const double listCutoff = h->cutoff+THREE_BODY_SKIN;
+        if (h->hostFlag == nullptr) {
+            IXH_HIP_CHECK(hipHostMalloc((void**) &h->hostFlag, 2*sizeof(int), hipHostMallocCoherent));
+            h->hostFlag[0] = h->hostFlag[1] = 0;
+        }
+        if (h->checked == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&h->checked, hipEventDisableTiming));
+        const bool listed = (h->listBuilt && h->listSites == n);
+        const bool defer = listed && (flags & IXH_DEFER_LIST_CHECK) != 0;
+        const int f = h->frame ^ 1;
+        h->frame = f;
+        h->deferredCheck = false;
+        hipExtLaunchKernelGGL(k_sites_check, dim3(grid_of(3*n, 256)), dim3(256), 0, stream, nullptr, defer ? h->checked : nullptr, 0,
+                              b->posq, corr, b->precision, (const int*) h->offsetsDev.ptr, (const int*) h->particlesDev.ptr, n,
+                              h->sites.ptr, listed ? (const double*) h->buildSites.ptr : (const double*) nullptr, h->periodic, box,
+                              0.25*THREE_BODY_SKIN*THREE_BODY_SKIN, h->hostFlag+f, h->hostFlag+(f ^ 1), h->siteForce.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        bool rebuild = !listed;
+        if (defer)
+            h->deferredCheck = true;
+        else if (listed) {
+            IXH_HIP_CHECK(hipStreamSynchronize(stream));
+            rebuild = (h->hostFlag[f] != 0);
+        }
+        if (!rebuild) {
+            finish_plain(h, b, box, cutoff2, flags, stream);
+            return;
+        }
+        CellGrid g = make_grid(h, b->box, listCutoff, stream);
         const int numCells = g.n[0]*g.n[1]*g.n[2];
         h->cellKeys.reserve(n);
         h->cellKeysSorted.reserve(n);
@@ -1400,7 +1657,7 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                                h->cellEnd.ptr);
         }
         IXH_HIP_CHECK(hipGetLastError());
-        const int exactImage = (h->periodic && 2*h->cutoff >= std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
+        const int exactImage = (h->periodic && 2*listCutoff >= std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
         // Lists of a fixed capacity per site, written in one pass; only the longest list comes back to the host (it
         // sizes the triple kernel's LDS), and a list longer than the capacity makes the search run again with more.
         int maxCount = 0;
@@ -1410,13 +1667,12 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                 h->listSites = n;
                 h->start.reserve(n);
                 h->nbrIndex.reserve((size_t) n*h->listCapacity);
-                h->nbrDelta.reserve((size_t) n*h->listCapacity);
                 hipLaunchKernelGGL(k_strided_starts, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->start.ptr, n, h->listCapacity);
                 IXH_HIP_CHECK(hipGetLastError());
             }
-            hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box, cutoff2, g,
-                               exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr, h->listCapacity,
-                               h->nbrIndex.ptr, h->nbrDelta.ptr);
+            hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box,
+                               listCutoff*listCutoff, g, exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr,
+                               h->listCapacity, h->nbrIndex.ptr, (double4*) nullptr);
             IXH_HIP_CHECK(hipGetLastError());
             IXH_HIP_CHECK(rocprim::reduce(h->sortTemp.ptr, maxBytes, h->count.ptr, h->listTotals.ptr, 0, (size_t) n,
                                           rocprim::maximum<int>(), stream));
@@ -1433,23 +1689,33 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         }
         if (maxCount > h->listCapacity)
             throw Error(IXH_ERROR_HIP, "ixh_threebody_execute: neighbour list capacity not reached after a resize");
-        if (!gridFromBox) {
-            hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
+        // the positions the lists hold for, for the checks of the next calls
+        h->buildSites.reserve(3*(size_t) n);
+        IXH_HIP_CHECK(hipMemcpyAsync(h->buildSites.ptr, h->sites.ptr, sizeof(double)*3*(size_t) n, hipMemcpyDeviceToDevice, stream));
+        h->listBuilt = true;
+        h->listMax = maxCount;
+        // the reverse lists: which list entries name each site
+        {
+            const long long entries = (long long) n*h->listCapacity;
+            h->contribution.reserve(3*(size_t) entries);
+            h->revList.reserve((size_t) entries);
+            h->revStart.reserve(n);
+            h->revEnd.reserve(n);
+            h->revCursor.reserve(n);
+            if (h->revCount.count < (size_t) n || h->revCount.ptr == nullptr) {
+                h->revCount.reserve(n);
+                hipLaunchKernelGGL(k_fill_int, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->revCount.ptr, n, 0);
+            }
+            const int grid = (int) std::max(1LL, (entries+255)/256);
+            hipLaunchKernelGGL(k_reverse_lists, dim3(grid), dim3(256), 0, stream, (const int*) h->nbrIndex.ptr, (const int*) h->count.ptr, n,
+                               h->listCapacity, h->revCount.ptr, (int*) nullptr, (int*) nullptr);
+            hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, stream, h->revCount.ptr, n, h->revStart.ptr, h->revEnd.ptr,
+                               h->revCursor.ptr);
+            hipLaunchKernelGGL(k_reverse_lists, dim3(grid), dim3(256), 0, stream, (const int*) h->nbrIndex.ptr, (const int*) h->count.ptr, n,
+                               h->listCapacity, h->revCount.ptr, h->revCursor.ptr, h->revList.ptr);
             IXH_HIP_CHECK(hipGetLastError());
         }
-        if (h->precision == IXH_DOUBLE)
-            launch_triples<double>(h, b, box, cutoff2, maxCount, stream);
-        else
-            launch_triples<float>(h, b, box, cutoff2, maxCount, stream);
-        if (flags & IXH_INCLUDE_FORCES) {
-            hipLaunchKernelGGL(k_apply_site_forces, dim3(grid_of(n, 128)), dim3(128), 0, stream, h->siteForce.ptr, n, h->offsetsDev.ptr,
-                               h->particlesDev.ptr, b->force, b->force_stride);
-            IXH_HIP_CHECK(hipGetLastError());
-        }
-        hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(1024), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, n, b->energy,
-                           h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
-        IXH_HIP_CHECK(hipGetLastError());
-        IXH_HIP_CHECK(hipEventRecord(h->done, stream));
+        finish_plain(h, b, box, cutoff2, flags, stream);
     });
 }
 
@@ -1472,6 +1738,19 @@ int ixh_threebody_reset_kernel_time(ixh_threebody* h) {
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_reset_kernel_time: null handle");
         DeviceGuard guard(h->device);
         h->timer.reset();
+    });
+}
+
+int ixh_threebody_list_was_valid(ixh_threebody* h, int* valid) {
+    return guarded([&] {
+        if (h == nullptr || valid == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_list_was_valid: null argument");
+        *valid = 1;
+        if (h->deferredCheck) {
+            DeviceGuard guard(h->device);
+            IXH_HIP_CHECK(hipEventSynchronize(h->checked));
+            *valid = (h->hostFlag[h->frame] == 0);
+        }
     });
 }
 

@@ -110,6 +110,7 @@ SIGNATURES = {
     "ixh_threebody_destroy": [ctypes.c_void_p],
     "ixh_threebody_set_parameters": [ctypes.c_void_p, ctypes.POINTER(ThreeBodyDesc)],
     "ixh_threebody_execute": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int],
+    "ixh_threebody_list_was_valid": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     "ixh_threebody_last_triple_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
     "ixh_set_profiling": [ctypes.c_int],
     "ixh_pair_kernel_time": [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)],
