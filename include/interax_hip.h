@@ -178,9 +178,33 @@ int ixh_pair_destroy(ixh_pair* pair);
 /*
  * CalcInterax*ForceKernel::copyParametersToContext (ReferenceInteraxKernels.cpp:98-112, 142-148, 185-191): new
  * per-particle parameters and damping scale.  The number of particles and the overlap methods must not change.
- * The long range correction is recomputed.
+ * The long range correction is recomputed.  Synchronous: the host arrays may be reused on return.
  */
 int ixh_pair_set_parameters(ixh_pair* pair, const ixh_pair_desc* desc);
+/*
+ * The same update from per-particle arrays in device memory (no counterpart in the reference, whose
+ * copyParametersToContext reads the Force's host-side parameters, ReferenceInteraxKernels.cpp:98-112, 142-148,
+ * 185-191): for callers whose parameters live on the GPU (fitting loops, alchemical schedules).  The arrays are in the
+ * System's particle order and the descriptor's layout: dispersion m1, m2, m3, alpha, width (5 doubles per particle),
+ * exchange amplitude, width (2), short range K, L, width (3); NULL for a term the handle lacks.  damping_scale is
+ * ignored without dispersion.  The call is asynchronous on `stream`: it copies the arrays there (keep them valid until
+ * that work has run) and rederives, on the device, the evaluation's arrays, their copies in the neighbour list's order
+ * (the list is kept) and the long range correction; it waits for nothing.  The next evaluation, on any stream, is
+ * ordered after it.  Two cases make that evaluation wait for the update on the host first: type tables (SINGLE and
+ * MIXED precision with few parameter classes: the update is checked against the classes, the table rows are formed
+ * from the classes' parameters, and the classes are formed anew only if a particle no longer matches its class) and
+ * one overlap shared by exchange and short range (the widths must still agree).  ixh_pair_long_range_* wait for the
+ * update's correction.  ixh_pair_set_parameters runs the same device work, so both give the same results bit for
+ * bit.  ixh_pair_execute_noalloc returns IXH_ERROR_STATE while an update that needs the host is pending.
+ */
+typedef struct {
+    const double* dispersion;
+    const double* exchange;
+    const double* short_range;
+    double damping_scale;
+    void* stream;   /* hipStream_t */
+} ixh_pair_device_parameters;
+int ixh_pair_set_parameters_device(ixh_pair* pair, const ixh_pair_device_parameters* parameters);
 /*
  * CalcInterax*ForceKernel::execute (ReferenceInteraxKernels.cpp:79-96, 127-140, 170-183): add the forces and
  * energy of the selected terms to the buffers.  The long range correction is not included: it is returned by
@@ -286,6 +310,24 @@ int ixh_threebody_destroy(ixh_threebody* threebody);
 /* copyParametersToContext (ReferenceInteraxKernels.cpp:215-217): alpha, frequency, width, scale factor, damping
    scale and a nonzero centroid damping radius may change. */
 int ixh_threebody_set_parameters(ixh_threebody* threebody, const ixh_threebody_desc* desc);
+/*
+ * The same update from per-site arrays in device memory (no counterpart in the reference, whose copyParametersToContext
+ * reads the Force's host-side parameters, ReferenceInteraxKernels.cpp:215-217): alpha, frequency and width, FP64, one
+ * per site in site order, plus the scalars.  Asynchronous on `stream` (keep the arrays valid until that work has run);
+ * the neighbour lists are kept.  The next evaluation, on any stream, is ordered after it; with Tang-Toennies damping
+ * that evaluation first waits for the update's check of the widths on the device and fails with
+ * IXH_ERROR_INVALID_ARGUMENT for a width that is not positive.  ixh_threebody_set_parameters runs the same device work.
+ */
+typedef struct {
+    const double* alpha;
+    const double* frequency;
+    const double* width;
+    double scale_factor;
+    double damping_scale;
+    double centroid_damping_radius;   /* may change, but not between zero and nonzero */
+    void* stream;                     /* hipStream_t */
+} ixh_threebody_device_parameters;
+int ixh_threebody_set_parameters_device(ixh_threebody* threebody, const ixh_threebody_device_parameters* parameters);
 /* execute (ReferenceInteraxKernels.cpp:210-213). */
 int ixh_threebody_execute(ixh_threebody* threebody, const ixh_buffers* buffers, int flags);
 /* Batches of independent nonperiodic systems (no counterpart in the reference): segment s owns sites

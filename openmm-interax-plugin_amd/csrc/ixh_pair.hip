@@ -1308,7 +1308,7 @@ __global__ void __launch_bounds__(256) k_reduce_segments(const double* tileEnerg
 
 struct LrcArgs {
     const double* classes;   // 7 per class: m1, m2, m3, alpha, width, alchemical, count
-    long long numClasses;
+    const long long* numClasses;   // device
     double cutoff, switchDistance, switchWidth, dampingScale;
     int useSwitch;
     double nodes[32], weights[32];
@@ -1327,10 +1327,12 @@ __device__ double lrc_pair_energy(double r, const double* c1, const double* c2, 
     return e;
 }
 
-__global__ void k_long_range(LrcArgs a, double* partial /* 2 per thread */) {
+constexpr int LRC_THREADS = 256, LRC_BLOCKS = 2048;
+
+__global__ void __launch_bounds__(LRC_THREADS) k_long_range(LrcArgs a, double* partial /* 2 per workgroup */) {
     long long tid = (long long) blockIdx.x*blockDim.x+threadIdx.x;
     long long nthreads = (long long) gridDim.x*blockDim.x;
-    long long C = a.numClasses;
+    long long C = *a.numClasses;
     long long totalPairs = C*(C+1)/2;
     long long begin = totalPairs*tid/nthreads, end = totalPairs*(tid+1)/nthreads;
     double sum = 0, alch = 0;
@@ -1375,8 +1377,24 @@ __global__ void k_long_range(LrcArgs a, double* partial /* 2 per thread */) {
             }
         }
     }
-    partial[2*tid] = sum;
-    partial[2*tid+1] = alch;
+    // workgroup sums in a fixed order
+    __shared__ double sSum[LRC_THREADS/64], sAlch[LRC_THREADS/64];
+    sum = wave_sum(sum);
+    alch = wave_sum(alch);
+    if (threadIdx.x%64 == 0) {
+        sSum[threadIdx.x/64] = sum;
+        sAlch[threadIdx.x/64] = alch;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0, t = 0;
+        for (int w = 0; w < LRC_THREADS/64; w++) {
+            s += sSum[w];
+            t += sAlch[w];
+        }
+        partial[2*blockIdx.x] = s;
+        partial[2*blockIdx.x+1] = t;
+    }
 }
 
 void gauss_legendre(int n, double* nodes, double* weights) {
@@ -1415,6 +1433,203 @@ BoxD make_box(const double* box) {
 
 int grid_for(long long n, int threads) {
     return (int) std::max(1LL, (n+threads-1)/threads);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Parameter pipeline: every parameter update, from host arrays (ixh_pair_set_parameters) or device arrays
+// (ixh_pair_set_parameters_device), lands in FP64 device arrays in the reference's per-particle order, from which
+// these kernels derive the evaluation's arrays, check the type-table classes, and compute the long range correction
+// with the classes grouped on the device.
+// ------------------------------------------------------------------------------------------------------------
+
+struct DeriveArgs {
+    const double* disp;      // 5 per particle (m1, m2, m3, alpha, width), or nullptr
+    const double* exch;      // 2 per particle (amplitude, width), or nullptr
+    const double* sr;        // 3 per particle (k, l, width), or nullptr
+    int n;
+    int precision;
+    int useTables;
+    int shared;              // one overlap for exchange and short range (both present): their widths must be equal
+    const int* cls;          // type tables: dispersion | exchange << 8 | short range << 16 class of every particle
+    const int* reps;         // type tables: the first particle of every dispersion, exchange, short-range class
+    int repExch, repSr;      // offsets of the exchange and short-range classes in reps
+    void* A;                 // the evaluation's per-particle vectors (float4 or double4), particle order
+    void* B;
+    void* C;
+    double* lambdaParams;    // 12 per particle (PairParticle order)
+    int* mismatch;           // type tables: set when a particle no longer has the parameters of its class's first one
+};
+
+__global__ void k_derive_params(DeriveArgs a) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= a.n)
+        return;
+    double A[4] = {0, 0, 0, 0}, B[4] = {0, 0, 0, 0}, C[4] = {0, 0, 0, 0};
+    if (a.disp != nullptr) {
+        for (int k = 0; k < 4; k++)
+            A[k] = a.disp[5*(size_t) i+k];
+        B[0] = a.disp[5*(size_t) i+4];
+    }
+    if (a.exch != nullptr) {
+        B[1] = a.exch[2*(size_t) i];
+        B[2] = a.exch[2*(size_t) i+1];
+        B[3] = (B[2] != 0 ? 1.0/B[2] : 0.0);
+    }
+    if (a.sr != nullptr) {
+        for (int k = 0; k < 3; k++)
+            C[k] = a.sr[3*(size_t) i+k];
+        C[3] = (C[2] != 0 ? 1.0/C[2] : 0.0);
+    }
+    double* q = a.lambdaParams+12*(size_t) i;
+    for (int k = 0; k < 4; k++) {
+        q[k] = A[k];
+        q[4+k] = B[k];
+        q[8+k] = C[k];
+    }
+    // one overlap for both terms needs equal widths; with type tables, every particle must still equal its class's
+    // first particle in every parameter the class stands for
+    bool same = (!a.shared || B[2] == C[2]);
+    if (a.useTables) {
+        const int c = a.cls[i];
+        if (a.disp != nullptr) {
+            const int r = a.reps[c & 0xff];
+            for (int k = 0; k < 4; k++)
+                same = same && (a.disp[5*(size_t) r+k] == A[k]);
+        }
+        if (a.exch != nullptr) {
+            const int r = a.reps[a.repExch+((c >> 8) & 0xff)];
+            same = same && a.exch[2*(size_t) r] == B[1] && a.exch[2*(size_t) r+1] == B[2];
+            if (a.shared)
+                same = same && a.sr[3*(size_t) r] == C[0] && a.sr[3*(size_t) r+1] == C[1];
+        }
+        if (a.sr != nullptr && !a.shared) {
+            const int r = a.reps[a.repSr+((c >> 16) & 0xff)];
+            for (int k = 0; k < 3; k++)
+                same = same && (a.sr[3*(size_t) r+k] == C[k]);
+        }
+    }
+    if (!same)
+        *a.mismatch = 1;
+    if (a.precision == IXH_DOUBLE) {
+        ((double4*) a.A)[i] = make_double4(A[0], A[1], A[2], A[3]);
+        ((double4*) a.B)[i] = make_double4(B[0], B[1], B[2], B[3]);
+        ((double4*) a.C)[i] = make_double4(C[0], C[1], C[2], C[3]);
+        return;
+    }
+    // with type tables the first vector holds the dispersion width only (the classes carry the rest)
+    if (a.useTables)
+        ((float4*) a.A)[i] = make_float4((float) B[0], 0.0f, 0.0f, 0.0f);
+    else
+        ((float4*) a.A)[i] = make_float4((float) A[0], (float) A[1], (float) A[2], (float) A[3]);
+    ((float4*) a.B)[i] = make_float4((float) B[0], (float) B[1], (float) B[2], (float) B[3]);
+    ((float4*) a.C)[i] = make_float4((float) C[0], (float) C[1], (float) C[2], (float) C[3]);
+}
+
+__global__ void k_clear_int(int* p) {
+    *p = 0;
+}
+
+// The FP64 parameters of the class representatives (10 per representative: 5 dispersion, 2 exchange, 3 short range),
+// from which the host forms the type tables.
+__global__ void k_gather_reps(const double* disp, const double* exch, const double* sr, const int* reps, int count, double* out) {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= count)
+        return;
+    const int i = reps[k];
+    double* o = out+10*k;
+    for (int c = 0; c < 5; c++)
+        o[c] = (disp != nullptr ? disp[5*(size_t) i+c] : 0.0);
+    for (int c = 0; c < 2; c++)
+        o[5+c] = (exch != nullptr ? exch[2*(size_t) i+c] : 0.0);
+    for (int c = 0; c < 3; c++)
+        o[7+c] = (sr != nullptr ? sr[3*(size_t) i+c] : 0.0);
+}
+
+// Dispersion classes of the long range correction: particles of identical (m1, m2, m3, alpha, width, alchemical), in
+// the lexicographic order of that tuple (InteraxDispersionForceImpl.cpp:94-101), found on the device by six stable
+// radix sorts of the particle order, least significant field first.  A double's bits, with the sign bit flipped (or
+// all bits, for a negative value), sort as the value does; -0.0 is taken as +0.0.
+__device__ __forceinline__ double class_field(const double* disp, const int* flags, int i, int field) {
+    return (field < 5 ? disp[5*(size_t) i+field] : ((flags[i] & ALCHEMICAL_BIT) != 0 ? 1.0 : 0.0));
+}
+
+__global__ void k_lrc_keys(const double* disp, const int* flags, int n, int field, const int* perm, unsigned long long* keys) {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= n)
+        return;
+    double x = class_field(disp, flags, perm[k], field);
+    if (x == 0.0)
+        x = 0.0;
+    const unsigned long long bits = (unsigned long long) __double_as_longlong(x);
+    keys[k] = ((bits >> 63) != 0 ? ~bits : bits | 0x8000000000000000ULL);
+}
+
+__global__ void k_iota(int* p, int n) {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k < n)
+        p[k] = k;
+}
+
+// first[k] = 1 where sorted position k starts a class
+__global__ void k_lrc_bounds(const double* disp, const int* flags, const int* perm, int n, int* first) {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= n)
+        return;
+    bool start = (k == 0);
+    for (int f = 0; f < 6 && !start; f++)
+        start = class_field(disp, flags, perm[k], f) != class_field(disp, flags, perm[k-1], f);
+    first[k] = start ? 1 : 0;
+}
+
+// index[k]: the inclusive prefix sum of first[]: class index[k] - 1 holds sorted position k.  Writes the first sorted
+// position of every class, the class count and classStart[count] = n.
+__global__ void k_lrc_starts(const int* first, const int* index, int n, int* classStart, long long* numClasses) {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= n)
+        return;
+    if (first[k])
+        classStart[index[k]-1] = k;
+    if (k == n-1) {
+        *numClasses = index[k];
+        classStart[index[k]] = n;
+    }
+}
+
+// 7 values per class: m1, m2, m3, alpha, width, alchemical, count
+__global__ void k_lrc_classes(const double* disp, const int* flags, const int* perm, const int* classStart,
+                              const long long* numClasses, double* classes) {
+    const int c = blockIdx.x*blockDim.x+threadIdx.x;
+    if (c >= *numClasses)
+        return;
+    const int i = perm[classStart[c]];
+    for (int f = 0; f < 6; f++)
+        classes[7*(size_t) c+f] = class_field(disp, flags, i, f);
+    classes[7*(size_t) c+6] = (double) (classStart[c+1]-classStart[c]);
+}
+
+// The correction's coefficients from the partial sums of k_long_range (two per thread), summed in a fixed order:
+// out = normalization (sum, alchemical sum) (InteraxDispersionForceImpl.cpp:145-148).
+__global__ void __launch_bounds__(1024) k_lrc_reduce(const double* partial, int threads, double normalization, double* out) {
+    __shared__ double sS[1024], sA[1024];
+    double sum = 0, alch = 0;
+    for (int t = threadIdx.x; t < threads; t += 1024) {
+        sum += partial[2*(size_t) t];
+        alch += partial[2*(size_t) t+1];
+    }
+    sS[threadIdx.x] = sum;
+    sA[threadIdx.x] = alch;
+    __syncthreads();
+    for (int stride = 512; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) {
+            sS[threadIdx.x] += sS[threadIdx.x+stride];
+            sA[threadIdx.x] += sA[threadIdx.x+stride];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = normalization*sS[0];
+        out[1] = normalization*sA[0];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------------------
@@ -1611,11 +1826,37 @@ struct ixh_pair {
     int exchangeMethod = 0, shortRangeMethod = 0;
     std::vector<unsigned char> alchemical;
     std::vector<int> exclusions;
-    std::vector<double> dispersion, exchange, shortRange;
     double dampingScale = 1;
     int useLRC = 0;
     double lrcCoefficient = 0, lrcAlchemical = 0;
     PairSettings settings;
+    // Parameter updates.  Every update writes the FP64 per-particle parameters in the reference's layout (rawDisp 5,
+    // rawExch 2, rawSr 3 per particle) and runs the same device work: derived arrays, their slot-ordered copies,
+    // class check, long range correction.  updateDone marks its end; what the host still has to do (type-table rows,
+    // new classes after a mismatch, reading the correction) happens at the next call that needs it.
+    DeviceArray<double> rawDisp, rawExch, rawSr;
+    hipEvent_t updateDone = nullptr;
+    bool updateRecorded = false;
+    bool updatePending = false;    // the next evaluation must wait for updateDone
+    bool updateNeedsHost = false;  // ... and, before that, check the classes and form the type-table rows
+    bool lrcPending = false;       // the coefficients are in staging once updateDone has completed
+    DeviceArray<int> mismatch;
+    struct Staging {               // pinned
+        double lrc[2];
+        int mismatch;
+        int unused;
+        double reps[10*(MAX_DISPERSION_CLASSES+2*MAX_WIDTH_CLASSES)];
+    }* staging = nullptr;
+    // type tables: the first particle of every dispersion, exchange and short-range class, and their parameters
+    DeviceArray<int> reps;
+    DeviceArray<double> repParams;
+    int numReps = 0, repExch = 0, repSr = 0;
+    // long range correction: classes grouped on the device
+    DeviceArray<unsigned long long> lrcKeys[2];
+    DeviceArray<int> lrcPerm[2], lrcFirst, lrcIndex, lrcStart;
+    DeviceArray<long long> lrcCount;
+    DeviceArray<double> lrcClasses, lrcPartial, lrcValues;
+    DeviceArray<char> lrcTemp;
 
     int numBlocks = 0, padded = 0;
     bool built = false;
@@ -1629,6 +1870,7 @@ struct ixh_pair {
     DeviceArray<double> centerFrame[2];
     hipEvent_t frameDone[2] = {nullptr, nullptr};   // the end of the last call that used each frame
     hipStream_t frameStream[2] = {nullptr, nullptr};   // the stream of that call
+    bool frameUsed[2] = {false, false};
     int frame = 0;
     DeviceArray<int> tileImage;
     DeviceArray<int> flags, sortedFlags, perm, slotOf, exclusionOffsets, exclusionPartners, blockStart, blockCount, blockList,
@@ -1669,9 +1911,8 @@ struct ixh_pair {
     DeviceArray<unsigned long long> masks;
     DeviceArray<char> sortTemp;
     DeviceArray<long long> pairCount;
-    // energies at several lambda values: generic FP32 parameters in particle order, the alchemical particles, and the
-    // scratch buffers of the handle's own evaluation
-    int paramVersion = 0, lambdaParamVersion = -1;
+    // energies at several lambda values: FP64 parameters in particle order (12 per particle, written by every
+    // parameter update), the alchemical particles, and the scratch buffers of the handle's own evaluation
     DeviceArray<double> lambdaParams;
     DeviceArray<int> alchList;
     int numAlchemical = 0;
@@ -1831,230 +2072,409 @@ void overlap_rows(double a, double b, const OverlapScheme& s, OverlapRow rows[2]
     }
 }
 
-// Parameter classes and type tables.  Returns false, leaving the per-pair path in use, when the particles fall into
-// more classes than the LDS tables hold.  packed receives wd, amp, K/2, L/2 per particle.
-bool build_tables(ixh_pair* p, std::vector<float>& packed, std::vector<int>& cls, std::vector<float>& tables) {
+// Whether one overlap serves both overlap terms: both present, the same scheme, and equal widths everywhere.
+bool shared_overlap(const ixh_pair* p, const double* exch, const double* sr) {
+    bool shared = (p->terms & IXH_TERM_EXCHANGE) && (p->terms & IXH_TERM_SHORT_RANGE) && p->exchangeMethod == p->shortRangeMethod;
+    for (int i = 0; shared && i < p->n; i++)
+        shared = (exch[2*(size_t) i+1] == sr[3*(size_t) i+2]);
+    return shared;
+}
+
+// Parameter classes of the type tables (FP32 modes): particles of identical dispersion (m1, m2, m3, alpha), of
+// identical exchange (amplitude, width), of identical short range (K, L, width) parameters, or with one overlap of
+// identical (amplitude, width, K, L).  Fills cls (dispersion | exchange << 8 | short range << 16 class per particle)
+// and reps (the first particle of every dispersion, exchange and short-range class) and returns true; returns false,
+// leaving the per-pair path in use, when the particles fall into more classes than the LDS tables hold.
+bool form_classes(ixh_pair* p, const double* disp, const double* exch, const double* sr, std::vector<int>& cls, std::vector<int>& reps) {
     const int n = p->n;
-    p->highOrder = 0;
-    p->singleBranch = 0;
     typedef std::tuple<double, double, double, double> Key;
     std::map<Key, int> dispMap, exchMap, srMap;
-    std::vector<Key> dispKeys, exchKeys, srKeys;    // exchange: (width, amplitude); short range: (width, K, L)
-    cls.assign(n, 0);
-    packed.assign(4*(size_t) n, 0.0f);
-    auto classOf = [](std::map<Key, int>& m, std::vector<Key>& keys, const Key& k) {
+    std::vector<int> dispReps, exchReps, srReps;
+    cls.assign(std::max(n, 1), 0);
+    auto classOf = [](std::map<Key, int>& m, std::vector<int>& first, const Key& k, int i) {
         auto it = m.find(k);
         if (it != m.end())
             return it->second;
-        int c = (int) keys.size();
+        const int c = (int) first.size();
         m.emplace(k, c);
-        keys.push_back(k);
+        first.push_back(i);
         return c;
     };
     const bool shared = p->settings.shared_overlap != 0;
+    const bool ex = (p->terms & IXH_TERM_EXCHANGE) != 0, srTerm = (p->terms & IXH_TERM_SHORT_RANGE) != 0;
     for (int i = 0; i < n; i++) {
         int d = 0, x = 0, s = 0;
         if (p->terms & IXH_TERM_DISPERSION) {
-            const double* q = &p->dispersion[5*(size_t) i];
-            d = classOf(dispMap, dispKeys, std::make_tuple(q[0], q[1], q[2], q[3]));
-            packed[4*i] = (float) q[4];
+            const double* q = &disp[5*(size_t) i];
+            d = classOf(dispMap, dispReps, std::make_tuple(q[0], q[1], q[2], q[3]), i);
         }
-        const bool ex = (p->terms & IXH_TERM_EXCHANGE) != 0, sr = (p->terms & IXH_TERM_SHORT_RANGE) != 0;
-        if (shared) {
-            // one class of (width, amplitude, K, L) for both overlap terms
-            x = s = classOf(exchMap, exchKeys, std::make_tuple(p->exchange[2*i+1], p->exchange[2*i], p->shortRange[3*i],
-                                                               p->shortRange[3*i+1]));
-        }
+        if (shared)
+            x = s = classOf(exchMap, exchReps, std::make_tuple(exch[2*(size_t) i+1], exch[2*(size_t) i], sr[3*(size_t) i],
+                                                               sr[3*(size_t) i+1]), i);
         else {
             if (ex)
-                x = classOf(exchMap, exchKeys, std::make_tuple(p->exchange[2*i+1], p->exchange[2*i], 0.0, 0.0));
-            if (sr)
-                s = classOf(srMap, srKeys, std::make_tuple(p->shortRange[3*i+2], 0.0, p->shortRange[3*i], p->shortRange[3*i+1]));
+                x = classOf(exchMap, exchReps, std::make_tuple(exch[2*(size_t) i+1], exch[2*(size_t) i], 0.0, 0.0), i);
+            if (srTerm)
+                s = classOf(srMap, srReps, std::make_tuple(sr[3*(size_t) i+2], 0.0, sr[3*(size_t) i], sr[3*(size_t) i+1]), i);
         }
-        if (dispKeys.size() > (size_t) MAX_DISPERSION_CLASSES || exchKeys.size() > (size_t) MAX_WIDTH_CLASSES ||
-            srKeys.size() > (size_t) MAX_WIDTH_CLASSES)
+        if (dispReps.size() > (size_t) MAX_DISPERSION_CLASSES || exchReps.size() > (size_t) MAX_WIDTH_CLASSES ||
+            srReps.size() > (size_t) MAX_WIDTH_CLASSES)
             return false;
         cls[i] = d | (x << 8) | (s << 16);
     }
-    p->numDisp = std::max<int>(1, (int) dispKeys.size());
-    p->numExch = std::max<int>(1, (int) exchKeys.size());
-    p->numSr = (shared ? p->numExch : std::max<int>(1, (int) srKeys.size()));
+    for (std::vector<int>* r : {&dispReps, &exchReps, &srReps})
+        if (r->empty())
+            r->push_back(0);
+    p->numDisp = (int) dispReps.size();
+    p->numExch = (int) exchReps.size();
+    p->numSr = (shared ? p->numExch : (int) srReps.size());
     p->exchOffset = 4*p->numDisp*p->numDisp;
-    int exchFloats = OVERLAP_PAIR_FLOATS*p->numExch*p->numExch;
+    const int exchFloats = OVERLAP_PAIR_FLOATS*p->numExch*p->numExch;
     p->srOffset = (shared ? p->exchOffset : p->exchOffset+exchFloats);
     p->tableFloats = p->srOffset+(shared ? exchFloats : OVERLAP_PAIR_FLOATS*p->numSr*p->numSr);
+    reps = dispReps;
+    reps.insert(reps.end(), exchReps.begin(), exchReps.end());
+    p->repExch = p->numDisp;
+    p->repSr = p->repExch;
+    if (!shared) {
+        p->repSr = p->repExch+p->numExch;
+        reps.insert(reps.end(), srReps.begin(), srReps.end());
+    }
+    p->numReps = (int) reps.size();
+    return true;
+}
+
+// The rows of the type tables from the parameters of the class representatives (10 per representative: dispersion
+// m1, m2, m3, alpha, width; exchange amplitude, width; short range K, L, width).  Also sets highOrder and singleBranch.
+void form_tables(ixh_pair* p, const double* rec, std::vector<float>& tables) {
+    p->highOrder = 0;
     tables.assign(p->tableFloats, 0.0f);
     // dispersion rows: C6, C8, C10 combined as in InteraxPairTerms.h:271-279, and the gate alpha_i alpha_j != 0
-    for (int ci = 0; ci < (int) dispKeys.size(); ci++)
-        for (int cj = 0; cj < (int) dispKeys.size(); cj++) {
-            double m1i, m2i, m3i, ai, m1j, m2j, m3j, aj;
-            std::tie(m1i, m2i, m3i, ai) = dispKeys[ci];
-            std::tie(m1j, m2j, m3j, aj) = dispKeys[cj];
-            double pre = ai*aj/std::max(ai*m1j+aj*m1i, 1e-30);
-            float* row = &tables[4*(ci*p->numDisp+cj)];
-            row[0] = (float) (pre*m1i*m1j);
-            row[1] = (float) (1.5*pre*(m1i*m2j+m2i*m1j));
-            row[2] = (float) (pre*(2*(m1i*m3j+m3i*m1j)+4.2*m2i*m2j));
-            row[3] = (ai*aj != 0 ? 1.0f : 0.0f);
-        }
-    // overlap rows of a class pair, with the amplitude product (exchange) and K, L averages (short range)
-    auto fill = [&](const std::vector<Key>& keys, int offset, int count, const OverlapScheme& scheme) {
-        for (int ci = 0; ci < (int) keys.size(); ci++)
-            for (int cj = 0; cj < (int) keys.size(); cj++) {
+    if (p->terms & IXH_TERM_DISPERSION)
+        for (int ci = 0; ci < p->numDisp; ci++)
+            for (int cj = 0; cj < p->numDisp; cj++) {
+                const double* qi = rec+10*ci;
+                const double* qj = rec+10*cj;
+                const double m1i = qi[0], m2i = qi[1], m3i = qi[2], ai = qi[3];
+                const double m1j = qj[0], m2j = qj[1], m3j = qj[2], aj = qj[3];
+                const double pre = ai*aj/std::max(ai*m1j+aj*m1i, 1e-30);
+                float* row = &tables[4*(ci*p->numDisp+cj)];
+                row[0] = (float) (pre*m1i*m1j);
+                row[1] = (float) (1.5*pre*(m1i*m2j+m2i*m1j));
+                row[2] = (float) (pre*(2*(m1i*m3j+m3i*m1j)+4.2*m2i*m2j));
+                row[3] = (ai*aj != 0 ? 1.0f : 0.0f);
+            }
+    // overlap rows of a class pair, with the amplitude product (exchange) and K, L averages (short range); a class
+    // key is (width, amplitude, K, L)
+    const bool shared = p->settings.shared_overlap != 0;
+    auto key = [&](int rep, bool exchange, bool shortRange) {
+        const double* q = rec+10*rep;
+        return std::make_tuple(exchange ? q[6] : q[9], exchange ? q[5] : 0.0, shortRange ? q[7] : 0.0, shortRange ? q[8] : 0.0);
+    };
+    auto fill = [&](int firstRep, int count, bool exchange, bool shortRange, int offset, const OverlapScheme& scheme) {
+        for (int ci = 0; ci < count; ci++)
+            for (int cj = 0; cj < count; cj++) {
+                const auto ki = key(firstRep+ci, exchange, shortRange), kj = key(firstRep+cj, exchange, shortRange);
                 OverlapRow w[2];
-                overlap_rows(std::get<0>(keys[ci]), std::get<0>(keys[cj]), scheme, w);
+                overlap_rows(std::get<0>(ki), std::get<0>(kj), scheme, w);
                 for (OverlapRow& row : w) {
                     if (row.high != 0.0f)
                         p->highOrder = 1;
-                    row.amplitude = (float) (std::get<1>(keys[ci])*std::get<1>(keys[cj]));
-                    row.kk = (float) (0.5*(std::get<2>(keys[ci])+std::get<2>(keys[cj])));
-                    row.ll = (float) (0.5*(std::get<3>(keys[ci])+std::get<3>(keys[cj])));
+                    row.amplitude = (float) (std::get<1>(ki)*std::get<1>(kj));
+                    row.kk = (float) (0.5*(std::get<2>(ki)+std::get<2>(kj)));
+                    row.ll = (float) (0.5*(std::get<3>(ki)+std::get<3>(kj)));
                 }
                 std::memcpy(&tables[offset+OVERLAP_PAIR_FLOATS*(ci*count+cj)], w, sizeof(w));
             }
     };
-    if (p->terms & (IXH_TERM_EXCHANGE | IXH_TERM_SHORT_RANGE)) {
-        if (shared)
-            fill(exchKeys, p->exchOffset, p->numExch, p->settings.exchange_scheme);
-        else {
-            if (p->terms & IXH_TERM_EXCHANGE)
-                fill(exchKeys, p->exchOffset, p->numExch, p->settings.exchange_scheme);
-            if (p->terms & IXH_TERM_SHORT_RANGE)
-                fill(srKeys, p->srOffset, p->numSr, p->settings.short_range_scheme);
-        }
+    if (shared)
+        fill(p->repExch, p->numExch, true, true, p->exchOffset, p->settings.exchange_scheme);
+    else {
+        if (p->terms & IXH_TERM_EXCHANGE)
+            fill(p->repExch, p->numExch, true, false, p->exchOffset, p->settings.exchange_scheme);
+        if (p->terms & IXH_TERM_SHORT_RANGE)
+            fill(p->repSr, p->numSr, false, true, p->srOffset, p->settings.short_range_scheme);
     }
     // rows of a width pair that takes one branch only are equal: the kernel then reads the first without selecting
     p->singleBranch = 1;
     for (int offset = p->exchOffset; offset < p->tableFloats; offset += OVERLAP_PAIR_FLOATS)
         if (std::memcmp(&tables[offset], &tables[offset+OVERLAP_PAIR_FLOATS/2], sizeof(OverlapRow)) != 0)
             p->singleBranch = 0;
-    return true;
 }
 
-// Copy the per-particle parameters of a descriptor into the handle and onto the device.
-void load_parameters(ixh_pair* p, const ixh_pair_desc* d, hipStream_t stream) {
-    int n = p->n;
-    p->paramVersion++;
-    if (d->terms & IXH_TERM_DISPERSION) {
-        p->dispersion.assign(d->dispersion, d->dispersion+5*(size_t) n);
-        p->dampingScale = d->damping_scale;
-    }
-    if (d->terms & IXH_TERM_EXCHANGE)
-        p->exchange.assign(d->exchange, d->exchange+2*(size_t) n);
-    if (d->terms & IXH_TERM_SHORT_RANGE)
-        p->shortRange.assign(d->short_range, d->short_range+3*(size_t) n);
-    bool doublePrecision = (p->precision == IXH_DOUBLE);
-    p->settings.exchange_scheme = make_scheme(p->exchangeMethod, doublePrecision);
-    p->settings.short_range_scheme = make_scheme(p->shortRangeMethod, doublePrecision);
-    p->settings.damping_scale = p->dampingScale;
-    p->settings.damping_scale_f = (float) p->dampingScale;
-    // One overlap per pair when both forces have the same widths and scheme.
-    bool shared = (p->terms & IXH_TERM_EXCHANGE) && (p->terms & IXH_TERM_SHORT_RANGE) && p->exchangeMethod == p->shortRangeMethod;
-    for (int i = 0; shared && i < n; i++)
-        shared = (p->exchange[2*i+1] == p->shortRange[3*i+2]);
-    p->settings.shared_overlap = shared ? 1 : 0;
-
-    std::vector<double> A(4*(size_t) n, 0.0), B(4*(size_t) n, 0.0), C(4*(size_t) n, 0.0);
-    std::vector<int> flags(n);
-    for (int i = 0; i < n; i++) {
-        if (p->terms & IXH_TERM_DISPERSION) {
-            for (int k = 0; k < 4; k++)
-                A[4*i+k] = p->dispersion[5*i+k];
-            B[4*i] = p->dispersion[5*i+4];
-        }
-        if (p->terms & IXH_TERM_EXCHANGE) {
-            B[4*i+1] = p->exchange[2*i];
-            B[4*i+2] = p->exchange[2*i+1];
-            B[4*i+3] = (p->exchange[2*i+1] != 0 ? 1.0/p->exchange[2*i+1] : 0.0);
-        }
-        if (p->terms & IXH_TERM_SHORT_RANGE) {
-            C[4*i] = p->shortRange[3*i];
-            C[4*i+1] = p->shortRange[3*i+1];
-            C[4*i+2] = p->shortRange[3*i+2];
-            C[4*i+3] = (p->shortRange[3*i+2] != 0 ? 1.0/p->shortRange[3*i+2] : 0.0);
-        }
-        flags[i] = i | (p->alchemical[i] ? ALCHEMICAL_BIT : 0);
-    }
-    if (p->precision == IXH_DOUBLE) {
-        p->paramA.upload((const char*) A.data(), sizeof(double)*A.size(), stream);
-        p->paramB.upload((const char*) B.data(), sizeof(double)*B.size(), stream);
-        p->paramC.upload((const char*) C.data(), sizeof(double)*C.size(), stream);
-    }
-    else {
-        std::vector<float> Af(A.begin(), A.end()), Bf(B.begin(), B.end()), Cf(C.begin(), C.end());
-        std::vector<float> packed, tables;
-        std::vector<int> cls;
-        p->useTables = build_tables(p, packed, cls, tables);
-        if (p->useTables) {
-            // with type tables the first parameter vector holds wd, amp, K/2, L/2
-            Af.swap(packed);
-            if (cls.empty())
-                cls.push_back(0);
-            p->classes.upload(cls.data(), cls.size(), stream);
-            p->tables.upload(tables.data(), tables.size(), stream);
-        }
-        p->paramA.upload((const char*) Af.data(), sizeof(float)*Af.size(), stream);
-        p->paramB.upload((const char*) Bf.data(), sizeof(float)*Bf.size(), stream);
-        p->paramC.upload((const char*) Cf.data(), sizeof(float)*Cf.size(), stream);
-    }
-    p->flags.upload(flags.data(), flags.size(), stream);
-    IXH_HIP_CHECK(hipStreamSynchronize(stream));
-    p->built = false;   // the slot-ordered copies are refreshed by the next build
-}
-
-// Long range correction coefficients on the device.
-void compute_long_range(ixh_pair* p, hipStream_t stream) {
-    p->lrcCoefficient = 0;
-    p->lrcAlchemical = 0;
-    if (!(p->terms & IXH_TERM_DISPERSION) || p->method != IXH_CUTOFF_PERIODIC || !p->useLRC)
+// Classes, shared-overlap choice and type-table layout from host copies of the parameters (at creation, and when an
+// update no longer fits the classes).  Synchronous.
+void classify(ixh_pair* p, const double* disp, const double* exch, const double* sr) {
+    p->settings.shared_overlap = shared_overlap(p, exch, sr) ? 1 : 0;
+    p->useTables = false;
+    p->numReps = 0;
+    if (p->precision == IXH_DOUBLE)
         return;
-    int n = p->n;
-    // classes of identical (m1, m2, m3, alpha, width, alchemical), in the order of a std::map over the tuple
-    std::map<std::tuple<double, double, double, double, double, bool>, int> counts;
-    for (int i = 0; i < n; i++) {
-        const double* q = &p->dispersion[5*(size_t) i];
-        counts[std::make_tuple(q[0], q[1], q[2], q[3], q[4], (bool) p->alchemical[i])]++;
+    std::vector<int> cls, reps;
+    p->useTables = form_classes(p, disp, exch, sr, cls, reps);
+    if (p->useTables) {
+        p->classes.reserve(cls.size());
+        IXH_HIP_CHECK(hipMemcpy(p->classes.ptr, cls.data(), sizeof(int)*cls.size(), hipMemcpyHostToDevice));
+        p->reps.reserve(reps.size());
+        IXH_HIP_CHECK(hipMemcpy(p->reps.ptr, reps.data(), sizeof(int)*reps.size(), hipMemcpyHostToDevice));
     }
-    std::vector<double> classes;
-    classes.reserve(7*counts.size());
-    for (auto& c : counts) {
-        classes.push_back(std::get<0>(c.first));
-        classes.push_back(std::get<1>(c.first));
-        classes.push_back(std::get<2>(c.first));
-        classes.push_back(std::get<3>(c.first));
-        classes.push_back(std::get<4>(c.first));
-        classes.push_back(std::get<5>(c.first) ? 1.0 : 0.0);
-        classes.push_back((double) c.second);
+}
+
+// The slot-ordered copies of the per-particle arrays (after a list build, or a parameter update of a built list).
+void gather_sorted(ixh_pair* p, hipStream_t stream) {
+    if (p->precision == IXH_DOUBLE)
+        hipLaunchKernelGGL(k_gather_params<double>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
+                           (const double4*) p->paramA.ptr, (const double4*) p->paramB.ptr, (const double4*) p->paramC.ptr, p->flags.ptr,
+                           (double4*) p->sortedA.ptr, (double4*) p->sortedB.ptr, (double4*) p->sortedC.ptr, p->sortedFlags.ptr);
+    else
+        hipLaunchKernelGGL(k_gather_params<float>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
+                           (const float4*) p->paramA.ptr, (const float4*) p->paramB.ptr, (const float4*) p->paramC.ptr, p->flags.ptr,
+                           (float4*) p->sortedA.ptr, (float4*) p->sortedB.ptr, (float4*) p->sortedC.ptr, p->sortedFlags.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    if (p->useTables) {
+        p->sortedClasses.reserve(p->padded);
+        hipLaunchKernelGGL(k_gather_int, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
+                           (const int*) p->classes.ptr, p->sortedClasses.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
     }
-    DeviceArray<double> classesDev, partial;
-    classesDev.upload(classes.data(), classes.size(), stream);
+}
+
+// The evaluation's arrays from the FP64 parameters, the class check, the representatives' parameters (type tables),
+// and the slot-ordered copies when a list is built.
+void derive_parameters(ixh_pair* p, hipStream_t stream) {
+    const int n = p->n;
+    const size_t r4 = (p->precision == IXH_DOUBLE ? sizeof(double4) : sizeof(float4));
+    p->paramA.reserve(r4*std::max(n, 1));
+    p->paramB.reserve(r4*std::max(n, 1));
+    p->paramC.reserve(r4*std::max(n, 1));
+    p->lambdaParams.reserve(12*(size_t) std::max(n, 1));
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(1), 0, stream, p->mismatch.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    DeriveArgs a;
+    a.disp = (p->terms & IXH_TERM_DISPERSION) ? p->rawDisp.ptr : nullptr;
+    a.exch = (p->terms & IXH_TERM_EXCHANGE) ? p->rawExch.ptr : nullptr;
+    a.sr = (p->terms & IXH_TERM_SHORT_RANGE) ? p->rawSr.ptr : nullptr;
+    a.n = n;
+    a.precision = p->precision;
+    a.useTables = p->useTables ? 1 : 0;
+    a.shared = p->settings.shared_overlap;
+    a.cls = p->classes.ptr;
+    a.reps = p->reps.ptr;
+    a.repExch = p->repExch;
+    a.repSr = p->repSr;
+    a.A = p->paramA.ptr;
+    a.B = p->paramB.ptr;
+    a.C = p->paramC.ptr;
+    a.lambdaParams = p->lambdaParams.ptr;
+    a.mismatch = p->mismatch.ptr;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_derive_params, dim3(grid_for(n, 256)), dim3(256), 0, stream, a);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
+    IXH_HIP_CHECK(hipMemcpyAsync(&p->staging->mismatch, p->mismatch.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (p->useTables) {
+        p->repParams.reserve(10*(size_t) p->numReps);
+        hipLaunchKernelGGL(k_gather_reps, dim3(1), dim3(64), 0, stream, a.disp, a.exch, a.sr, (const int*) p->reps.ptr, p->numReps,
+                           p->repParams.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        IXH_HIP_CHECK(hipMemcpyAsync(p->staging->reps, p->repParams.ptr, sizeof(double)*10*p->numReps, hipMemcpyDeviceToHost, stream));
+    }
+    if (p->built)
+        gather_sorted(p, stream);
+}
+
+// Long range correction coefficients (InteraxDispersionForceImpl.cpp:86-149) on the device, into the staging area:
+// classes of identical (m1, m2, m3, alpha, width, alchemical) in the order of the tuple, grouped by stable radix
+// sorts; the class-pair integrals of k_long_range; their sum in a fixed order.
+void launch_long_range(ixh_pair* p, hipStream_t stream) {
+    const int n = p->n;
+    if (!(p->terms & IXH_TERM_DISPERSION) || p->method != IXH_CUTOFF_PERIODIC || !p->useLRC || n == 0) {
+        p->lrcCoefficient = p->lrcAlchemical = 0;
+        p->lrcPending = false;
+        return;
+    }
+    for (int b = 0; b < 2; b++) {
+        p->lrcKeys[b].reserve(n);
+        p->lrcPerm[b].reserve(n);
+    }
+    p->lrcFirst.reserve(n);
+    p->lrcIndex.reserve(n);
+    p->lrcStart.reserve(n+1);
+    p->lrcCount.reserve(1);
+    p->lrcClasses.reserve(7*(size_t) n);
+    p->lrcPartial.reserve(2*LRC_BLOCKS);
+    p->lrcValues.reserve(2);
+    size_t sortBytes = 0, scanBytes = 0;
+    IXH_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, sortBytes, p->lrcKeys[0].ptr, p->lrcKeys[1].ptr, p->lrcPerm[0].ptr,
+                                            p->lrcPerm[1].ptr, (size_t) n, 0, 64, stream));
+    IXH_HIP_CHECK(rocprim::inclusive_scan(nullptr, scanBytes, p->lrcFirst.ptr, p->lrcIndex.ptr, (size_t) n, rocprim::plus<int>(),
+                                          stream));
+    size_t tempBytes = std::max(sortBytes, scanBytes);
+    p->lrcTemp.reserve(tempBytes);
+    const int grid = grid_for(n, 256);
+    const int* flags = p->flags.ptr;
+    hipLaunchKernelGGL(k_iota, dim3(grid), dim3(256), 0, stream, p->lrcPerm[0].ptr, n);
+    IXH_HIP_CHECK(hipGetLastError());
+    // least significant field first: alchemical, width, alpha, m3, m2, m1
+    for (int field = 5; field >= 0; field--) {
+        hipLaunchKernelGGL(k_lrc_keys, dim3(grid), dim3(256), 0, stream, p->rawDisp.ptr, flags, n, field, p->lrcPerm[0].ptr,
+                           p->lrcKeys[0].ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+        size_t bytes = tempBytes;
+        IXH_HIP_CHECK(rocprim::radix_sort_pairs(p->lrcTemp.ptr, bytes, p->lrcKeys[0].ptr, p->lrcKeys[1].ptr, p->lrcPerm[0].ptr,
+                                                p->lrcPerm[1].ptr, (size_t) n, 0, 64, stream));
+        std::swap(p->lrcPerm[0].ptr, p->lrcPerm[1].ptr);   // both hold n entries
+        std::swap(p->lrcPerm[0].count, p->lrcPerm[1].count);
+    }
+    hipLaunchKernelGGL(k_lrc_bounds, dim3(grid), dim3(256), 0, stream, p->rawDisp.ptr, flags, p->lrcPerm[0].ptr, n, p->lrcFirst.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    size_t bytes = tempBytes;
+    IXH_HIP_CHECK(rocprim::inclusive_scan(p->lrcTemp.ptr, bytes, p->lrcFirst.ptr, p->lrcIndex.ptr, (size_t) n, rocprim::plus<int>(),
+                                          stream));
+    hipLaunchKernelGGL(k_lrc_starts, dim3(grid), dim3(256), 0, stream, p->lrcFirst.ptr, p->lrcIndex.ptr, n, p->lrcStart.ptr,
+                       p->lrcCount.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_lrc_classes, dim3(grid), dim3(256), 0, stream, p->rawDisp.ptr, flags, p->lrcPerm[0].ptr, p->lrcStart.ptr,
+                       p->lrcCount.ptr, p->lrcClasses.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
     LrcArgs a;
-    a.classes = classesDev.ptr;
-    a.numClasses = (long long) counts.size();
+    a.classes = p->lrcClasses.ptr;
+    a.numClasses = p->lrcCount.ptr;
     a.cutoff = p->cutoff;
     a.useSwitch = p->useSwitch;
     a.switchDistance = (p->useSwitch ? p->switchDistance : p->cutoff);
     a.switchWidth = p->cutoff-a.switchDistance;
     a.dampingScale = p->dampingScale;
     gauss_legendre(32, a.nodes, a.weights);
-    long long pairs = a.numClasses*(a.numClasses+1)/2;
-    int threads = 256;
-    int blocks = (int) std::min<long long>(2048, std::max<long long>(1, (pairs+threads*64-1)/(threads*64)));
-    partial.reserve(2*(size_t) blocks*threads);
-    hipLaunchKernelGGL(k_long_range, dim3(blocks), dim3(threads), 0, stream, a, partial.ptr);
+    hipLaunchKernelGGL(k_long_range, dim3(LRC_BLOCKS), dim3(LRC_THREADS), 0, stream, a, p->lrcPartial.ptr);
     IXH_HIP_CHECK(hipGetLastError());
-    std::vector<double> host(2*(size_t) blocks*threads);
-    IXH_HIP_CHECK(hipMemcpyAsync(host.data(), partial.ptr, sizeof(double)*host.size(), hipMemcpyDeviceToHost, stream));
-    IXH_HIP_CHECK(hipStreamSynchronize(stream));
-    double sum = 0, alch = 0;
-    for (size_t k = 0; k < host.size(); k += 2) {
-        sum += host[k];
-        alch += host[k+1];
+    const double N = n;
+    const double normalization = 2*M_PI*N*N/(0.5*N*(N+1));
+    hipLaunchKernelGGL(k_lrc_reduce, dim3(1), dim3(1024), 0, stream, (const double*) p->lrcPartial.ptr, LRC_BLOCKS, normalization,
+                       p->lrcValues.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    IXH_HIP_CHECK(hipMemcpyAsync(p->staging->lrc, p->lrcValues.ptr, 2*sizeof(double), hipMemcpyDeviceToHost, stream));
+    p->lrcPending = true;
+}
+
+// The device work of a parameter update, once the FP64 parameters are in rawDisp, rawExch, rawSr (stream order).
+void update_parameters(ixh_pair* p, double dampingScale, hipStream_t stream) {
+    p->dampingScale = dampingScale;
+    p->settings.damping_scale = dampingScale;
+    p->settings.damping_scale_f = (float) dampingScale;
+    derive_parameters(p, stream);
+    launch_long_range(p, stream);
+    IXH_HIP_CHECK(hipEventRecord(p->updateDone, stream));
+    p->updateRecorded = true;
+    p->updatePending = true;
+    p->updateNeedsHost = (p->useTables || p->settings.shared_overlap);
+}
+
+// Before an update writes the handle's arrays: the evaluations and the update before it have finished with them.
+void order_update(ixh_pair* p, hipStream_t stream) {
+    for (int f = 0; f < 2; f++)
+        if (p->frameStream[f] != stream && p->frameUsed[f])
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->frameDone[f], 0));
+    if (p->updateRecorded)
+        IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->updateDone, 0));
+}
+
+// The host's part of the last update, before an evaluation on `stream` (which then waits for the update's device
+// work).  With type tables (or one shared overlap) it waits for the update: the class check decides whether the
+// classes still hold (then the table rows are formed from the representatives' parameters) or are formed anew.
+void finish_parameters(ixh_pair* p, hipStream_t stream) {
+    if (!p->updatePending)
+        return;
+    if (p->updateNeedsHost) {
+        IXH_HIP_CHECK(hipEventSynchronize(p->updateDone));
+        if (p->staging->mismatch) {
+            const int n = p->n;
+            std::vector<double> disp(5*(size_t) n), exch(2*(size_t) n), sr(3*(size_t) n);
+            if (p->terms & IXH_TERM_DISPERSION)
+                IXH_HIP_CHECK(hipMemcpy(disp.data(), p->rawDisp.ptr, sizeof(double)*disp.size(), hipMemcpyDeviceToHost));
+            if (p->terms & IXH_TERM_EXCHANGE)
+                IXH_HIP_CHECK(hipMemcpy(exch.data(), p->rawExch.ptr, sizeof(double)*exch.size(), hipMemcpyDeviceToHost));
+            if (p->terms & IXH_TERM_SHORT_RANGE)
+                IXH_HIP_CHECK(hipMemcpy(sr.data(), p->rawSr.ptr, sizeof(double)*sr.size(), hipMemcpyDeviceToHost));
+            classify(p, disp.data(), exch.data(), sr.data());
+            derive_parameters(p, nullptr);
+            IXH_HIP_CHECK(hipStreamSynchronize(nullptr));
+            if (p->staging->mismatch)
+                throw Error(IXH_ERROR_HIP, "ixh_pair: the parameter classes do not hold after they were formed anew");
+        }
+        if (p->useTables) {
+            std::vector<float> tables;
+            form_tables(p, p->staging->reps, tables);
+            p->tables.reserve(tables.size());
+            IXH_HIP_CHECK(hipMemcpy(p->tables.ptr, tables.data(), sizeof(float)*tables.size(), hipMemcpyHostToDevice));
+        }
+        p->updateNeedsHost = false;
     }
-    double N = n;
-    double normalization = 2*M_PI*N*N/(0.5*N*(N+1));
-    p->lrcCoefficient = normalization*sum;
-    p->lrcAlchemical = normalization*alch;
+    IXH_HIP_CHECK(hipStreamWaitEvent(stream, p->updateDone, 0));
+    p->updatePending = false;
+}
+
+// The long range correction of the last update (waits for it).
+void finish_long_range(ixh_pair* p) {
+    if (!p->lrcPending)
+        return;
+    IXH_HIP_CHECK(hipEventSynchronize(p->updateDone));
+    p->lrcCoefficient = p->staging->lrc[0];
+    p->lrcAlchemical = p->staging->lrc[1];
+    p->lrcPending = false;
+}
+
+// Checks of an update (ReferenceInteraxKernels.cpp:98-102, 142-148, 185-191).
+void check_update(const ixh_pair* p, int terms, int exchangeMethod, int shortRangeMethod) {
+    if (terms != p->terms)
+        throw Error(IXH_ERROR_STATE, "updateParametersInContext: The set of pair forces has changed");
+    if (((p->terms & IXH_TERM_EXCHANGE) && exchangeMethod != p->exchangeMethod) ||
+        ((p->terms & IXH_TERM_SHORT_RANGE) && shortRangeMethod != p->shortRangeMethod))
+        throw Error(IXH_ERROR_STATE, "updateParametersInContext: The overlap method has changed");
+}
+
+// Host arrays of a descriptor into the FP64 device arrays (synchronous: the caller may reuse them on return).
+void upload_raw(ixh_pair* p, const ixh_pair_desc* d) {
+    const size_t n = std::max(p->n, 1);
+    p->rawDisp.reserve(5*n);
+    p->rawExch.reserve(2*n);
+    p->rawSr.reserve(3*n);
+    if (p->n == 0)
+        return;
+    if (p->terms & IXH_TERM_DISPERSION)
+        IXH_HIP_CHECK(hipMemcpy(p->rawDisp.ptr, d->dispersion, sizeof(double)*5*(size_t) p->n, hipMemcpyHostToDevice));
+    if (p->terms & IXH_TERM_EXCHANGE)
+        IXH_HIP_CHECK(hipMemcpy(p->rawExch.ptr, d->exchange, sizeof(double)*2*(size_t) p->n, hipMemcpyHostToDevice));
+    if (p->terms & IXH_TERM_SHORT_RANGE)
+        IXH_HIP_CHECK(hipMemcpy(p->rawSr.ptr, d->short_range, sizeof(double)*3*(size_t) p->n, hipMemcpyHostToDevice));
+}
+
+// The parameters of a new handle: schemes, particle flags, alchemical list, classes, then the common update.
+void init_parameters(ixh_pair* p, const ixh_pair_desc* d, hipStream_t stream) {
+    const int n = p->n;
+    const bool doublePrecision = (p->precision == IXH_DOUBLE);
+    p->settings.exchange_scheme = make_scheme(p->exchangeMethod, doublePrecision);
+    p->settings.short_range_scheme = make_scheme(p->shortRangeMethod, doublePrecision);
+    std::vector<int> flags(std::max(n, 1), 0), alch;
+    for (int i = 0; i < n; i++) {
+        flags[i] = i | (p->alchemical[i] ? ALCHEMICAL_BIT : 0);
+        if (p->alchemical[i])
+            alch.push_back(i);
+    }
+    p->flags.upload(flags.data(), flags.size(), stream);
+    p->alchList.reserve(std::max<size_t>(alch.size(), 1));
+    if (!alch.empty())
+        p->alchList.upload(alch.data(), alch.size(), stream);
+    p->numAlchemical = (int) alch.size();
+    IXH_HIP_CHECK(hipStreamSynchronize(stream));
+    upload_raw(p, d);
+    const double* disp = d->dispersion;
+    const double* exch = d->exchange;
+    const double* sr = d->short_range;
+    classify(p, disp, exch, sr);
+    update_parameters(p, (p->terms & IXH_TERM_DISPERSION) ? d->damping_scale : 1.0, stream);
+    finish_parameters(p, stream);
+    finish_long_range(p);
 }
 
 // Build the slot order, the partner lists, the tiles and the candidate masks from the current positions.
@@ -2163,21 +2583,7 @@ void build_list(ixh_pair* p, const ixh_buffers* b, hipStream_t stream) {
     p->sortedB.reserve(r4*p->padded);
     p->sortedC.reserve(r4*p->padded);
     p->sortedFlags.reserve(p->padded);
-    if (p->precision == IXH_DOUBLE)
-        hipLaunchKernelGGL(k_gather_params<double>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
-                           (const double4*) p->paramA.ptr, (const double4*) p->paramB.ptr, (const double4*) p->paramC.ptr, p->flags.ptr,
-                           (double4*) p->sortedA.ptr, (double4*) p->sortedB.ptr, (double4*) p->sortedC.ptr, p->sortedFlags.ptr);
-    else
-        hipLaunchKernelGGL(k_gather_params<float>, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
-                           (const float4*) p->paramA.ptr, (const float4*) p->paramB.ptr, (const float4*) p->paramC.ptr, p->flags.ptr,
-                           (float4*) p->sortedA.ptr, (float4*) p->sortedB.ptr, (float4*) p->sortedC.ptr, p->sortedFlags.ptr);
-    IXH_HIP_CHECK(hipGetLastError());
-    if (p->useTables) {
-        p->sortedClasses.reserve(p->padded);
-        hipLaunchKernelGGL(k_gather_int, dim3(grid_for(p->padded, 256)), dim3(256), 0, stream, p->perm.ptr, p->padded,
-                           (const int*) p->classes.ptr, p->sortedClasses.ptr);
-        IXH_HIP_CHECK(hipGetLastError());
-    }
+    gather_sorted(p, stream);
 
     double listCutoff = p->cutoff+SKIN;
     if (cutoff) {
@@ -2567,6 +2973,10 @@ void execute_pair(ixh_pair* p, const ixh_buffers* b, int terms, int flags, doubl
     }
     DeviceGuard guard(p->device);
     hipStream_t stream = (hipStream_t) b->stream;
+    if (noalloc && p->updatePending && p->updateNeedsHost)
+        throw Error(IXH_ERROR_STATE, "ixh_pair_execute_noalloc: a parameter update is not complete (the next ixh_pair_execute "
+                                     "completes it)");
+    finish_parameters(p, stream);
     const float4* corr = (const float4*) b->posq_correction;
     bool rebuild = !p->built;
     const int periodic = (p->method == IXH_CUTOFF_PERIODIC);
@@ -2697,6 +3107,7 @@ void execute_pair(ixh_pair* p, const ixh_buffers* b, int terms, int flags, doubl
     if (marked == nullptr)
         IXH_HIP_CHECK(hipEventRecord(p->frameDone[p->frame], stream));
     p->frameStream[p->frame] = stream;
+    p->frameUsed[p->frame] = true;
 }
 
 }  // namespace
@@ -2795,13 +3206,19 @@ int ixh_pair_create_segmented(int device, const ixh_pair_desc* d, int num_segmen
             for (hipEvent_t& e : p->frameDone)
                 IXH_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             IXH_HIP_CHECK(hipEventCreateWithFlags(&p->flagCopied, hipEventDisableTiming));
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->updateDone, hipEventDisableTiming));
+            IXH_HIP_CHECK(hipHostMalloc((void**) &p->staging, sizeof(ixh_pair::Staging), hipHostMallocDefault));
             (void) static_scratch(device);
-            load_parameters(p, d, stream);
-            compute_long_range(p, stream);
+            p->mismatch.reserve(1);
+            init_parameters(p, d, stream);
         }
         catch (...) {
             if (p->hostFlag != nullptr)
                 (void) hipHostFree(p->hostFlag);
+            if (p->staging != nullptr)
+                (void) hipHostFree(p->staging);
+            if (p->updateDone != nullptr)
+                (void) hipEventDestroy(p->updateDone);
             delete p;
             throw;
         }
@@ -2821,6 +3238,10 @@ int ixh_pair_destroy(ixh_pair* p) {
             (void) hipEventDestroy(p->subsetCopied);
         if (p->flagCopied != nullptr)
             (void) hipEventDestroy(p->flagCopied);
+        if (p->updateDone != nullptr)
+            (void) hipEventDestroy(p->updateDone);
+        if (p->staging != nullptr)
+            (void) hipHostFree(p->staging);
         for (hipEvent_t e : p->frameDone)
             if (e != nullptr)
                 (void) hipEventDestroy(e);
@@ -2840,17 +3261,39 @@ int ixh_pair_set_parameters(ixh_pair* p, const ixh_pair_desc* d) {
         // ReferenceInteraxKernels.cpp:98-102, 142-148, 185-191
         if (d->num_particles != p->n)
             throw Error(IXH_ERROR_STATE, "updateParametersInContext: The number of particles has changed");
-        if (d->terms != p->terms)
-            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The set of pair forces has changed");
-        if (((p->terms & IXH_TERM_EXCHANGE) && d->exchange_overlap_method != p->exchangeMethod) ||
-            ((p->terms & IXH_TERM_SHORT_RANGE) && d->short_range_overlap_method != p->shortRangeMethod))
-            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The overlap method has changed");
+        check_update(p, d->terms, d->exchange_overlap_method, d->short_range_overlap_method);
         DeviceGuard guard(p->device);
-        if (p->lastStream != nullptr)
-            IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
         hipStream_t stream = nullptr;
-        load_parameters(p, d, stream);
-        compute_long_range(p, stream);
+        order_update(p, stream);
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        upload_raw(p, d);
+        update_parameters(p, d->damping_scale, stream);
+        finish_parameters(p, stream);
+        finish_long_range(p);
+    });
+}
+
+int ixh_pair_set_parameters_device(ixh_pair* p, const ixh_pair_device_parameters* d) {
+    return guarded([&] {
+        if (p == nullptr || d == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_set_parameters_device: null argument");
+        if (((p->terms & IXH_TERM_DISPERSION) && d->dispersion == nullptr) ||
+            ((p->terms & IXH_TERM_EXCHANGE) && d->exchange == nullptr) ||
+            ((p->terms & IXH_TERM_SHORT_RANGE) && d->short_range == nullptr))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_set_parameters_device: missing parameters of a term of the handle");
+        DeviceGuard guard(p->device);
+        hipStream_t stream = (hipStream_t) d->stream;
+        order_update(p, stream);
+        const size_t n = (size_t) p->n;
+        if (n > 0) {
+            if (p->terms & IXH_TERM_DISPERSION)
+                IXH_HIP_CHECK(hipMemcpyAsync(p->rawDisp.ptr, d->dispersion, sizeof(double)*5*n, hipMemcpyDeviceToDevice, stream));
+            if (p->terms & IXH_TERM_EXCHANGE)
+                IXH_HIP_CHECK(hipMemcpyAsync(p->rawExch.ptr, d->exchange, sizeof(double)*2*n, hipMemcpyDeviceToDevice, stream));
+            if (p->terms & IXH_TERM_SHORT_RANGE)
+                IXH_HIP_CHECK(hipMemcpyAsync(p->rawSr.ptr, d->short_range, sizeof(double)*3*n, hipMemcpyDeviceToDevice, stream));
+        }
+        update_parameters(p, (p->terms & IXH_TERM_DISPERSION) ? d->damping_scale : p->dampingScale, stream);
     });
 }
 
@@ -2884,6 +3327,10 @@ int ixh_pair_long_range_coefficients(ixh_pair* p, double* coefficient, double* a
     return guarded([&] {
         if (p == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_long_range_coefficients: null handle");
+        {
+            DeviceGuard guard(p->device);
+            finish_long_range(p);
+        }
         if (coefficient != nullptr)
             *coefficient = p->lrcCoefficient;
         if (alchemical != nullptr)
@@ -2895,6 +3342,10 @@ int ixh_pair_long_range_energy(ixh_pair* p, const double* box, double lambda, do
     return guarded([&] {
         if (p == nullptr || energy == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_long_range_energy: null argument");
+        {
+            DeviceGuard guard(p->device);
+            finish_long_range(p);
+        }
         double e = 0;
         if (p->lrcCoefficient != 0 || p->lrcAlchemical != 0) {
             if (box == nullptr)
@@ -2936,33 +3387,6 @@ int ixh_pair_lambda_energies(ixh_pair* p, const ixh_buffers* b, int terms, const
         // 2. the pairs of the alchemical particles at every lambda and at the buffers' own
         const int values = count+1;
         if (p->anyAlchemical) {
-            if (p->lambdaParamVersion != p->paramVersion) {
-                std::vector<double> q(12*(size_t) p->n, 0.0);
-                std::vector<int> alch;
-                for (int i = 0; i < p->n; i++) {
-                    double* r = &q[12*(size_t) i];
-                    if (p->terms & IXH_TERM_DISPERSION)
-                        for (int k = 0; k < 5; k++)
-                            r[k] = p->dispersion[5*(size_t) i+k];
-                    if (p->terms & IXH_TERM_EXCHANGE) {
-                        r[5] = p->exchange[2*(size_t) i];
-                        r[6] = p->exchange[2*(size_t) i+1];
-                        r[7] = (p->exchange[2*(size_t) i+1] != 0 ? 1.0/p->exchange[2*(size_t) i+1] : 0.0);
-                    }
-                    if (p->terms & IXH_TERM_SHORT_RANGE) {
-                        r[8] = p->shortRange[3*(size_t) i];
-                        r[9] = p->shortRange[3*(size_t) i+1];
-                        r[10] = p->shortRange[3*(size_t) i+2];
-                        r[11] = (p->shortRange[3*(size_t) i+2] != 0 ? 1.0/p->shortRange[3*(size_t) i+2] : 0.0);
-                    }
-                    if (p->alchemical[i])
-                        alch.push_back(i);
-                }
-                p->lambdaParams.upload(q.data(), q.size(), stream);
-                p->alchList.upload(alch.data(), std::max<size_t>(alch.size(), 1), stream);
-                p->numAlchemical = (int) alch.size();
-                p->lambdaParamVersion = p->paramVersion;
-            }
             LambdaArgs a;
             a.posq = b->posq;
             a.correction = (const float4*) b->posq_correction;

@@ -1086,6 +1086,12 @@ struct ixh_threebody {
     DeviceArray<long long> siteForce, tripleCount;
     hipStream_t lastStream = nullptr;
     hipEvent_t done = nullptr;   // recorded at the end of every execute: calls on other streams wait for it
+    // parameter updates: FP64 alpha, frequency, width of the host path, the update's end, and its width check
+    DeviceArray<double> rawSites;
+    DeviceArray<int> invalid;
+    int* hostInvalid = nullptr;   // pinned
+    hipEvent_t updateDone = nullptr;
+    bool updateRecorded = false, updatePending = false;
     KernelTimer timer;
 };
 
@@ -1115,6 +1121,61 @@ void check_threebody(const ixh_threebody_desc* d) {
     }
     if (d->cutoff <= 0)
         throw Error(IXH_ERROR_INVALID_ARGUMENT, "InteraxThreeBodyForce: The cutoff distance must be positive");
+}
+
+// alpha, frequency and width of every site into the kernels' parameter array (the alchemical flag is kept); a width
+// the Tang-Toennies damping cannot take sets *invalid.
+__global__ void k_site_params(const double* alpha, const double* frequency, const double* width, int n, int checkWidth,
+                              double* params, int* invalid) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n)
+        return;
+    params[4*(size_t) i] = alpha[i];
+    params[4*(size_t) i+1] = frequency[i];
+    params[4*(size_t) i+2] = width[i];
+    if (checkWidth && !(width[i] > 0))
+        *invalid = 1;
+}
+
+__global__ void k_clear_flag(int* p) {
+    *p = 0;
+}
+
+// The device work of a parameter update from FP64 arrays in device memory, on `stream`, after the evaluations and
+// the update before it.
+void update_site_params(ixh_threebody* h, const double* alpha, const double* frequency, const double* width, hipStream_t stream) {
+    if (h->done != nullptr)
+        IXH_HIP_CHECK(hipStreamWaitEvent(stream, h->done, 0));
+    if (h->updateRecorded)
+        IXH_HIP_CHECK(hipStreamWaitEvent(stream, h->updateDone, 0));
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, stream, h->invalid.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    if (h->numSites > 0) {
+        hipLaunchKernelGGL(k_site_params, dim3((h->numSites+255)/256), dim3(256), 0, stream, alpha, frequency, width, h->numSites,
+                           (int) (h->dampingMethod == IXH_DAMPING_TANG_TOENNIES), h->paramsDev.ptr, h->invalid.ptr);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
+    IXH_HIP_CHECK(hipMemcpyAsync(h->hostInvalid, h->invalid.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+    IXH_HIP_CHECK(hipEventRecord(h->updateDone, stream));
+    h->updateRecorded = true;
+    h->updatePending = true;
+}
+
+// Before an evaluation on `stream`: it is ordered after the last update, whose widths the Tang-Toennies damping has
+// checked on the device (the host waits for that check).
+void finish_site_params(ixh_threebody* h, hipStream_t stream) {
+    if (!h->updatePending)
+        return;
+    if (h->dampingMethod == IXH_DAMPING_TANG_TOENNIES) {
+        IXH_HIP_CHECK(hipEventSynchronize(h->updateDone));
+        if (*h->hostInvalid) {
+            h->updatePending = false;
+            throw Error(IXH_ERROR_INVALID_ARGUMENT,
+                        "InteraxThreeBodyForce: The Tang-Toennies damping requires a positive width for every site");
+        }
+    }
+    IXH_HIP_CHECK(hipStreamWaitEvent(stream, h->updateDone, 0));
+    h->updatePending = false;
 }
 
 void load_threebody_params(ixh_threebody* h, const ixh_threebody_desc* d) {
@@ -1399,9 +1460,17 @@ int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* d, int 
                 h->nbrIndex.reserve((size_t) std::max<long long>(total, 1));
                 h->nbrDelta.reserve((size_t) std::max<long long>(total, 1));
             }
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&h->updateDone, hipEventDisableTiming));
+            IXH_HIP_CHECK(hipHostMalloc((void**) &h->hostInvalid, sizeof(int), hipHostMallocDefault));
+            h->invalid.reserve(1);
+            h->rawSites.reserve(3*(size_t) std::max(h->numSites, 1));
             load_threebody_params(h, d);
         }
         catch (...) {
+            if (h->updateDone != nullptr)
+                (void) hipEventDestroy(h->updateDone);
+            if (h->hostInvalid != nullptr)
+                (void) hipHostFree(h->hostInvalid);
             delete h;
             throw;
         }
@@ -1429,6 +1498,10 @@ int ixh_threebody_destroy(ixh_threebody* h) {
             (void) hipEventDestroy(h->done);
         if (h->checked != nullptr)
             (void) hipEventDestroy(h->checked);
+        if (h->updateDone != nullptr)
+            (void) hipEventDestroy(h->updateDone);
+        if (h->hostInvalid != nullptr)
+            (void) hipHostFree(h->hostInvalid);
         delete h;
     });
 }
@@ -1443,9 +1516,39 @@ int ixh_threebody_set_parameters(ixh_threebody* h, const ixh_threebody_desc* d) 
         if ((d->centroid_damping_radius > 0) != (h->centroidDampingRadius > 0))
             throw Error(IXH_ERROR_STATE, "updateParametersInContext: The centroid damping cannot be switched on or off");
         DeviceGuard guard(h->device);
-        if (h->lastStream != nullptr)
-            IXH_HIP_CHECK(hipStreamSynchronize(h->lastStream));
-        load_threebody_params(h, d);
+        // the same device work as ixh_threebody_set_parameters_device, from a copy of the host arrays
+        const size_t n = (size_t) h->numSites;
+        hipStream_t stream = nullptr;
+        if (h->done != nullptr)
+            IXH_HIP_CHECK(hipStreamWaitEvent(stream, h->done, 0));
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        if (n > 0) {
+            IXH_HIP_CHECK(hipMemcpy(h->rawSites.ptr, d->alpha, sizeof(double)*n, hipMemcpyHostToDevice));
+            IXH_HIP_CHECK(hipMemcpy(h->rawSites.ptr+n, d->frequency, sizeof(double)*n, hipMemcpyHostToDevice));
+            IXH_HIP_CHECK(hipMemcpy(h->rawSites.ptr+2*n, d->width, sizeof(double)*n, hipMemcpyHostToDevice));
+        }
+        h->scaleFactor = d->scale_factor;
+        h->dampingScale = d->damping_scale;
+        h->centroidDampingRadius = d->centroid_damping_radius;
+        update_site_params(h, h->rawSites.ptr, h->rawSites.ptr+n, h->rawSites.ptr+2*n, stream);
+        finish_site_params(h, stream);
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+    });
+}
+
+int ixh_threebody_set_parameters_device(ixh_threebody* h, const ixh_threebody_device_parameters* d) {
+    return guarded([&] {
+        if (h == nullptr || d == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_set_parameters_device: null argument");
+        if (h->numSites > 0 && (d->alpha == nullptr || d->frequency == nullptr || d->width == nullptr))
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_set_parameters_device: missing per-site arrays");
+        if ((d->centroid_damping_radius > 0) != (h->centroidDampingRadius > 0))
+            throw Error(IXH_ERROR_STATE, "updateParametersInContext: The centroid damping cannot be switched on or off");
+        DeviceGuard guard(h->device);
+        h->scaleFactor = d->scale_factor;
+        h->dampingScale = d->damping_scale;
+        h->centroidDampingRadius = d->centroid_damping_radius;
+        update_site_params(h, d->alpha, d->frequency, d->width, (hipStream_t) d->stream);
     });
 }
 
@@ -1471,6 +1574,7 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
             return;
         DeviceGuard guard(h->device);
         hipStream_t stream = (hipStream_t) b->stream;
+        finish_site_params(h, stream);
         if (h->done == nullptr)
             IXH_HIP_CHECK(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
         else if (stream != h->lastStream)

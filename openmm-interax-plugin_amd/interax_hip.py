@@ -73,6 +73,17 @@ class ThreeBodyDesc(ctypes.Structure):
                 ("precision", ctypes.c_int)]
 
 
+class PairDeviceParameters(ctypes.Structure):
+    _fields_ = [("dispersion", ctypes.c_void_p), ("exchange", ctypes.c_void_p), ("short_range", ctypes.c_void_p),
+                ("damping_scale", ctypes.c_double), ("stream", ctypes.c_void_p)]
+
+
+class ThreeBodyDeviceParameters(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_void_p), ("frequency", ctypes.c_void_p), ("width", ctypes.c_void_p),
+                ("scale_factor", ctypes.c_double), ("damping_scale", ctypes.c_double),
+                ("centroid_damping_radius", ctypes.c_double), ("stream", ctypes.c_void_p)]
+
+
 class BatchDesc(ctypes.Structure):
     _fields_ = [("num_systems", ctypes.c_int), ("particle_offsets", _ip), ("pair", ctypes.POINTER(PairDesc)),
                 ("site_offsets", _ip), ("threebody", ctypes.POINTER(ThreeBodyDesc)), ("precision", ctypes.c_int)]
@@ -97,6 +108,7 @@ SIGNATURES = {
     "ixh_pair_create": [ctypes.c_int, ctypes.POINTER(PairDesc), ctypes.POINTER(ctypes.c_void_p)],
     "ixh_pair_destroy": [ctypes.c_void_p],
     "ixh_pair_set_parameters": [ctypes.c_void_p, ctypes.POINTER(PairDesc)],
+    "ixh_pair_set_parameters_device": [ctypes.c_void_p, ctypes.POINTER(PairDeviceParameters)],
     "ixh_pair_execute": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int],
     "ixh_pair_execute_noalloc": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int],
     "ixh_pair_long_range_coefficients": [ctypes.c_void_p, _dp, _dp],
@@ -109,6 +121,7 @@ SIGNATURES = {
     "ixh_threebody_create": [ctypes.c_int, ctypes.POINTER(ThreeBodyDesc), ctypes.POINTER(ctypes.c_void_p)],
     "ixh_threebody_destroy": [ctypes.c_void_p],
     "ixh_threebody_set_parameters": [ctypes.c_void_p, ctypes.POINTER(ThreeBodyDesc)],
+    "ixh_threebody_set_parameters_device": [ctypes.c_void_p, ctypes.POINTER(ThreeBodyDeviceParameters)],
     "ixh_threebody_execute": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int],
     "ixh_threebody_list_was_valid": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)],
     "ixh_threebody_last_triple_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
