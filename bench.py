@@ -456,8 +456,18 @@ def bench_batch(args, sample_every=16):
     t0 = time.time()
     batch = ih.Batch(systems, "mixed")
     batch.setPositions(positions)
+    default_shape = batch.shape()
+    for _ in range(max(1, args.warmup)):
+        batch.execute()
+    batch.energies()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        batch.execute()
+    batch.energies()
+    default_s = (time.perf_counter()-start)/args.steps
     chosen = batch.autotune()
-    log(f"[bench] batch created and tuned in {time.time()-t0:.1f} s (pair tile launch: {chosen} workgroups)")
+    log(f"[bench] batch created and tuned in {time.time()-t0:.1f} s: {default_s*1e3:.3f} ms per batch with the default "
+        f"shape {default_shape}, chosen {chosen}")
     for _ in range(args.warmup):
         batch.execute()
     batch.energies()
@@ -491,7 +501,10 @@ def bench_batch(args, sample_every=16):
         "config": {"workload": "1024 independent water-like clusters, molecule counts log-uniform in [2, 256], no cutoff, "
                                "dispersion+exchange+short range plus ATM on the oxygens (cutoff 50 nm)",
                    "clusters": len(systems), "atoms": atoms, "pairs_per_batch": pairs, "triples_per_batch": triples,
-                   "pair_tile_workgroups": chosen, "graph_replay": True},
+                   "launch_shape": chosen, "graph_replay": True},
+        "autotune": {"chosen": chosen, "default": default_shape, "ms_per_step_default_shape": default_s*1e3,
+                     "searched": "pair tile launch 256-16384 workgroups, then triple kernel 1/2/4/8 wavefronts x "
+                                 "8192-65536 neighbour pairs per work item; every shape gives the same results bit for bit"},
         "plain": {"value": plain_rate, "unit": "cluster evals/s", "ms_per_1024": plain_s*1e3,
                   "form": "1024 sequential evaluations, one pair and one three-body handle per cluster"},
         "speedup_vs_plain": batch_rate/plain_rate,

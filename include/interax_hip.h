@@ -354,7 +354,7 @@ int ixh_threebody_reset_kernel_time(ixh_threebody* threebody);
  * three-body sites [site_offsets[s], site_offsets[s+1]); `pair` describes the concatenated pair forces (NoCutoff,
  * global particle indices) and `threebody` the concatenated nonperiodic three-body force; either may be NULL.  The
  * batch owns its device state.  ixh_batch_execute is asynchronous; from the second call on it replays a HIP graph of
- * the whole evaluation.  ixh_batch_autotune times the launch shapes of the pair tiles and keeps the fastest.
+ * the whole evaluation.
  */
 typedef struct ixh_batch ixh_batch;
 typedef struct {
@@ -373,7 +373,24 @@ int ixh_batch_execute(ixh_batch* batch, int flags);
 int ixh_batch_get_energies(ixh_batch* batch, double* energies);          /* waits; one per system, kJ/mol */
 int ixh_batch_get_forces(ixh_batch* batch, double* forces);              /* waits; 3 x total particles */
 int ixh_batch_set_graph(ixh_batch* batch, int enable);                   /* 0: launch the kernels eagerly */
-int ixh_batch_autotune(ixh_batch* batch, int flags, int* chosen_workgroups);
+/*
+ * Launch shape of a batch's evaluation: the workgroups of the persistent pair-tile launch, and the triple kernel's
+ * wavefronts per work item (1, 2, 4 or 8) and neighbour pairs per work item (a multiple of 64 in [1024, 2^22]; a
+ * work item is a site and consecutive shares of its neighbour pairs, staged once in LDS for its wavefronts).  Every
+ * shape gives the same per-system energies and forces bit for bit: the pair tiles and the shares (fixed by a site's
+ * list capacity, one wavefront each) add their forces in fixed point and their energies per tile or share, summed per
+ * system in a fixed order.  ixh_batch_autotune times the pair launch's workgroup counts, then the
+ * triple kernel's shapes with the chosen pair launch, on the batch's current positions, and keeps the fastest
+ * (the set and the choice are in chosen, which may be NULL).  Setting a shape waits for the device.
+ */
+typedef struct {
+    int pair_workgroups;
+    int triple_waves;
+    int triple_pairs_per_item;
+} ixh_batch_shape;
+int ixh_batch_get_shape(ixh_batch* batch, ixh_batch_shape* shape);
+int ixh_batch_set_shape(ixh_batch* batch, const ixh_batch_shape* shape);
+int ixh_batch_autotune(ixh_batch* batch, int flags, ixh_batch_shape* chosen);
 /* With profiling enabled (ixh_set_profiling), the summed durations of the batch's pair-tile and triple kernels since
    the last reset (eager evaluations only: profiling evaluates eagerly), and their launch counts. */
 int ixh_batch_kernel_time(ixh_batch* batch, double* pair_ms, long long* pair_launches, double* triple_ms, long long* triple_launches);

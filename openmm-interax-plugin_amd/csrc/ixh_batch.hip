@@ -5,8 +5,9 @@
 // handle (neighbour lists of static capacity), sharing one device state.  One evaluation is then a fixed set of
 // kernels with no host synchronization, the pair half and the three-body half on two streams (forked and joined by
 // events), captured once into a HIP graph and replayed; per-system energies are summed in a fixed order, so a
-// system's energy and forces are the same alone and in any batch.  A tuner times the
-// launch shapes of the pair tiles (workgroups of the persistent tile launch) and keeps the fastest.
+// system's energy and forces are the same alone and in any batch.  A tuner times the launch shapes of the pair tiles
+// (workgroups of the persistent tile launch) and of the triple kernel (wavefronts and neighbour pairs per work item)
+// and keeps the fastest.
 #include "ixh_common.h"
 #include "ixh_internal.h"
 
@@ -335,50 +336,115 @@ int ixh_batch_reset_kernel_time(ixh_batch* b) {
     });
 }
 
-int ixh_batch_autotune(ixh_batch* b, int flags, int* chosen) {
+int ixh_batch_get_shape(ixh_batch* b, ixh_batch_shape* shape) {
+    return guarded([&] {
+        if (b == nullptr || shape == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_batch_get_shape: null argument");
+        shape->pair_workgroups = b->gridLimit;
+        shape->triple_waves = 0;
+        shape->triple_pairs_per_item = 0;
+        if (b->threebody != nullptr)
+            threebody_launch_shape(b->threebody, &shape->triple_waves, &shape->triple_pairs_per_item);
+    });
+}
+
+int ixh_batch_set_shape(ixh_batch* b, const ixh_batch_shape* shape) {
+    return guarded([&] {
+        if (b == nullptr || shape == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_batch_set_shape: null argument");
+        if (shape->pair_workgroups < 1)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_batch_set_shape: the pair launch needs at least one workgroup");
+        DeviceGuard guard(b->device);
+        IXH_HIP_CHECK(hipStreamSynchronize((hipStream_t) b->buffers.stream));
+        b->gridLimit = shape->pair_workgroups;
+        if (b->pair != nullptr)
+            pair_set_grid_limit(b->pair, b->gridLimit);
+        if (b->threebody != nullptr)
+            threebody_set_launch_shape(b->threebody, shape->triple_waves, shape->triple_pairs_per_item);
+        destroy_graph(b);
+    });
+}
+
+int ixh_batch_autotune(ixh_batch* b, int flags, ixh_batch_shape* chosen) {
     return guarded([&] {
         if (b == nullptr)
             throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_batch_autotune: null argument");
         DeviceGuard guard(b->device);
         hipStream_t stream = (hipStream_t) b->buffers.stream;
-        if (b->pair != nullptr) {
-            if (b->evaluated == 0)
-                run(b, flags);   // builds the tile lists
-            const int tiles = pair_num_tiles(b->pair);
-            const int candidates[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
-            hipEvent_t start, stop;
-            IXH_HIP_CHECK(hipEventCreate(&start));
-            IXH_HIP_CHECK(hipEventCreate(&stop));
-            float best = 1e30f;
-            int bestLimit = b->gridLimit;
-            for (int limit : candidates) {
-                if (limit > 2*std::max(1, (tiles+3)/4) && limit != candidates[0])
-                    break;   // more workgroups than tiles: the same launch as the previous candidate
-                pair_set_grid_limit(b->pair, limit);
-                destroy_graph(b);
-                enqueue(b, flags);   // eager: sizes the buffers of this shape
-                run(b, flags);       // captures the graph of this shape
-                IXH_HIP_CHECK(hipStreamSynchronize(stream));
-                IXH_HIP_CHECK(hipEventRecord(start, stream));
-                for (int rep = 0; rep < 5; rep++)
-                    run(b, flags);
-                IXH_HIP_CHECK(hipEventRecord(stop, stream));
-                IXH_HIP_CHECK(hipEventSynchronize(stop));
-                float ms = 0;
-                IXH_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
-                if (ms < best) {
-                    best = ms;
-                    bestLimit = limit;
+        if (b->evaluated == 0)
+            run(b, flags);   // builds the tile lists
+        hipEvent_t start, stop;
+        IXH_HIP_CHECK(hipEventCreate(&start));
+        IXH_HIP_CHECK(hipEventCreate(&stop));
+        // milliseconds per evaluation of the current shape: one eager evaluation sizes its buffers, the next captures
+        // its graph, five replays are timed
+        auto time_shape = [&]() {
+            destroy_graph(b);
+            enqueue(b, flags);
+            run(b, flags);
+            IXH_HIP_CHECK(hipStreamSynchronize(stream));
+            IXH_HIP_CHECK(hipEventRecord(start, stream));
+            for (int rep = 0; rep < 5; rep++)
+                run(b, flags);
+            IXH_HIP_CHECK(hipEventRecord(stop, stream));
+            IXH_HIP_CHECK(hipEventSynchronize(stop));
+            float ms = 0;
+            IXH_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
+            return ms/5;
+        };
+        try {
+            if (b->pair != nullptr) {
+                const int tiles = pair_num_tiles(b->pair);
+                const int candidates[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
+                float best = 1e30f;
+                int bestLimit = b->gridLimit;
+                for (int limit : candidates) {
+                    if (limit > 2*std::max(1, (tiles+3)/4) && limit != candidates[0])
+                        break;   // more workgroups than tiles: the same launch as the previous candidate
+                    pair_set_grid_limit(b->pair, limit);
+                    const float ms = time_shape();
+                    if (ms < best) {
+                        best = ms;
+                        bestLimit = limit;
+                    }
                 }
+                b->gridLimit = bestLimit;
+                pair_set_grid_limit(b->pair, bestLimit);
             }
+            if (b->threebody != nullptr) {
+                int bestWaves = 4, bestChunk = 16384;
+                threebody_launch_shape(b->threebody, &bestWaves, &bestChunk);
+                float best = 1e30f;
+                for (int waves : {4, 8, 2, 1})
+                    for (int chunk : {8192, 16384, 32768, 65536}) {
+                        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+                        threebody_set_launch_shape(b->threebody, waves, chunk);
+                        const float ms = time_shape();
+                        if (ms < best) {
+                            best = ms;
+                            bestWaves = waves;
+                            bestChunk = chunk;
+                        }
+                    }
+                IXH_HIP_CHECK(hipStreamSynchronize(stream));
+                threebody_set_launch_shape(b->threebody, bestWaves, bestChunk);
+            }
+        }
+        catch (...) {
             (void) hipEventDestroy(start);
             (void) hipEventDestroy(stop);
-            b->gridLimit = bestLimit;
-            pair_set_grid_limit(b->pair, bestLimit);
-            destroy_graph(b);
+            throw;
         }
-        if (chosen != nullptr)
-            *chosen = b->gridLimit;
+        (void) hipEventDestroy(start);
+        (void) hipEventDestroy(stop);
+        destroy_graph(b);
+        if (chosen != nullptr) {
+            chosen->pair_workgroups = b->gridLimit;
+            chosen->triple_waves = 0;
+            chosen->triple_pairs_per_item = 0;
+            if (b->threebody != nullptr)
+                threebody_launch_shape(b->threebody, &chosen->triple_waves, &chosen->triple_pairs_per_item);
+        }
     });
 }
 

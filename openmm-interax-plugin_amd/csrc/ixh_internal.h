@@ -9,6 +9,10 @@ namespace ixh {
 // Largest number of workgroups of a pair tile launch (each wave then takes a contiguous range of tiles).
 void pair_set_grid_limit(ixh_pair* pair, int workgroups);
 int pair_num_tiles(const ixh_pair* pair);
+// Launch shape of a segmented three-body handle's triple kernel: wavefronts per work item (1, 2, 4, 8) and neighbour
+// pairs per work item (a multiple of 64).  Waits for the device when the work items change.
+void threebody_set_launch_shape(ixh_threebody* threebody, int waves, int chunkPairs);
+void threebody_launch_shape(const ixh_threebody* threebody, int* waves, int* chunkPairs);
 
 }  // namespace ixh
 

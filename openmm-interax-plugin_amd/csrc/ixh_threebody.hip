@@ -17,6 +17,7 @@
 //      point (integer atomics, so their order does not matter), the force on A stays in registers;
 //   4. the site forces are shared equally among the particles of each site.
 #include "ixh_common.h"
+#include "ixh_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -422,9 +423,10 @@ struct TripleArgs {
     const int* start;
     const int* count;            // segmented handles: neighbours of site i = [start[i], start[i]+count[i])
     const int* itemSite;         // work items: the site of item w (nullptr: item w is site w, all of its pairs)
-    const int* itemChunk;        // work items: the chunk of chunkPairs neighbour pairs of item w
+    const int* itemFirst;        // work items: the first neighbour pair of item w
+    const int2* itemShares;      // work items: the number of shares of item w and their size (pairs)
+    const int* itemShareBase;    // work items: the index of item w's first share among all shares
     const int* itemIndex;        // subsets: the item of every workgroup (nullptr: item = workgroup index)
-    int chunkPairs;
     const int* nbrIndex;
     const double4* nbrDelta;
     int numSites;
@@ -446,7 +448,7 @@ struct TripleArgs {
     long long* contribution;     // skin lists: the force on the site of list entry e from the triples of the list's
                                  // owner, component k at [3 e + k] (summed by k_gather_site_forces)
     long long* siteForce;        // 3 x numSites fixed point
-    double* itemEnergy;          // energy and triple count of every work item
+    double* itemEnergy;          // energy and triple count of every share (plain handles: of every site)
     int* itemCount;
 };
 
@@ -546,10 +548,22 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     Real fA[3] = {0, 0, 0};
     double energy = 0;
     int count = 0;
-    long long pBegin = 0, pEnd = (long long) m*(m-1)/2;
-    if (t.itemChunk != nullptr) {
-        pBegin = (long long) t.itemChunk[item]*t.chunkPairs;
-        pEnd = min(pEnd, pBegin+t.chunkPairs);
+    // Work items of segmented handles: consecutive shares of the site's neighbour pairs, each share taken by one
+    // wavefront, which sums its forces and energy and adds them in fixed point (forces) or stores them (energy, summed
+    // per system in share order later).  The shares depend on the site's list capacity only, so a site's energy and
+    // forces are the same for any number of wavefronts and any number of shares per item.  Plain handles: one share,
+    // all of the site's pairs.
+    // pair indices fit 32 bits: the LDS holds at most a few thousand neighbours
+    int pBegin = 0, pEnd = m*(m-1)/2;
+    int shareSize = pEnd;
+    int numShares = 1;
+    const bool shared = (t.itemShares != nullptr);
+    if (shared) {
+        const int2 sh = t.itemShares[item];
+        pBegin = t.itemFirst[item];
+        numShares = sh.x;
+        shareSize = sh.y;
+        pEnd = min(pEnd, pBegin+numShares*shareSize);
     }
     // Pair p of the site's neighbour pairs.  The first T = M(M-1)/2 (M = min(m, 64)), which are all the pairs within
     // the first 64 neighbours, are taken in rotation order: rotation s holds the pairs (x, (x+s+1) mod M), x < M (the
@@ -558,14 +572,11 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     // do not depend on the order of its two neighbours.  The rest, rows b >= 64, in row order p = b(b-1)/2 + a,
     // where a step's lanes share one b or two consecutive ones.
     const int M = min(m, 64);
-    const long long T = (long long) M*(M-1)/2;
-    // Each wavefront takes a contiguous share of the item's pairs, 64 per step, so that in the rows its steps follow
-    // each other along a row: the pair 64 further on is in the same row or the next (rows have b >= 64 pairs), and
-    // the forces on the row's b are summed per lane over the steps of the row and over the wavefront only when the
-    // wavefront leaves it.
-    const long long share = (((pEnd-pBegin)+WAVES-1)/WAVES+63) & ~63LL;
-    const long long myBegin = pBegin+share*wave, myEnd = min(pEnd, myBegin+share);
-    int a = 0, b = 0;
+    const int T = M*(M-1)/2;
+    // A wavefront takes a share's pairs 64 per step, so that in the rows its steps follow each other along a row: the
+    // pair 64 further on is in the same row or the next (rows have b >= 64 pairs), and the forces on the row's b are
+    // summed per lane over the steps of the row and over the wavefront only when the wavefront leaves it.  Share sizes
+    // are multiples of 64, so the steps are the same whichever wavefront takes a share.
     int R = -1;                 // the row whose forces accR holds (accR1: row R + 1); wavefront-uniform
     bool usedR1 = false;        // some lane has added to accR1
     Real accR[3] = {0, 0, 0}, accR1[3] = {0, 0, 0};
@@ -577,193 +588,218 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
             acc[k] = 0;
         }
     };
-    for (long long p0 = myBegin; p0 < myEnd; p0 += 64) {
-        const long long p = p0+lane;
-        if (p0 != myBegin && p0-64 >= T) {
-            // the previous step was all rows (b >= 64): advance by 64 pairs, at most into the next row
-            a += 64;
-            if (a >= b) {
-                a -= b;
-                b++;
-            }
-        }
-        else if (p < T) {
-            const int q = (int) p;
-            int r = (int) ((float) q*(1.0f/(float) M));
-            while (r > 0 && r*M > q)
-                r--;
-            while ((r+1)*M <= q)
-                r++;
-            a = q-r*M;
-            b = a+r+1;
-            if (b >= M)
-                b -= M;
-        }
-        else {
-            b = (int) ((1.0f+__builtin_amdgcn_sqrtf(1.0f+8.0f*(float) p))*0.5f);
-            while ((long long) b*(b-1)/2 > p)
-                b--;
-            while ((long long) (b+1)*b/2 <= p)
-                b++;
-            a = (int) (p-(long long) b*(b-1)/2);
-        }
-        const bool active = (p < myEnd);
-        // Rows of 64 pairs or more: a step's active lanes (lane 0 always is one) share b or b and b + 1, whose forces
-        // go to accR and accR1 instead of up to 64 atomics on one LDS address; the step that straddles the two
-        // orders, and the rotation order, add per lane.
-        const int bFirst = __builtin_amdgcn_readfirstlane(b);
-        const bool rows = (p0 >= T) && __all(!active || b == bFirst || b == bFirst+1);
-        if (rows && bFirst != R) {
-            if (R >= 0) {
-                flushRow(R, accR);
-                if (bFirst == R+1) {
-                    for (int k = 0; k < 3; k++) {
-                        accR[k] = accR1[k];
-                        accR1[k] = 0;
-                    }
+    for (int v = wave; v < numShares; v += WAVES) {
+        const int myBegin = pBegin+shareSize*v, myEnd = min(pEnd, myBegin+shareSize);
+        int a = 0, b = 0;
+        for (int p0 = myBegin; p0 < myEnd; p0 += 64) {
+            const int p = p0+lane;
+            if (p0 != myBegin && p0-64 >= T) {
+                // the previous step was all rows (b >= 64): advance by 64 pairs, at most into the next row
+                a += 64;
+                if (a >= b) {
+                    a -= b;
+                    b++;
                 }
-                else if (usedR1)
-                    flushRow(R+1, accR1);
             }
-            R = bFirst;
-            usedR1 = false;
-        }
-        if (rows)
-            usedR1 = usedR1 || __any(active && b != R);
-        if (active) {
-            bool ok = (sMol[a] != sMol[b]);
-            R4 va = sDelta[a], vb = sDelta[b];
-            Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
-            if (ok) {
-                bool exact = false;
-                if (t.periodic) {
-                    // the reference's minimum image of BC; near a change of image the choice is made in FP64
-                    Real u2 = c[2]*t.boxR[8]+Real(0.5);
-                    Real s2 = floor(u2);
-                    c[0] -= s2*t.boxR[3];
-                    c[1] -= s2*t.boxR[4];
-                    c[2] -= s2*t.boxR[5];
-                    Real u1 = c[1]*t.boxR[7]+Real(0.5);
-                    Real s1 = floor(u1);
-                    c[0] -= s1*t.boxR[1];
-                    c[1] -= s1*t.boxR[2];
-                    Real u0 = c[0]*t.boxR[6]+Real(0.5);
-                    Real s0 = floor(u0);
-                    c[0] -= s0*t.boxR[0];
-                    Real f2 = u2-s2, f1 = u1-s1, f0 = u0-s0;
-                    exact = (f2 < t.band || f2 > 1-t.band || f1 < t.band || f1 > 1-t.band || f0 < t.band || f0 > 1-t.band);
-                }
-                Real r2 = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
-                if (exact || fabs(r2-t.cutoff2r) < t.band*t.cutoff2r) {
-                    double d[3];
-                    ref_delta(t.sites+3*sIndex[a], t.sites+3*sIndex[b], t.periodic, t.box, d);
-                    ok = ref_norm2(d) < t.cutoff2;
-                    c[0] = Real(d[0]);
-                    c[1] = Real(d[1]);
-                    c[2] = Real(d[2]);
-                }
-                else
-                    ok = r2 < t.cutoff2r;
+            else if (p < T) {
+                const int q = (int) p;
+                int r = (int) ((float) q*(1.0f/(float) M));
+                while (r > 0 && r*M > q)
+                    r--;
+                while ((r+1)*M <= q)
+                    r++;
+                a = q-r*M;
+                b = a+r+1;
+                if (b >= M)
+                    b -= M;
             }
-            R4 pa = sParam[a], pb = sParam[b];
-            Real c9 = 0;
-            if (ok) {
-                // ReferenceInteraxThreeBodyIxn.cpp:116-120 and InteraxThreeBodyForceImpl.h:69-72:
-                // C9 = 1.5 a_A a_B a_C w_A w_B w_C (w_A+w_B+w_C) / ((w_A+w_B)(w_B+w_C)(w_C+w_A))
-                c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.z*(pa.y+pb.y)*pb.z);
-                if (alchI || pa.w < 0 || pb.w < 0)
-                    c9 *= t.lambda;
-                ok = (c9 != 0);
+            else {
+                b = (int) ((1.0f+__builtin_amdgcn_sqrtf(1.0f+8.0f*(float) p))*0.5f);
+                while (b*(b-1)/2 > p)
+                    b--;
+                while ((b+1)*b/2 <= p)
+                    b++;
+                a = p-b*(b-1)/2;
             }
-            if (ok) {
-                // InteraxThreeBodyTerms.h:97-146 with a = AB, b = AC, c = BC
-                const Real A[3] = {va.x, va.y, va.z}, B[3] = {vb.x, vb.y, vb.z};
-                const Real u = va.w, v = vb.w;
-                Real w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
-                Real pp = A[0]*B[0]+A[1]*B[1]+A[2]*B[2];
-                Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
-                Real s = B[0]*c[0]+B[1]*c[1]+B[2]*c[2];
-                // FP32: v_rcp_f32 and v_rsq_f32 (an ulp or two) in place of correctly rounded division and sqrt
-                // one reciprocal square root for (uvw)^-1/2, (uvw)^-1, (uvw)^-3/2 and 1/w = uv/(uvw)
-                const Real uvw = u*v*w;
-                const Real rs = fast_rsqrt(uvw);
-                Real invP = rs*rs;
-                Real invP32 = invP*rs;
-                Real invP52 = invP32*invP;
-                Real pqs = pp*q*s;
-                Real e = c9*(invP32-3*pqs*invP52);
-                Real radial = c9*(-3*invP32+15*pqs*invP52);
-                Real cross = -3*c9*invP52;
-                Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*(invP*(u*v));
-                // With X = pp s cross, Y = pp q cross, Z = q s cross the gradients with respect to A, B and C = B - A are
-                // dA = A ru + B Z + C X, dB = B rv + A Z + C Y, dC = C rw + A X + B Y; the forces on the neighbours
-                // a (dC - dA) and b (-(dB + dC)) and the sum dA + dB for A are combinations of A, B, C with these
-                // coefficients.
-                const Real X = pp*s*cross, Y = pp*q*cross, Z = q*s*cross;
-                Real aA = X-ru, aB = Y-Z, aC = rw-X;
-                Real bA = -(Z+X), bB = -(rv+Y), bC = -(Y+rw);
-                Real sA = ru+Z, sB = Z+rv, sC = X+Y;
-                if (t.useDamping || t.invDampingRadius > 0) {
-                    Real len[3] = {sqrt(u), sqrt(v), sqrt(w)};
-                    const Real widthA = sWidth[a], widthB = sWidth[b];
-                    Real rates[3] = {2*t.dampingScale/(sI+widthA), 2*t.dampingScale/(sI+widthB), 2*t.dampingScale/(widthA+widthB)};
-                    Real f[3], dfdr[3];
-                    for (int k = 0; k < 3; k++) {
-                        if (t.useDamping) {
-                            tt3<Real>(rates[k]*len[k], f[k], dfdr[k]);
-                            dfdr[k] *= rates[k];
-                        }
-                        else {
-                            f[k] = 1;
-                            dfdr[k] = 0;
-                        }
-                        if (t.invDampingRadius > 0) {
-                            // InteraxThreeBodyTerms.h:71-77
-                            Real x = len[k]*t.invDampingRadius;
-                            Real x6 = x*x*x;
-                            x6 *= x6;
-                            Real g = -expm1(-x6);
-                            Real dg = 6*x6*exp(-x6)/len[k];
-                            dfdr[k] = dfdr[k]*g + f[k]*dg;
-                            f[k] *= g;
+            const bool active = (p < myEnd);
+            // Rows of 64 pairs or more: a step's active lanes (lane 0 always is one) share b or b and b + 1, whose forces
+            // go to accR and accR1 instead of up to 64 atomics on one LDS address; the step that straddles the two
+            // orders, and the rotation order, add per lane.
+            const int bFirst = __builtin_amdgcn_readfirstlane(b);
+            const bool rows = (p0 >= T) && __all(!active || b == bFirst || b == bFirst+1);
+            if (rows && bFirst != R) {
+                if (R >= 0) {
+                    flushRow(R, accR);
+                    if (bFirst == R+1) {
+                        for (int k = 0; k < 3; k++) {
+                            accR[k] = accR1[k];
+                            accR1[k] = 0;
                         }
                     }
-                    // damped gradients: dA' = dA f + A sa, dB' = dB f + B sb, dC' = dC f + C sc
-                    Real damping = f[0]*f[1]*f[2];
-                    Real sa = e*dfdr[0]*f[1]*f[2]/len[0];
-                    Real sb = e*f[0]*dfdr[1]*f[2]/len[1];
-                    Real sc = e*f[0]*f[1]*dfdr[2]/len[2];
-                    aA = aA*damping-sa;
-                    aB *= damping;
-                    aC = aC*damping+sc;
-                    bA *= damping;
-                    bB = bB*damping-sb;
-                    bC = bC*damping-sc;
-                    sA = sA*damping+sa;
-                    sB = sB*damping+sb;
-                    sC *= damping;
-                    e *= damping;
+                    else if (usedR1)
+                        flushRow(R+1, accR1);
                 }
-                energy += (double) e;
-                count++;
-                for (int k = 0; k < 3; k++) {
-                    fA[k] += A[k]*sA+B[k]*sB+c[k]*sC;
-                    atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(A[k]*aA+B[k]*aB+c[k]*aC));
-                    const Real forceB = A[k]*bA+B[k]*bB+c[k]*bC;
-                    if (!rows)
-                        atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(forceB));
-                    else if (b == R)
-                        accR[k] += forceB;
+                R = bFirst;
+                usedR1 = false;
+            }
+            if (rows)
+                usedR1 = usedR1 || __any(active && b != R);
+            if (active) {
+                bool ok = (sMol[a] != sMol[b]);
+                R4 va = sDelta[a], vb = sDelta[b];
+                Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
+                if (ok) {
+                    bool exact = false;
+                    if (t.periodic) {
+                        // the reference's minimum image of BC; near a change of image the choice is made in FP64
+                        Real u2 = c[2]*t.boxR[8]+Real(0.5);
+                        Real s2 = floor(u2);
+                        c[0] -= s2*t.boxR[3];
+                        c[1] -= s2*t.boxR[4];
+                        c[2] -= s2*t.boxR[5];
+                        Real u1 = c[1]*t.boxR[7]+Real(0.5);
+                        Real s1 = floor(u1);
+                        c[0] -= s1*t.boxR[1];
+                        c[1] -= s1*t.boxR[2];
+                        Real u0 = c[0]*t.boxR[6]+Real(0.5);
+                        Real s0 = floor(u0);
+                        c[0] -= s0*t.boxR[0];
+                        Real f2 = u2-s2, f1 = u1-s1, f0 = u0-s0;
+                        exact = (f2 < t.band || f2 > 1-t.band || f1 < t.band || f1 > 1-t.band || f0 < t.band || f0 > 1-t.band);
+                    }
+                    Real r2 = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+                    if (exact || fabs(r2-t.cutoff2r) < t.band*t.cutoff2r) {
+                        double d[3];
+                        ref_delta(t.sites+3*sIndex[a], t.sites+3*sIndex[b], t.periodic, t.box, d);
+                        ok = ref_norm2(d) < t.cutoff2;
+                        c[0] = Real(d[0]);
+                        c[1] = Real(d[1]);
+                        c[2] = Real(d[2]);
+                    }
                     else
-                        accR1[k] += forceB;
+                        ok = r2 < t.cutoff2r;
+                }
+                R4 pa = sParam[a], pb = sParam[b];
+                Real c9 = 0;
+                if (ok) {
+                    // ReferenceInteraxThreeBodyIxn.cpp:116-120 and InteraxThreeBodyForceImpl.h:69-72:
+                    // C9 = 1.5 a_A a_B a_C w_A w_B w_C (w_A+w_B+w_C) / ((w_A+w_B)(w_B+w_C)(w_C+w_A))
+                    c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.z*(pa.y+pb.y)*pb.z);
+                    if (alchI || pa.w < 0 || pb.w < 0)
+                        c9 *= t.lambda;
+                    ok = (c9 != 0);
+                }
+                if (ok) {
+                    // InteraxThreeBodyTerms.h:97-146 with a = AB, b = AC, c = BC
+                    const Real A[3] = {va.x, va.y, va.z}, B[3] = {vb.x, vb.y, vb.z};
+                    const Real u = va.w, v = vb.w;
+                    Real w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+                    Real pp = A[0]*B[0]+A[1]*B[1]+A[2]*B[2];
+                    Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
+                    Real s = B[0]*c[0]+B[1]*c[1]+B[2]*c[2];
+                    // FP32: v_rcp_f32 and v_rsq_f32 (an ulp or two) in place of correctly rounded division and sqrt
+                    // one reciprocal square root for (uvw)^-1/2, (uvw)^-1, (uvw)^-3/2 and 1/w = uv/(uvw)
+                    const Real uvw = u*v*w;
+                    const Real rs = fast_rsqrt(uvw);
+                    Real invP = rs*rs;
+                    Real invP32 = invP*rs;
+                    Real invP52 = invP32*invP;
+                    Real pqs = pp*q*s;
+                    Real e = c9*(invP32-3*pqs*invP52);
+                    Real radial = c9*(-3*invP32+15*pqs*invP52);
+                    Real cross = -3*c9*invP52;
+                    Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*(invP*(u*v));
+                    // With X = pp s cross, Y = pp q cross, Z = q s cross the gradients with respect to A, B and C = B - A are
+                    // dA = A ru + B Z + C X, dB = B rv + A Z + C Y, dC = C rw + A X + B Y; the forces on the neighbours
+                    // a (dC - dA) and b (-(dB + dC)) and the sum dA + dB for A are combinations of A, B, C with these
+                    // coefficients.
+                    const Real X = pp*s*cross, Y = pp*q*cross, Z = q*s*cross;
+                    Real aA = X-ru, aB = Y-Z, aC = rw-X;
+                    Real bA = -(Z+X), bB = -(rv+Y), bC = -(Y+rw);
+                    Real sA = ru+Z, sB = Z+rv, sC = X+Y;
+                    if (t.useDamping || t.invDampingRadius > 0) {
+                        Real len[3] = {sqrt(u), sqrt(v), sqrt(w)};
+                        const Real widthA = sWidth[a], widthB = sWidth[b];
+                        Real rates[3] = {2*t.dampingScale/(sI+widthA), 2*t.dampingScale/(sI+widthB), 2*t.dampingScale/(widthA+widthB)};
+                        Real f[3], dfdr[3];
+                        for (int k = 0; k < 3; k++) {
+                            if (t.useDamping) {
+                                tt3<Real>(rates[k]*len[k], f[k], dfdr[k]);
+                                dfdr[k] *= rates[k];
+                            }
+                            else {
+                                f[k] = 1;
+                                dfdr[k] = 0;
+                            }
+                            if (t.invDampingRadius > 0) {
+                                // InteraxThreeBodyTerms.h:71-77
+                                Real x = len[k]*t.invDampingRadius;
+                                Real x6 = x*x*x;
+                                x6 *= x6;
+                                Real g = -expm1(-x6);
+                                Real dg = 6*x6*exp(-x6)/len[k];
+                                dfdr[k] = dfdr[k]*g + f[k]*dg;
+                                f[k] *= g;
+                            }
+                        }
+                        // damped gradients: dA' = dA f + A sa, dB' = dB f + B sb, dC' = dC f + C sc
+                        Real damping = f[0]*f[1]*f[2];
+                        Real sa = e*dfdr[0]*f[1]*f[2]/len[0];
+                        Real sb = e*f[0]*dfdr[1]*f[2]/len[1];
+                        Real sc = e*f[0]*f[1]*dfdr[2]/len[2];
+                        aA = aA*damping-sa;
+                        aB *= damping;
+                        aC = aC*damping+sc;
+                        bA *= damping;
+                        bB = bB*damping-sb;
+                        bC = bC*damping-sc;
+                        sA = sA*damping+sa;
+                        sB = sB*damping+sb;
+                        sC *= damping;
+                        e *= damping;
+                    }
+                    energy += (double) e;
+                    count++;
+                    for (int k = 0; k < 3; k++) {
+                        fA[k] += A[k]*sA+B[k]*sB+c[k]*sC;
+                        atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(A[k]*aA+B[k]*aB+c[k]*aC));
+                        const Real forceB = A[k]*bA+B[k]*bB+c[k]*bC;
+                        if (!rows)
+                            atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(forceB));
+                        else if (b == R)
+                            accR[k] += forceB;
+                        else
+                            accR1[k] += forceB;
+                    }
                 }
             }
         }
-    }
-    if (R >= 0) {
-        flushRow(R, accR);
-        if (usedR1)
-            flushRow(R+1, accR1);
+        if (R >= 0) {
+            flushRow(R, accR);
+            if (usedR1)
+                flushRow(R+1, accR1);
+        }
+        R = -1;
+        usedR1 = false;
+        if (shared) {
+            // the share's force on A and its energy and triple count
+            for (int k = 0; k < 3; k++) {
+                long long f = to_fixed(fA[k]);
+                for (int offset = 32; offset > 0; offset >>= 1)
+                    f += __shfl_xor(f, offset, 64);
+                if (lane == 0)
+                    add_fixed(&t.siteForce[(size_t) k*t.numSites+i], f);
+                fA[k] = 0;
+            }
+            const double e = wave_sum(energy);
+            const int c = (int) wave_sum((double) count);
+            if (lane == 0) {
+                t.itemEnergy[t.itemShareBase[item]+v] = e;
+                t.itemCount[t.itemShareBase[item]+v] = c;
+            }
+            energy = 0;
+            count = 0;
+        }
     }
     __syncthreads();
     if (t.skinLists) {
@@ -783,20 +819,15 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
         }
     }
     else {
-        // flush the neighbour forces and the force on A (integer sums: the order of the atomics does not matter)
+        // flush the neighbour forces (integer sums: the order of the atomics does not matter)
         for (int a = threadIdx.x; a < m; a += 64*WAVES) {
             int j = sIndex[a];
             for (int k = 0; k < 3; k++)
                 add_fixed(&t.siteForce[(size_t) k*t.numSites+j], (long long) sForce[3*a+k]);
         }
-        for (int k = 0; k < 3; k++) {
-            long long v = to_fixed(fA[k]);
-            for (int offset = 32; offset > 0; offset >>= 1)
-                v += __shfl_xor(v, offset, 64);
-            if (lane == 0)
-                add_fixed(&t.siteForce[(size_t) k*t.numSites+i], v);
-        }
     }
+    if (shared)
+        return;
     // energy and count: per wavefront, then summed in wavefront order
     __shared__ double sWaveEnergy[WAVES];
     __shared__ int sWaveCount[WAVES];
@@ -851,24 +882,37 @@ __global__ void k_neighbors_segments(const double* sites, const int* molecule, c
         count[i] = written;
 }
 
-// Energy of every segment: the energies of its work items (ranges[s] .. ranges[s+1]) summed in order (one workgroup
-// per segment, fixed tree), added to energies[s].
-__global__ void __launch_bounds__(256) k_reduce_site_segments(const double* siteEnergy, const int* segOffsets, const int* segList,
+// Energy and triple count of every segment: those of its shares (ranges[s] .. ranges[s+1]) summed in order (one
+// workgroup per segment, fixed tree), into segEnergy[s] and segCount[s], and the energy added to energies[s] (if not
+// NULL).
+__global__ void __launch_bounds__(256) k_reduce_site_segments(const double* shareEnergy, const int* shareCount, const int* ranges,
+                                                               const int* segList, double* segEnergy, int* segCount,
                                                                double* energies) {
     __shared__ double sE[256];
+    __shared__ int sC[256];
     const int seg = (segList != nullptr ? segList[blockIdx.x] : (int) blockIdx.x);
     double e = 0;
-    for (int k = segOffsets[seg]+threadIdx.x; k < segOffsets[seg+1]; k += 256)
-        e += siteEnergy[k];
+    int c = 0;
+    for (int k = ranges[seg]+threadIdx.x; k < ranges[seg+1]; k += 256) {
+        e += shareEnergy[k];
+        c += shareCount[k];
+    }
     sE[threadIdx.x] = e;
+    sC[threadIdx.x] = c;
     __syncthreads();
     for (int stride = 128; stride > 0; stride >>= 1) {
-        if (threadIdx.x < stride)
+        if (threadIdx.x < stride) {
             sE[threadIdx.x] += sE[threadIdx.x+stride];
+            sC[threadIdx.x] += sC[threadIdx.x+stride];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        energies[seg] += sE[0];
+    if (threadIdx.x == 0) {
+        segEnergy[seg] = sE[0];
+        segCount[seg] = sC[0];
+        if (energies != nullptr)
+            energies[seg] += sE[0];
+    }
 }
 
 __global__ void k_zero_long(long long* p, long long n) {
@@ -1069,12 +1113,19 @@ struct ixh_threebody {
     int numSegments = 0;
     std::vector<int> segOffsets;
     DeviceArray<int> segOffsetsDev, siteEnd;
+    DeviceArray<double> segEnergy;   // energy and triple count of every segment of the last evaluation
+    DeviceArray<int> segCount;
     int maxSegment = 0;
-    // work items of the triple kernel (segmented handles): site and pair chunk, items of segment s in
-    // [segItems[s], segItems[s+1])
-    int numItems = 0;
-    std::vector<int> segItems;
-    DeviceArray<int> itemSite, itemChunk, segItemsDev, subsetItems;
+    // work items of the triple kernel (segmented handles): a site and consecutive shares of its neighbour pairs, items
+    // of segment s in [segItems[s], segItems[s+1]); shares of segment s in [segShares[s], segShares[s+1])
+    int numItems = 0, numShares = 0;
+    std::vector<int> segItems, segShares;
+    // launch shape of the triple kernel of segmented handles: wavefronts per work item, neighbour pairs per work item
+    int tripleWaves = 4;
+    int chunkPairs = TRIPLE_CHUNK;
+    DeviceArray<int> itemSite, itemShareBase, segItemsDev, segSharesDev, subsetItems;
+    DeviceArray<int> itemFirst;
+    DeviceArray<int2> itemShares;
     DeviceArray<int> itemOrder;   // all items, most pairs first (the launch order of a full evaluation)
     // subsets of segments: site list and segment list of the last call
     int* subsetStaging = nullptr;
@@ -1096,6 +1147,69 @@ struct ixh_threebody {
 };
 
 namespace {
+
+// Shares of a site's neighbour pairs (segmented handles): about a quarter of the pairs its list can hold, a multiple of
+// 64 between SHARE_MIN and SHARE_MAX pairs.  They depend on the capacity alone, never on the launch shape.
+constexpr long long SHARE_MIN = 64, SHARE_MAX = 4096;
+
+// The work items of a segmented handle's triple kernel: every site's shares, about chunkPairs pairs of them per item
+// (at least one share), launched with the items of the most pairs first.
+void build_items(ixh_threebody* h) {
+    std::vector<int> itemSite, shareBase, itemFirst;
+    std::vector<long long> work;
+    std::vector<int2> itemShares;
+    h->segItems.assign(h->numSegments+1, 0);
+    h->segShares.assign(h->numSegments+1, 0);
+    long long shares = 0;
+    for (int g = 0; g < h->numSegments; g++) {
+        for (int i = h->segOffsets[g]; i < h->segOffsets[g+1]; i++) {
+            const long long cap = h->segOffsets[g+1]-i-1;
+            const long long pairs = (cap >= 2 ? cap*(cap-1)/2 : 0);
+            if (pairs > INT32_MAX)
+                throw Error(IXH_ERROR_CAPACITY, "ixh_threebody_create_segmented: a segment has too many sites for the triple kernel");
+            const long long size = std::min(SHARE_MAX, std::max(SHARE_MIN, ((pairs+3)/4+63)/64*64));
+            const long long count = std::max(1LL, (pairs+size-1)/size);
+            const long long perItem = std::max(1LL, (long long) h->chunkPairs/size);
+            for (long long f = 0; f < count; f += perItem) {
+                const long long n = std::min(perItem, count-f);
+                itemSite.push_back(i);
+                itemFirst.push_back((int) (f*size));
+                itemShares.push_back(make_int2((int) n, (int) size));
+                shareBase.push_back((int) (shares+f));
+                work.push_back(std::max(0LL, std::min(pairs, (f+n)*size)-f*size));
+            }
+            shares += count;
+            if (shares > INT32_MAX || itemSite.size() > (size_t) INT32_MAX)
+                throw Error(IXH_ERROR_CAPACITY, "ixh_threebody_create_segmented: too many work items");
+        }
+        h->segItems[g+1] = (int) itemSite.size();
+        h->segShares[g+1] = (int) shares;
+    }
+    h->numItems = (int) itemSite.size();
+    h->numShares = (int) shares;
+    std::vector<int> order(itemSite.size());
+    for (size_t w = 0; w < order.size(); w++)
+        order[w] = (int) w;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    const size_t items = std::max<size_t>(order.size(), 1);
+    h->itemOrder.reserve(items);
+    h->itemSite.reserve(items);
+    h->itemShareBase.reserve(items);
+    h->itemFirst.reserve(items);
+    h->itemShares.reserve(items);
+    if (!order.empty()) {
+        h->itemOrder.upload(order.data(), order.size(), nullptr);
+        h->itemSite.upload(itemSite.data(), itemSite.size(), nullptr);
+        h->itemShareBase.upload(shareBase.data(), shareBase.size(), nullptr);
+        h->itemFirst.upload(itemFirst.data(), itemFirst.size(), nullptr);
+        h->itemShares.upload(itemShares.data(), itemShares.size(), nullptr);
+    }
+    h->segItemsDev.upload(h->segItems.data(), h->segItems.size(), nullptr);
+    h->segSharesDev.upload(h->segShares.data(), h->segShares.size(), nullptr);
+    h->siteEnergy.reserve(std::max(h->numShares, 1));
+    h->siteCount.reserve(std::max(h->numShares, 1));
+    IXH_HIP_CHECK(hipStreamSynchronize(nullptr));
+}
 
 void check_threebody(const ixh_threebody_desc* d) {
     if (d == nullptr)
@@ -1275,9 +1389,10 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.count = h->count.ptr;
     const bool items = (h->numSegments > 0);
     t.itemSite = (items ? h->itemSite.ptr : nullptr);
-    t.itemChunk = (items ? h->itemChunk.ptr : nullptr);
+    t.itemFirst = (items ? h->itemFirst.ptr : nullptr);
+    t.itemShares = (items ? h->itemShares.ptr : nullptr);
+    t.itemShareBase = (items ? h->itemShareBase.ptr : nullptr);
     t.itemIndex = (subsetItems >= 0 ? h->subsetItems.ptr : (items ? h->itemOrder.ptr : nullptr));
-    t.chunkPairs = TRIPLE_CHUNK;
     t.nbrIndex = h->nbrIndex.ptr;
     t.nbrDelta = h->nbrDelta.ptr;
     t.numSites = h->numSites;
@@ -1313,7 +1428,13 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     bool timed = h->timer.start(stream);
     const int workgroups = (subsetItems >= 0 ? subsetItems : (items ? h->numItems : h->numSites));
     if (workgroups > 0) {
-        if (items)
+        if (items && h->tripleWaves == 1)
+            hipLaunchKernelGGL((k_triples<Real, 1>), dim3(workgroups), dim3(64), shared, stream, t);
+        else if (items && h->tripleWaves == 2)
+            hipLaunchKernelGGL((k_triples<Real, 2>), dim3(workgroups), dim3(128), shared, stream, t);
+        else if (items && h->tripleWaves == 8)
+            hipLaunchKernelGGL((k_triples<Real, 8>), dim3(workgroups), dim3(512), shared, stream, t);
+        else if (items)
             hipLaunchKernelGGL((k_triples<Real, 4>), dim3(workgroups), dim3(256), shared, stream, t);
         else
             hipLaunchKernelGGL((k_triples<Real, 1>), dim3(workgroups), dim3(64), shared, stream, t);
@@ -1346,6 +1467,31 @@ void finish_plain(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, doubl
 }
 
 }  // namespace
+
+namespace ixh {
+
+void threebody_set_launch_shape(ixh_threebody* h, int waves, int chunkPairs) {
+    if (waves != 1 && waves != 2 && waves != 4 && waves != 8)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody: the triple kernel takes 1, 2, 4 or 8 wavefronts per work item");
+    if (chunkPairs < 1024 || chunkPairs > (1 << 22) || chunkPairs%64 != 0)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody: pairs per work item must be a multiple of 64 in [1024, 2^22]");
+    h->tripleWaves = waves;
+    if (chunkPairs != h->chunkPairs) {
+        h->chunkPairs = chunkPairs;
+        if (h->numSegments > 0) {
+            DeviceGuard guard(h->device);
+            IXH_HIP_CHECK(hipDeviceSynchronize());   // no launch in flight reads the items
+            build_items(h);
+        }
+    }
+}
+
+void threebody_launch_shape(const ixh_threebody* h, int* waves, int* chunkPairs) {
+    *waves = h->tripleWaves;
+    *chunkPairs = h->chunkPairs;
+}
+
+}  // namespace ixh
 
 extern "C" {
 
@@ -1416,44 +1562,9 @@ int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* d, int 
                     }
                 }
                 start[n] = (int) total;
-                // work items: every site's neighbour pairs in chunks
-                std::vector<int> itemSite, itemChunk;
-                h->segItems.assign(num_segments+1, 0);
-                for (int g = 0; g < num_segments; g++) {
-                    for (int i = site_offsets[g]; i < site_offsets[g+1]; i++) {
-                        const long long cap = site_offsets[g+1]-i-1;
-                        const long long pairs = cap*(cap-1)/2;
-                        const long long chunks = std::max(1LL, (pairs+TRIPLE_CHUNK-1)/TRIPLE_CHUNK);
-                        for (long long c = 0; c < chunks; c++) {
-                            itemSite.push_back(i);
-                            itemChunk.push_back((int) c);
-                        }
-                    }
-                    if (itemSite.size() > (size_t) INT32_MAX)
-                        throw Error(IXH_ERROR_CAPACITY, "ixh_threebody_create_segmented: too many work items");
-                    h->segItems[g+1] = (int) itemSite.size();
-                }
-                h->numItems = (int) itemSite.size();
-                // launch order: items with the most pairs first, so that the workgroups dispatched last are the lightest
-                {
-                    std::vector<long long> work(itemSite.size());
-                    for (size_t w = 0; w < itemSite.size(); w++) {
-                        const int i = itemSite[w];
-                        const long long cap = end[i]-i-1;
-                        const long long pairs = cap*(cap-1)/2;
-                        work[w] = std::min<long long>(TRIPLE_CHUNK, std::max(0LL, pairs-(long long) itemChunk[w]*TRIPLE_CHUNK));
-                    }
-                    std::vector<int> order(itemSite.size());
-                    for (size_t w = 0; w < order.size(); w++)
-                        order[w] = (int) w;
-                    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
-                    h->itemOrder.upload(order.data(), std::max<size_t>(order.size(), 1), nullptr);
-                }
-                h->itemSite.upload(itemSite.data(), std::max<size_t>(itemSite.size(), 1), nullptr);
-                h->itemChunk.upload(itemChunk.data(), std::max<size_t>(itemChunk.size(), 1), nullptr);
-                h->segItemsDev.upload(h->segItems.data(), h->segItems.size(), nullptr);
-                h->siteEnergy.reserve(std::max(h->numItems, 1));
-                h->siteCount.reserve(std::max(h->numItems, 1));
+                build_items(h);
+                h->segEnergy.reserve(num_segments);
+                h->segCount.reserve(num_segments);
                 h->segOffsetsDev.upload(h->segOffsets.data(), h->segOffsets.size(), nullptr);
                 h->siteEnd.upload(end.data(), std::max<size_t>(end.size(), 1), nullptr);
                 h->start.upload(start.data(), start.size(), nullptr);
@@ -1654,15 +1765,18 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                                    h->offsetsDev.ptr, h->particlesDev.ptr, b->force, b->force_stride);
                 IXH_HIP_CHECK(hipGetLastError());
             }
-            if (subsetSites < 0) {
-                hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(1024), 0, stream, h->siteEnergy.ptr, h->siteCount.ptr, h->numItems,
-                                   b->energy, h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
+            // energies and triple counts per segment, then (whole evaluations) the total over the segments in order
+            const int reduced = (subsetSites >= 0 ? numSelected : h->numSegments);
+            if (reduced > 0) {
+                hipLaunchKernelGGL(k_reduce_site_segments, dim3(reduced), dim3(256), 0, stream, (const double*) h->siteEnergy.ptr,
+                                   (const int*) h->siteCount.ptr, (const int*) h->segSharesDev.ptr,
+                                   subsetSites >= 0 ? (const int*) h->subsetSegments.ptr : (const int*) nullptr,
+                                   h->segEnergy.ptr, h->segCount.ptr, segmentEnergies);
                 IXH_HIP_CHECK(hipGetLastError());
             }
-            const int reduced = (subsetSites >= 0 ? numSelected : h->numSegments);
-            if (segmentEnergies != nullptr && reduced > 0) {
-                hipLaunchKernelGGL(k_reduce_site_segments, dim3(reduced), dim3(256), 0, stream, h->siteEnergy.ptr, h->segItemsDev.ptr,
-                                   subsetSites >= 0 ? (const int*) h->subsetSegments.ptr : (const int*) nullptr, segmentEnergies);
+            if (subsetSites < 0) {
+                hipLaunchKernelGGL(k_reduce_sites, dim3(1), dim3(1024), 0, stream, h->segEnergy.ptr, h->segCount.ptr, h->numSegments,
+                                   b->energy, h->tripleCount.ptr, (flags & IXH_INCLUDE_ENERGY) != 0);
                 IXH_HIP_CHECK(hipGetLastError());
             }
             IXH_HIP_CHECK(hipEventRecord(h->done, stream));

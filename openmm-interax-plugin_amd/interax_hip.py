@@ -84,6 +84,10 @@ class ThreeBodyDeviceParameters(ctypes.Structure):
                 ("centroid_damping_radius", ctypes.c_double), ("stream", ctypes.c_void_p)]
 
 
+class BatchShape(ctypes.Structure):
+    _fields_ = [("pair_workgroups", ctypes.c_int), ("triple_waves", ctypes.c_int), ("triple_pairs_per_item", ctypes.c_int)]
+
+
 class BatchDesc(ctypes.Structure):
     _fields_ = [("num_systems", ctypes.c_int), ("particle_offsets", _ip), ("pair", ctypes.POINTER(PairDesc)),
                 ("site_offsets", _ip), ("threebody", ctypes.POINTER(ThreeBodyDesc)), ("precision", ctypes.c_int)]
@@ -145,7 +149,9 @@ SIGNATURES = {
     "ixh_batch_get_energies": [ctypes.c_void_p, _dp],
     "ixh_batch_get_forces": [ctypes.c_void_p, _dp],
     "ixh_batch_set_graph": [ctypes.c_void_p, ctypes.c_int],
-    "ixh_batch_autotune": [ctypes.c_void_p, ctypes.c_int, _ip],
+    "ixh_batch_autotune": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(BatchShape)],
+    "ixh_batch_get_shape": [ctypes.c_void_p, ctypes.POINTER(BatchShape)],
+    "ixh_batch_set_shape": [ctypes.c_void_p, ctypes.POINTER(BatchShape)],
     "ixh_pair_execute_subset": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_int, ctypes.c_void_p, _ip,
                                 ctypes.c_int],
     "ixh_threebody_execute_subset": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, ctypes.c_void_p, _ip, ctypes.c_int],
@@ -1188,10 +1194,20 @@ class Batch:
     def set_graph(self, enable: bool) -> None:
         _check(_lib.ixh_batch_set_graph(self.handle, int(enable)))
 
-    def autotune(self) -> int:
-        chosen = ctypes.c_int()
+    def autotune(self) -> dict:
+        """Time the launch shapes on the current positions and keep the fastest; returns the chosen shape."""
+        chosen = BatchShape()
         _check(_lib.ixh_batch_autotune(self.handle, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY, ctypes.byref(chosen)))
-        return chosen.value
+        return {name: getattr(chosen, name) for name, _ in BatchShape._fields_}
+
+    def shape(self) -> dict:
+        s = BatchShape()
+        _check(_lib.ixh_batch_get_shape(self.handle, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in BatchShape._fields_}
+
+    def set_shape(self, pair_workgroups: int, triple_waves: int, triple_pairs_per_item: int) -> None:
+        s = BatchShape(pair_workgroups, triple_waves, triple_pairs_per_item)
+        _check(_lib.ixh_batch_set_shape(self.handle, ctypes.byref(s)))
 
     def close(self) -> None:
         if self.handle:

@@ -89,12 +89,39 @@ def test_autotune_keeps_results():
     batch = ih.Batch(systems, "mixed")
     e0, f0 = batch.evaluate(positions)
     chosen = batch.autotune()
-    assert chosen > 0
+    assert chosen["pair_workgroups"] > 0 and chosen["triple_waves"] in (1, 2, 4, 8)
+    assert chosen == batch.shape()
     e1, f1 = batch.evaluate(positions)
     assert np.array_equal(e0, e1)
     for a, b in zip(f0, f1):
         assert np.array_equal(a, b)
     batch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_every_launch_shape_gives_the_same_results(precision):
+    # the shares of a site's neighbour pairs depend on its list capacity only, and are summed in fixed point (forces)
+    # or in share order (energies): the wavefronts per work item and the pairs per work item change nothing
+    systems, positions = clusters([70, 3, 130, 9, 41], seed=5)
+    batch = ih.Batch(systems, precision)
+    reference = None
+    try:
+        for workgroups, waves, pairs in [(4096, 4, 16384), (256, 1, 1024), (512, 2, 8192), (1024, 8, 65536), (2048, 4, 4096),
+                                         (8192, 8, 1024), (4096, 1, 262144)]:
+            print(f"shape: {workgroups} pair workgroups, {waves} waves, {pairs} pairs per item", flush=True)
+            batch.set_shape(workgroups, waves, pairs)
+            e, f = batch.evaluate(positions)
+            e_graph, f_graph = batch.evaluate(positions)
+            assert np.array_equal(e, e_graph)
+            if reference is None:
+                reference = (e, f)
+                continue
+            assert np.array_equal(e, reference[0])
+            for a, b in zip(f, reference[1]):
+                assert np.array_equal(a, b)
+    finally:
+        batch.close()
 
 
 @pytest.mark.gpu
