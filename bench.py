@@ -166,6 +166,11 @@ def bench_pair(args):
         ih._check(lib.ixh_state_set_stream(st, stream))
         ih._check(lib.ixh_state_get_buffers(st, ctypes.byref(frames.buffers[k])))
     reruns = [0]
+    # launch shape of the tile kernel: tiles per work item, tuned on the first coordinate set
+    t0 = time.time()
+    tiles_per_item = ctypes.c_int()
+    ih._check(lib.ixh_pair_autotune(group.handle, ctypes.byref(frames.buffers[0]), terms, ctypes.byref(tiles_per_item)))
+    log(f"[bench] tile kernel tuned in {time.time()-t0:.2f} s: {tiles_per_item.value} tiles per work item")
 
     def step(k):
         ih._check(lib.ixh_state_clear(frames.states[k]))
@@ -218,9 +223,10 @@ def bench_pair(args):
                                "cutoff 1.0 nm, long range correction, dispersion+exchange+short range fused, coordinates "
                                "perturbed by N(0, 0.005 nm) per step",
                    "atoms": n, "pairs_per_step": pairs, "tiles": tiles.value, "candidate_pairs": candidates.value,
-                   "stale_list_reevaluations": reruns[0],
+                   "stale_list_reevaluations": reruns[0], "tiles_per_work_item": tiles_per_item.value,
                    "precision": "mixed (FP64 positions, FP32 arithmetic, "
                                                                      "64-bit fixed-point forces, FP64 energy)"},
+        "autotune": {"tiles_per_work_item": tiles_per_item.value, "searched": [1, 2, 3, 4, 6]},
         "roofline": {"bound": "valu", "note": "compute-bound: FP32 VALU (no MFMA on this path: nothing here is a dense "
                                               "contraction) against the MI355X FP32 peak; 240 algorithmic flops per "
                                               "interacting pair",

@@ -265,6 +265,11 @@ int ixh_pair_last_pair_count(ixh_pair* pair, long long* count);
    Takes effect at the next evaluation, which rebuilds the neighbour list; results do not depend on it beyond the
    order of the FP64 energy sum. */
 int ixh_pair_set_tiles_per_item(ixh_pair* pair, int tiles_per_item);
+/* Tuner of that shape (no counterpart in the reference): times 1, 2, 3, 4 and 6 tiles per work item on the positions
+   of the buffers (evaluating into scratch buffers of the handle; the buffers' forces and energy are not touched) and
+   keeps the fastest, which is returned in tiles_per_item (may be NULL).  A handle with a cutoff only; waits for the
+   device. */
+int ixh_pair_autotune(ixh_pair* pair, const ixh_buffers* buffers, int terms, int* tiles_per_item);
 /* How the current neighbour list takes periodic images (no counterpart in the reference): of its `tiles` tiles,
    `per_block_tiles` take one image shift per pair of 64-particle blocks (exact for every pair in range when the box is
    wider than twice the cutoff plus the extents of both blocks and the skin); the others take the minimum image of

@@ -3512,6 +3512,67 @@ int ixh_pair_set_tiles_per_item(ixh_pair* p, int tiles_per_item) {
     });
 }
 
+int ixh_pair_autotune(ixh_pair* p, const ixh_buffers* b, int terms, int* chosen) {
+    return guarded([&] {
+        if (p == nullptr || b == nullptr)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_autotune: null argument");
+        if (p->method == IXH_NO_CUTOFF)
+            throw Error(IXH_ERROR_UNSUPPORTED, "ixh_pair_autotune: work items are formed for cutoff handles only");
+        if (b->num_particles != p->n)
+            throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_autotune: the buffers hold a different number of particles");
+        DeviceGuard guard(p->device);
+        hipStream_t stream = (hipStream_t) b->stream;
+        // scratch forces and energy
+        const long long forceWords = 3LL*b->force_stride;
+        p->lambdaForce.reserve(forceWords);
+        p->lambdaEnergy.reserve(1);
+        ixh_buffers own = *b;
+        own.force = p->lambdaForce.ptr;
+        own.energy = p->lambdaEnergy.ptr;
+        hipEvent_t start, stop;
+        IXH_HIP_CHECK(hipEventCreate(&start));
+        IXH_HIP_CHECK(hipEventCreate(&stop));
+        int best = p->tilesPerItem;
+        float bestMs = 1e30f;
+        try {
+            for (int shape : {3, 1, 2, 4, 6}) {
+                if (p->lastStream != nullptr)
+                    IXH_HIP_CHECK(hipStreamSynchronize(p->lastStream));
+                p->tilesPerItem = shape;
+                p->built = false;
+                for (int k = 0; k < 2; k++)   // the list build, then one evaluation with the new list
+                    execute_pair(p, &own, terms, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY, nullptr, nullptr, -1, nullptr);
+                IXH_HIP_CHECK(hipEventRecord(start, stream));
+                for (int k = 0; k < 5; k++)
+                    execute_pair(p, &own, terms, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY | IXH_DEFER_LIST_CHECK, nullptr, nullptr,
+                                 -1, nullptr);
+                IXH_HIP_CHECK(hipEventRecord(stop, stream));
+                IXH_HIP_CHECK(hipEventSynchronize(stop));
+                float ms = 0;
+                IXH_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    best = shape;
+                }
+            }
+        }
+        catch (...) {
+            (void) hipEventDestroy(start);
+            (void) hipEventDestroy(stop);
+            throw;
+        }
+        (void) hipEventDestroy(start);
+        (void) hipEventDestroy(stop);
+        IXH_HIP_CHECK(hipStreamSynchronize(stream));
+        if (best != p->tilesPerItem) {
+            p->tilesPerItem = best;
+            p->built = false;
+        }
+        if (chosen != nullptr)
+            *chosen = best;
+    });
+}
+
 int ixh_pair_image_mode(ixh_pair* p, long long* per_block_tiles, long long* tiles) {
     return guarded([&] {
         if (p == nullptr || per_block_tiles == nullptr || tiles == nullptr)

@@ -120,6 +120,7 @@ SIGNATURES = {
     "ixh_pair_lambda_energies": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp],
     "ixh_pair_last_pair_count": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)],
     "ixh_pair_set_tiles_per_item": [ctypes.c_void_p, ctypes.c_int],
+    "ixh_pair_autotune": [ctypes.c_void_p, ctypes.POINTER(Buffers), ctypes.c_int, _ip],
     "ixh_pair_image_mode": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "ixh_pair_list_statistics": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "ixh_threebody_create": [ctypes.c_int, ctypes.POINTER(ThreeBodyDesc), ctypes.POINTER(ctypes.c_void_p)],
