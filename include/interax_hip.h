@@ -403,26 +403,45 @@ int ixh_batch_reset_kernel_time(ixh_batch* batch);
 
 /*
  * Serving (no counterpart in the reference): a request loop over device-resident systems, with its own thread.
- * Register systems, start, then submit requests (coordinates, lambda) and wait for their tickets.  Cluster systems
- * (NoCutoff pair forces and an optional nonperiodic three-body force, all with the same settings) form a catalog that
- * is evaluated in batches: whenever one of two execution lanes is free, the pending cluster requests are admitted
- * FIFO up to max_batch requests and memory_budget bytes of work in flight.  Periodic systems (cutoff methods, e.g.
- * alchemical boxes evaluated at a lambda) have their own handle and are evaluated one request at a time.  A
- * request's result does not depend on the other requests of its batch.
- * ixh_server_replay submits n requests at the given arrival times (seconds from the call) and returns each one's
- * latency (scheduled arrival to completion) and energy, and the time from the call to the last completion.
+ * Register systems, start, then submit requests and wait for their tickets; systems may also be registered while the
+ * server runs.  Cluster systems (NoCutoff pair forces and an optional nonperiodic three-body force, all with the same
+ * settings) form a catalog that is evaluated in batches: whenever one of two execution lanes is free, the pending
+ * cluster requests are admitted in arrival order up to max_batch requests and the memory budget; a system with several
+ * requests in a batch is evaluated in rounds of one request per system.  A cluster registered while the server runs is
+ * appended to a new catalog, built by the registering call while the lanes serve the old one and taken up by each lane
+ * when it is next idle; its requests wait until then.  Periodic systems (cutoff methods, e.g. alchemical boxes) have
+ * execution slots of their own (a handle and a state each, one request per slot, all on one stream): a request at one
+ * lambda (energy and forces), or, through ixh_server_submit_windows, the energies of one configuration at up to 16
+ * values of lambda by the fused lambda-window evaluation (ixh_pair_lambda_energies; long range correction included).
+ * The optional positions of ixh_server_add_periodic (3 per particle) are evaluated once at registration, which builds
+ * the slots' neighbour lists and sizes their buffers.  A request's result does not depend on the other requests of
+ * its batch.
+ * Memory: memory_budget bounds the memory of the requests in flight.  A cluster request holds its share of its lane's
+ * staging (pinned host memory that the scatter and gather kernels read and write directly: 56 bytes per particle and
+ * 24 per request); a periodic request holds its slot, whose device memory is measured when the slot is created (the
+ * library's own allocations).  A request is admitted when it fits beside
+ * those in flight, or when nothing is in flight.  ixh_server_memory reports the resident device memory of the catalogs
+ * and slots, the bytes in flight and their peak.
+ * ixh_server_replay submits n requests at the given arrival times (seconds from the call; window_counts, optional,
+ * gives each request's number of lambda values, taken in order from window_lambdas) and returns each one's latency
+ * (scheduled arrival to completion) and energy (a window request's first), and the time to the last completion.
  */
 typedef struct ixh_server ixh_server;
 int ixh_server_create(int device, int precision, int max_batch, long long memory_budget, ixh_server** server);
 int ixh_server_add_cluster(ixh_server* server, const ixh_pair_desc* pair, const ixh_threebody_desc* threebody, int* entry);
-int ixh_server_add_periodic(ixh_server* server, const ixh_pair_desc* pair, const double* box, int* entry);
+int ixh_server_add_periodic(ixh_server* server, const ixh_pair_desc* pair, const double* box, const double* positions,
+                            int* entry);
 int ixh_server_start(ixh_server* server);
 int ixh_server_submit(ixh_server* server, int entry, const double* positions, double lambda, long long* ticket);
+int ixh_server_submit_windows(ixh_server* server, int entry, const double* positions, const double* lambdas, int count,
+                              long long* ticket);
 int ixh_server_wait(ixh_server* server, long long ticket, double* energy, double* forces, double* latency_ms);
+int ixh_server_wait_windows(ixh_server* server, long long ticket, double* energies, double* latency_ms);
 int ixh_server_statistics(ixh_server* server, long long* batches, long long* batched_requests);
+int ixh_server_memory(ixh_server* server, long long* resident_bytes, long long* in_flight_bytes, long long* peak_in_flight_bytes);
 int ixh_server_replay(ixh_server* server, int n, const double* arrival_s, const int* entries, const double* lambdas,
-                      const long long* position_offsets, const double* positions, double* latency_ms, double* energies,
-                      double* makespan_s);
+                      const long long* position_offsets, const double* positions, const int* window_counts,
+                      const double* window_lambdas, double* latency_ms, double* energies, double* makespan_s);
 int ixh_server_destroy(ixh_server* server);
 
 #ifdef __cplusplus

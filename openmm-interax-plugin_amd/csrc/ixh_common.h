@@ -5,6 +5,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -53,35 +55,96 @@ int guarded(F&& body) {
     }
 }
 
+// Bytes of device memory held by the library's DeviceArrays (all handles, all devices), for the server's admission.
+extern std::atomic<long long> deviceBytes;
+
 // Device memory owned by a handle.
 template <class T>
 struct DeviceArray {
     T* ptr = nullptr;
     size_t count = 0;
+    size_t bytes = 0;
     DeviceArray() = default;
     DeviceArray(const DeviceArray&) = delete;
     DeviceArray& operator=(const DeviceArray&) = delete;
     ~DeviceArray() { release(); }
     void release() {
-        if (ptr != nullptr)
+        if (ptr != nullptr) {
             (void) hipFree(ptr);
+            deviceBytes.fetch_sub((long long) bytes, std::memory_order_relaxed);
+        }
         ptr = nullptr;
         count = 0;
+        bytes = 0;
     }
     // Grow to at least n elements (contents are not kept).
     void reserve(size_t n) {
         if (n <= count && ptr != nullptr)
             return;
         release();
-        size_t bytes = (n == 0 ? 1 : n)*sizeof(T);
-        IXH_HIP_CHECK(hipMalloc((void**) &ptr, bytes));
+        const size_t size = (n == 0 ? 1 : n)*sizeof(T);
+        IXH_HIP_CHECK(hipMalloc((void**) &ptr, size));
         count = n;
+        bytes = size;
+        deviceBytes.fetch_add((long long) size, std::memory_order_relaxed);
     }
     void upload(const T* host, size_t n, hipStream_t stream) {
         reserve(n);
         if (n > 0)
             IXH_HIP_CHECK(hipMemcpyAsync(ptr, host, n*sizeof(T), hipMemcpyHostToDevice, stream));
     }
+};
+
+// Pinned host staging of the small host-to-device copies of an asynchronous call, in a ring of buffers: a call waits
+// only until the copy made from the same buffer RING calls earlier has been read, not for the call just before it
+// (whose copy is queued behind the kernels before it).  Growing one buffer grows all of them.
+struct StagingRing {
+    static constexpr int RING = 8;
+    int* buffer[RING] = {};
+    hipEvent_t copied[RING] = {};
+    size_t capacity = 0;
+    int next = 0;
+    StagingRing() = default;
+    StagingRing(const StagingRing&) = delete;
+    StagingRing& operator=(const StagingRing&) = delete;
+    ~StagingRing() { release(); }
+    void release() {
+        for (int k = 0; k < RING; k++) {
+            if (copied[k] != nullptr) {
+                (void) hipEventSynchronize(copied[k]);
+                (void) hipEventDestroy(copied[k]);
+            }
+            if (buffer[k] != nullptr)
+                (void) hipHostFree(buffer[k]);
+            buffer[k] = nullptr;
+            copied[k] = nullptr;
+        }
+        capacity = 0;
+    }
+    // A buffer of at least n ints that no pending copy reads; `slot` identifies it for recorded().
+    int* acquire(size_t n, int& slot) {
+        if (n > capacity) {
+            for (int k = 0; k < RING; k++) {
+                if (copied[k] != nullptr)
+                    IXH_HIP_CHECK(hipEventSynchronize(copied[k]));
+                if (buffer[k] != nullptr)
+                    IXH_HIP_CHECK(hipHostFree(buffer[k]));
+                buffer[k] = nullptr;
+            }
+            capacity = std::max<size_t>(n, 1024);
+            for (int k = 0; k < RING; k++)
+                IXH_HIP_CHECK(hipHostMalloc((void**) &buffer[k], sizeof(int)*capacity, hipHostMallocDefault));
+        }
+        slot = next;
+        next = (next+1)%RING;
+        if (copied[slot] == nullptr)
+            IXH_HIP_CHECK(hipEventCreateWithFlags(&copied[slot], hipEventDisableTiming));
+        else
+            IXH_HIP_CHECK(hipEventSynchronize(copied[slot]));
+        return buffer[slot];
+    }
+    // The copies from buffer `slot` have been queued on `stream`.
+    void recorded(int slot, hipStream_t stream) { IXH_HIP_CHECK(hipEventRecord(copied[slot], stream)); }
 };
 
 // Sets the device for the lifetime of the object and restores the previous one.

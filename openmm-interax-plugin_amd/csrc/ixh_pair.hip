@@ -1885,9 +1885,7 @@ struct ixh_pair {
     DeviceArray<double> tileEnergy;
     int gridLimit = 1 << 20;   // largest number of workgroups of a tile launch (default: one tile per wave)
     // subsets of segments: the tile list and segment list of the last call (pinned staging, device copies)
-    int* subsetStaging = nullptr;
-    size_t subsetCapacity = 0;
-    hipEvent_t subsetCopied = nullptr;
+    StagingRing subsetStaging;
     DeviceArray<int> subsetTiles, subsetSegments;
     // launch order of the tiles: decreasing candidate pairs (cutoff lists) or particle pairs (all-pairs tiles), so
     // that the workgroups dispatched last are the lightest
@@ -2325,6 +2323,7 @@ void launch_long_range(ixh_pair* p, hipStream_t stream) {
                                                 p->lrcPerm[1].ptr, (size_t) n, 0, 64, stream));
         std::swap(p->lrcPerm[0].ptr, p->lrcPerm[1].ptr);   // both hold n entries
         std::swap(p->lrcPerm[0].count, p->lrcPerm[1].count);
+        std::swap(p->lrcPerm[0].bytes, p->lrcPerm[1].bytes);
     }
     hipLaunchKernelGGL(k_lrc_bounds, dim3(grid), dim3(256), 0, stream, p->rawDisp.ptr, flags, p->lrcPerm[0].ptr, n, p->lrcFirst.ptr);
     IXH_HIP_CHECK(hipGetLastError());
@@ -3051,18 +3050,8 @@ void execute_pair(ixh_pair* p, const ixh_buffers* b, int terms, int flags, doubl
             needed += p->segTiles[g+1]-p->segTiles[g];
         }
         needed += numSelected;
-        if (p->subsetCopied == nullptr)
-            IXH_HIP_CHECK(hipEventCreateWithFlags(&p->subsetCopied, hipEventDisableTiming));
-        else
-            IXH_HIP_CHECK(hipEventSynchronize(p->subsetCopied));   // the staging of the previous call has been read
-        if (needed > p->subsetCapacity) {
-            if (p->subsetStaging != nullptr)
-                IXH_HIP_CHECK(hipHostFree(p->subsetStaging));
-            p->subsetStaging = nullptr;
-            p->subsetCapacity = std::max<size_t>(needed, 1024);
-            IXH_HIP_CHECK(hipHostMalloc((void**) &p->subsetStaging, sizeof(int)*p->subsetCapacity, hipHostMallocDefault));
-        }
-        int* tileList = p->subsetStaging;
+        int slot = 0;
+        int* tileList = p->subsetStaging.acquire(needed, slot);
         int count = 0;
         for (int k = 0; k < numSelected; k++)
             for (int t = p->segTiles[segments[k]]; t < p->segTiles[segments[k]+1]; t++)
@@ -3075,7 +3064,7 @@ void execute_pair(ixh_pair* p, const ixh_buffers* b, int terms, int flags, doubl
         if (numSelected > 0)
             IXH_HIP_CHECK(hipMemcpyAsync(p->subsetSegments.ptr, tileList+count, sizeof(int)*numSelected, hipMemcpyHostToDevice,
                                          stream));
-        IXH_HIP_CHECK(hipEventRecord(p->subsetCopied, stream));
+        p->subsetStaging.recorded(slot, stream);
         subsetTileCount = count;
     }
     const ListCheck none = {};
@@ -3234,8 +3223,6 @@ int ixh_pair_destroy(ixh_pair* p) {
         (void) hipDeviceSynchronize();
         if (p->hostFlag != nullptr)
             (void) hipHostFree(p->hostFlag);
-        if (p->subsetCopied != nullptr)
-            (void) hipEventDestroy(p->subsetCopied);
         if (p->flagCopied != nullptr)
             (void) hipEventDestroy(p->flagCopied);
         if (p->updateDone != nullptr)
@@ -3245,8 +3232,6 @@ int ixh_pair_destroy(ixh_pair* p) {
         for (hipEvent_t e : p->frameDone)
             if (e != nullptr)
                 (void) hipEventDestroy(e);
-        if (p->subsetStaging != nullptr)
-            (void) hipHostFree(p->subsetStaging);
         if (p->lambdaHost != nullptr)
             (void) hipHostFree(p->lambdaHost);
         delete p;
@@ -3357,6 +3342,119 @@ int ixh_pair_long_range_energy(ixh_pair* p, const double* box, double lambda, do
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Energies at several lambda values, asynchronously on the buffers' stream (the work of ixh_pair_lambda_energies up
+// to the copies into the handle's pinned lambdaHost; executeFlags may hold IXH_DEFER_LIST_CHECK).  The caller's
+// device must be current, terms must be nonzero and the handle nonempty.
+void lambda_launch(ixh_pair* p, const ixh_buffers* b, int terms, const double* lambdas, int count, int executeFlags) {
+    hipStream_t stream = (hipStream_t) b->stream;
+    // 1. the handle's own evaluation at the buffers' lambda, into scratch forces and energy
+    const long long forceWords = 3LL*b->force_stride;
+    p->lambdaForce.reserve(forceWords);
+    p->lambdaEnergy.reserve(1);
+    hipLaunchKernelGGL(k_zero_fixed, dim3(grid_for(forceWords, 256)), dim3(256), 0, stream, p->lambdaForce.ptr, forceWords,
+                       p->lambdaEnergy.ptr);
+    IXH_HIP_CHECK(hipGetLastError());
+    ixh_buffers own = *b;
+    own.force = p->lambdaForce.ptr;
+    own.energy = p->lambdaEnergy.ptr;
+    execute_pair(p, &own, terms, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY | (executeFlags & IXH_DEFER_LIST_CHECK), nullptr, nullptr, -1,
+                 nullptr);
+    // 2. the pairs of the alchemical particles at every lambda and at the buffers' own
+    const int values = count+1;
+    if (p->anyAlchemical) {
+        LambdaArgs a;
+        a.posq = b->posq;
+        a.correction = (const float4*) b->posq_correction;
+        a.precision = b->precision;
+        a.n = p->n;
+        a.params = p->lambdaParams.ptr;
+        a.flags = p->flags.ptr;
+        a.atomIndex = b->atom_index;
+        a.alchList = p->alchList.ptr;
+        a.exclusionOffsets = p->exclusionOffsets.ptr;
+        a.exclusionPartners = p->exclusionPartners.ptr;
+        a.method = p->method;
+        a.box = make_box(b->box);
+        a.cutoff = p->cutoff;
+        a.settings = p->settings;
+        a.useSwitch = (p->method != IXH_NO_CUTOFF && p->useSwitch);
+        a.switchDistance = p->switchDistance;
+        a.invSwitchWidth = (a.useSwitch ? 1.0/(p->cutoff-p->switchDistance) : 0.0);
+        a.softcoreR6 = std::pow(p->softcoreRadius, 6);
+        a.count = values;
+        for (int k = 0; k < count; k++)
+            a.lambdas[k] = lambdas[k];
+        a.lambdas[count] = b->lambda;
+        const int chunks = (p->n+LAMBDA_CHUNK-1)/LAMBDA_CHUNK;
+        const int rows = p->numAlchemical*chunks;
+        p->lambdaOut.reserve((size_t) rows*values+values);
+        double* sums = p->lambdaOut.ptr+(size_t) rows*values;
+        if (rows > 0) {
+            switch (terms) {
+#define IXH_LAMBDA_CASE(T) case T: \
+                if (p->precision == IXH_DOUBLE) \
+                    hipLaunchKernelGGL((k_lambda_energies<double, T>), dim3(rows), dim3(64), 0, stream, a, chunks, p->lambdaOut.ptr); \
+                else \
+                    hipLaunchKernelGGL((k_lambda_energies<float, T>), dim3(rows), dim3(64), 0, stream, a, chunks, p->lambdaOut.ptr); \
+                break;
+                IXH_LAMBDA_CASE(1) IXH_LAMBDA_CASE(2) IXH_LAMBDA_CASE(3) IXH_LAMBDA_CASE(4) IXH_LAMBDA_CASE(5) IXH_LAMBDA_CASE(6)
+                IXH_LAMBDA_CASE(7)
+#undef IXH_LAMBDA_CASE
+            }
+            IXH_HIP_CHECK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_lambda_columns, dim3(values), dim3(64), 0, stream, p->lambdaOut.ptr, rows, values, sums);
+        IXH_HIP_CHECK(hipGetLastError());
+    }
+    // 3. E(lambda_k) = E(own lambda) - E_alchemical(own lambda) + E_alchemical(lambda_k)
+    const size_t words = 1+(p->anyAlchemical ? (size_t) values : 0);
+    if (words > p->lambdaHostCount) {
+        if (p->lambdaHost != nullptr)
+            IXH_HIP_CHECK(hipHostFree(p->lambdaHost));
+        p->lambdaHost = nullptr;
+        IXH_HIP_CHECK(hipHostMalloc((void**) &p->lambdaHost, sizeof(double)*words, hipHostMallocDefault));
+        p->lambdaHostCount = words;
+    }
+    IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost, p->lambdaEnergy.ptr, sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (words > 1) {
+        const size_t rows = (size_t) p->numAlchemical*((p->n+LAMBDA_CHUNK-1)/LAMBDA_CHUNK);
+        IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost+1, p->lambdaOut.ptr+rows*values, sizeof(double)*values,
+                                     hipMemcpyDeviceToHost, stream));
+    }
+}
+
+// E(lambda_k) from the copies of lambda_launch, once they have arrived.
+void lambda_collect(const ixh_pair* p, int count, double* energies) {
+    const bool windows = p->anyAlchemical;
+    const double ownEnergy = p->lambdaHost[0];
+    for (int k = 0; k < count; k++)
+        energies[k] = ownEnergy+(windows ? p->lambdaHost[1+k]-p->lambdaHost[1+count] : 0.0);
+}
+
+}  // namespace
+
+namespace ixh {
+
+void pair_lambda_launch(ixh_pair* pair, const ixh_buffers* buffers, const double* lambdas, int count, int flags) {
+    if (count < 1 || count > MAX_LAMBDA_WINDOWS)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_lambda_energies: between 1 and 16 lambda values");
+    if (pair->n == 0)
+        throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_pair_lambda_energies: no particles");
+    lambda_launch(pair, buffers, pair->terms, lambdas, count, flags);
+}
+
+void pair_lambda_collect(const ixh_pair* pair, int count, double* energies) {
+    lambda_collect(pair, count, energies);
+}
+
+}  // namespace ixh
+
+extern "C" {
+
 int ixh_pair_lambda_energies(ixh_pair* p, const ixh_buffers* b, int terms, const double* lambdas, int count, double* energies) {
     return guarded([&] {
         if (p == nullptr || b == nullptr || lambdas == nullptr || energies == nullptr)
@@ -3370,86 +3468,9 @@ int ixh_pair_lambda_energies(ixh_pair* p, const ixh_buffers* b, int terms, const
         if (terms == 0 || p->n == 0)
             return;
         DeviceGuard guard(p->device);
-        hipStream_t stream = (hipStream_t) b->stream;
-        // 1. the handle's own evaluation at the buffers' lambda, into scratch forces and energy
-        const long long forceWords = 3LL*b->force_stride;
-        p->lambdaForce.reserve(forceWords);
-        p->lambdaEnergy.reserve(1);
-        hipLaunchKernelGGL(k_zero_fixed, dim3(grid_for(forceWords, 256)), dim3(256), 0, stream, p->lambdaForce.ptr, forceWords,
-                           p->lambdaEnergy.ptr);
-        IXH_HIP_CHECK(hipGetLastError());
-        ixh_buffers own = *b;
-        own.force = p->lambdaForce.ptr;
-        own.energy = p->lambdaEnergy.ptr;
-        const int status = ixh_pair_execute(p, &own, terms, IXH_INCLUDE_FORCES | IXH_INCLUDE_ENERGY);
-        if (status != IXH_OK)
-            throw Error(status, ixh_last_error());
-        // 2. the pairs of the alchemical particles at every lambda and at the buffers' own
-        const int values = count+1;
-        if (p->anyAlchemical) {
-            LambdaArgs a;
-            a.posq = b->posq;
-            a.correction = (const float4*) b->posq_correction;
-            a.precision = b->precision;
-            a.n = p->n;
-            a.params = p->lambdaParams.ptr;
-            a.flags = p->flags.ptr;
-            a.atomIndex = b->atom_index;
-            a.alchList = p->alchList.ptr;
-            a.exclusionOffsets = p->exclusionOffsets.ptr;
-            a.exclusionPartners = p->exclusionPartners.ptr;
-            a.method = p->method;
-            a.box = make_box(b->box);
-            a.cutoff = p->cutoff;
-            a.settings = p->settings;
-            a.useSwitch = (p->method != IXH_NO_CUTOFF && p->useSwitch);
-            a.switchDistance = p->switchDistance;
-            a.invSwitchWidth = (a.useSwitch ? 1.0/(p->cutoff-p->switchDistance) : 0.0);
-            a.softcoreR6 = std::pow(p->softcoreRadius, 6);
-            a.count = values;
-            for (int k = 0; k < count; k++)
-                a.lambdas[k] = lambdas[k];
-            a.lambdas[count] = b->lambda;
-            const int chunks = (p->n+LAMBDA_CHUNK-1)/LAMBDA_CHUNK;
-            const int rows = p->numAlchemical*chunks;
-            p->lambdaOut.reserve((size_t) rows*values+values);
-            double* sums = p->lambdaOut.ptr+(size_t) rows*values;
-            if (rows > 0) {
-                switch (terms) {
-#define IXH_LAMBDA_CASE(T) case T: \
-                    if (p->precision == IXH_DOUBLE) \
-                        hipLaunchKernelGGL((k_lambda_energies<double, T>), dim3(rows), dim3(64), 0, stream, a, chunks, p->lambdaOut.ptr); \
-                    else \
-                        hipLaunchKernelGGL((k_lambda_energies<float, T>), dim3(rows), dim3(64), 0, stream, a, chunks, p->lambdaOut.ptr); \
-                    break;
-                    IXH_LAMBDA_CASE(1) IXH_LAMBDA_CASE(2) IXH_LAMBDA_CASE(3) IXH_LAMBDA_CASE(4) IXH_LAMBDA_CASE(5) IXH_LAMBDA_CASE(6)
-                    IXH_LAMBDA_CASE(7)
-#undef IXH_LAMBDA_CASE
-                }
-                IXH_HIP_CHECK(hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_lambda_columns, dim3(values), dim3(64), 0, stream, p->lambdaOut.ptr, rows, values, sums);
-            IXH_HIP_CHECK(hipGetLastError());
-        }
-        // 3. E(lambda_k) = E(own lambda) - E_alchemical(own lambda) + E_alchemical(lambda_k)
-        const size_t words = 1+(p->anyAlchemical ? (size_t) values : 0);
-        if (words > p->lambdaHostCount) {
-            if (p->lambdaHost != nullptr)
-                IXH_HIP_CHECK(hipHostFree(p->lambdaHost));
-            p->lambdaHost = nullptr;
-            IXH_HIP_CHECK(hipHostMalloc((void**) &p->lambdaHost, sizeof(double)*words, hipHostMallocDefault));
-            p->lambdaHostCount = words;
-        }
-        IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost, p->lambdaEnergy.ptr, sizeof(double), hipMemcpyDeviceToHost, stream));
-        if (words > 1) {
-            const size_t rows = (size_t) p->numAlchemical*((p->n+LAMBDA_CHUNK-1)/LAMBDA_CHUNK);
-            IXH_HIP_CHECK(hipMemcpyAsync(p->lambdaHost+1, p->lambdaOut.ptr+rows*values, sizeof(double)*values,
-                                         hipMemcpyDeviceToHost, stream));
-        }
-        IXH_HIP_CHECK(hipStreamSynchronize(stream));
-        const double ownEnergy = p->lambdaHost[0];
-        for (int k = 0; k < count; k++)
-            energies[k] = ownEnergy+(words > 1 ? p->lambdaHost[1+k]-p->lambdaHost[1+count] : 0.0);
+        lambda_launch(p, b, terms, lambdas, count, 0);
+        IXH_HIP_CHECK(hipStreamSynchronize((hipStream_t) b->stream));
+        lambda_collect(p, count, energies);
     });
 }
 

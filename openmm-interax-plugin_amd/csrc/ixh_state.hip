@@ -5,6 +5,7 @@
 #include "ixh_common.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -12,10 +13,12 @@
 namespace ixh {
 
 static thread_local std::string lastError;
-static bool profiling = false;
+// set by any thread, read by the server's thread among others
+static std::atomic<bool> profiling{false};
+std::atomic<long long> deviceBytes{0};
 
 bool profiling_enabled() {
-    return profiling;
+    return profiling.load(std::memory_order_relaxed);
 }
 
 void set_last_error(const std::string& message) {
@@ -72,7 +75,7 @@ int ixh_version(int* major, int* minor) {
 }
 
 int ixh_set_profiling(int enable) {
-    return guarded([&] { ixh::profiling = (enable != 0); });
+    return guarded([&] { ixh::profiling.store(enable != 0, std::memory_order_relaxed); });
 }
 
 int ixh_device_count(int* count) {

@@ -1128,9 +1128,7 @@ struct ixh_threebody {
     DeviceArray<int2> itemShares;
     DeviceArray<int> itemOrder;   // all items, most pairs first (the launch order of a full evaluation)
     // subsets of segments: site list and segment list of the last call
-    int* subsetStaging = nullptr;
-    size_t subsetCapacity = 0;
-    hipEvent_t subsetCopied = nullptr;
+    StagingRing subsetStaging;
     DeviceArray<int> subsetSites, subsetSegments;
     DeviceArray<double> paramsDev, sites, siteEnergy;
     DeviceArray<double4> nbrDelta;
@@ -1593,12 +1591,6 @@ int ixh_threebody_destroy(ixh_threebody* h) {
     return guarded([&] {
         if (h == nullptr)
             return;
-        if (h->subsetCopied != nullptr) {
-            (void) hipEventSynchronize(h->subsetCopied);
-            (void) hipEventDestroy(h->subsetCopied);
-        }
-        if (h->subsetStaging != nullptr)
-            (void) hipHostFree(h->subsetStaging);
         if (h->hostTotals != nullptr)
             (void) hipHostFree(h->hostTotals);
         if (h->hostFlag != nullptr)
@@ -1713,23 +1705,14 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                         throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody_execute_subset: segment index out of range");
                     needed += h->segOffsets[g+1]-h->segOffsets[g]+h->segItems[g+1]-h->segItems[g];
                 }
-                if (h->subsetCopied == nullptr)
-                    IXH_HIP_CHECK(hipEventCreateWithFlags(&h->subsetCopied, hipEventDisableTiming));
-                else
-                    IXH_HIP_CHECK(hipEventSynchronize(h->subsetCopied));
-                if (needed > h->subsetCapacity) {
-                    if (h->subsetStaging != nullptr)
-                        IXH_HIP_CHECK(hipHostFree(h->subsetStaging));
-                    h->subsetStaging = nullptr;
-                    h->subsetCapacity = std::max<size_t>(needed, 1024);
-                    IXH_HIP_CHECK(hipHostMalloc((void**) &h->subsetStaging, sizeof(int)*h->subsetCapacity, hipHostMallocDefault));
-                }
+                int slot = 0;
+                int* staging = h->subsetStaging.acquire(needed, slot);
                 int count = 0;
                 for (int k = 0; k < numSelected; k++)
                     for (int i = h->segOffsets[segments[k]]; i < h->segOffsets[segments[k]+1]; i++)
-                        h->subsetStaging[count++] = i;
+                        staging[count++] = i;
                 int itemCount = 0;
-                int* itemList = h->subsetStaging+count;
+                int* itemList = staging+count;
                 for (int k = 0; k < numSelected; k++)
                     for (int w = h->segItems[segments[k]]; w < h->segItems[segments[k]+1]; w++)
                         itemList[itemCount++] = w;
@@ -1738,13 +1721,13 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
                 h->subsetItems.reserve(std::max(itemCount, 1));
                 h->subsetSegments.reserve(std::max(numSelected, 1));
                 if (count > 0)
-                    IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSites.ptr, h->subsetStaging, sizeof(int)*count, hipMemcpyHostToDevice, stream));
+                    IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSites.ptr, staging, sizeof(int)*count, hipMemcpyHostToDevice, stream));
                 if (itemCount > 0)
                     IXH_HIP_CHECK(hipMemcpyAsync(h->subsetItems.ptr, itemList, sizeof(int)*itemCount, hipMemcpyHostToDevice, stream));
                 if (numSelected > 0)
                     IXH_HIP_CHECK(hipMemcpyAsync(h->subsetSegments.ptr, itemList+itemCount, sizeof(int)*numSelected,
                                                  hipMemcpyHostToDevice, stream));
-                IXH_HIP_CHECK(hipEventRecord(h->subsetCopied, stream));
+                h->subsetStaging.recorded(slot, stream);
                 subsetSites = count;
                 subsetItemCount = itemCount;
             }

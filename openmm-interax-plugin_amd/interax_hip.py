@@ -159,12 +159,17 @@ SIGNATURES = {
     "ixh_pair_list_was_valid": [ctypes.c_void_p, _ip],
     "ixh_server_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)],
     "ixh_server_add_cluster": [ctypes.c_void_p, ctypes.POINTER(PairDesc), ctypes.POINTER(ThreeBodyDesc), _ip],
-    "ixh_server_add_periodic": [ctypes.c_void_p, ctypes.POINTER(PairDesc), _dp, _ip],
+    "ixh_server_add_periodic": [ctypes.c_void_p, ctypes.POINTER(PairDesc), _dp, _dp, _ip],
     "ixh_server_start": [ctypes.c_void_p],
     "ixh_server_submit": [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.POINTER(ctypes.c_longlong)],
+    "ixh_server_submit_windows": [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)],
     "ixh_server_wait": [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _dp],
+    "ixh_server_wait_windows": [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp],
+    "ixh_server_memory": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                          ctypes.POINTER(ctypes.c_longlong)],
     "ixh_server_statistics": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
-    "ixh_server_replay": [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _dp, ctypes.POINTER(ctypes.c_longlong), _dp, _dp, _dp, _dp],
+    "ixh_server_replay": [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _dp, ctypes.POINTER(ctypes.c_longlong), _dp, _ip, _dp, _dp, _dp,
+                          _dp],
     "ixh_server_destroy": [ctypes.c_void_p],
 }
 
@@ -1229,9 +1234,11 @@ class Server:
         self.precision = PRECISIONS[precision]
         self.sizes: List[int] = []
         self._keep = []
+        self._windows = {}
 
-    def add_system(self, system: "System") -> int:
-        """Register a System; returns its entry index."""
+    def add_system(self, system: "System", positions: Optional[np.ndarray] = None) -> int:
+        """Register a System (also while the server runs); returns its entry index.  The positions of a periodic
+        System, if given, are evaluated once at registration (neighbour lists built, buffers sized)."""
         pair_forces = [f for f in system.getForces() if isinstance(f, InteraxPairForce)]
         tbs = [f for f in system.getForces() if isinstance(f, InteraxThreeBodyForce)]
         if not pair_forces:
@@ -1255,7 +1262,13 @@ class Server:
                 raise InteraxError("Server.add_system: periodic Systems with a three-body force are not served")
             box = np.ascontiguousarray(system.getDefaultPeriodicBoxVectors().reshape(-1))
             self._keep.append(box)
-            _check(_lib.ixh_server_add_periodic(self.handle, ctypes.byref(desc), _dptr(box), ctypes.byref(entry)))
+            pos = None
+            if positions is not None:
+                pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
+                if pos.size != 3*system.getNumParticles():
+                    raise InteraxError("Server.add_system: wrong number of positions")
+            _check(_lib.ixh_server_add_periodic(self.handle, ctypes.byref(desc), _dptr(box), _dptr(pos) if pos is not None else None,
+                                                ctypes.byref(entry)))
         self.sizes.append(system.getNumParticles())
         return entry.value
 
@@ -1270,6 +1283,29 @@ class Server:
         _check(_lib.ixh_server_submit(self.handle, entry, _dptr(pos), lam, ctypes.byref(ticket)))
         return ticket.value
 
+    def submit_windows(self, entry: int, positions: np.ndarray, lambdas) -> int:
+        """The energies of one configuration of a periodic System at up to 16 lambda values (fused evaluation)."""
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
+        if pos.size != 3*self.sizes[entry]:
+            raise InteraxError("Server.submit_windows: wrong number of particles")
+        lam = np.ascontiguousarray(lambdas, dtype=np.float64)
+        ticket = ctypes.c_longlong()
+        _check(_lib.ixh_server_submit_windows(self.handle, entry, _dptr(pos), _dptr(lam), len(lam), ctypes.byref(ticket)))
+        self._windows[ticket.value] = len(lam)
+        return ticket.value
+
+    def wait_windows(self, ticket: int):
+        """(energies at the requested lambda values, latency in ms) of a window request."""
+        energies = np.zeros(self._windows.pop(ticket))
+        ms = ctypes.c_double()
+        _check(_lib.ixh_server_wait_windows(self.handle, ticket, _dptr(energies), ctypes.byref(ms)))
+        return energies, ms.value
+
+    def memory(self) -> dict:
+        resident, in_flight, peak = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        _check(_lib.ixh_server_memory(self.handle, ctypes.byref(resident), ctypes.byref(in_flight), ctypes.byref(peak)))
+        return {"resident_bytes": resident.value, "in_flight_bytes": in_flight.value, "peak_in_flight_bytes": peak.value}
+
     def wait(self, ticket: int, entry: int):
         """(energy, forces, latency in ms) of a submitted request."""
         e = ctypes.c_double()
@@ -1283,9 +1319,10 @@ class Server:
         _check(_lib.ixh_server_statistics(self.handle, ctypes.byref(b), ctypes.byref(r)))
         return b.value, r.value
 
-    def replay(self, arrival_s: np.ndarray, entries: np.ndarray, positions: List[np.ndarray], lambdas=None):
+    def replay(self, arrival_s: np.ndarray, entries: np.ndarray, positions: List[np.ndarray], lambdas=None, windows=None):
         """Submit requests at the given arrival times (seconds from now) from a native thread; returns the latency (ms)
-        and energy of every request and the time to the last completion (s)."""
+        and energy of every request and the time to the last completion (s).  windows, optional: per request None
+        (one lambda, forces) or the lambda values of a window request (the energy returned is the first window's)."""
         n = len(entries)
         arrival = np.ascontiguousarray(arrival_s, dtype=np.float64)
         ent = np.ascontiguousarray(entries, dtype=np.int32)
@@ -1293,11 +1330,20 @@ class Server:
         flat = [np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(-1)) for p in positions]
         offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum([f.size for f in flat])[:-1]]), dtype=np.int64)
         allpos = np.ascontiguousarray(np.concatenate(flat))
+        counts = values = None
+        if windows is not None:
+            counts = np.ascontiguousarray([0 if w is None else len(w) for w in windows], dtype=np.int32)
+            values = np.ascontiguousarray(np.concatenate([np.zeros(0)]+[np.asarray(w, dtype=np.float64) for w in windows
+                                                                          if w is not None]), dtype=np.float64)
+            if values.size == 0:
+                values = np.zeros(1)
         latency = np.zeros(n)
         energies = np.zeros(n)
         makespan = ctypes.c_double()
         _check(_lib.ixh_server_replay(self.handle, n, _dptr(arrival), _iptr(ent), _dptr(lam),
                                       offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _dptr(allpos),
+                                      _iptr(counts) if counts is not None else None,
+                                      _dptr(values) if values is not None else None,
                                       _dptr(latency), _dptr(energies), ctypes.byref(makespan)))
         return latency, energies, makespan.value
 
