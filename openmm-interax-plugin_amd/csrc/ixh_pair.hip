@@ -704,13 +704,33 @@ __device__ __forceinline__ int lds_flags(int flags, int orig) {
     return (flags < 0 ? -1 : (flags & ALCHEMICAL_BIT) | orig);
 }
 
+// The fixed table form (k_pair_queue with FIXED): in place of its class word, a particle in LDS carries the byte
+// offsets, from the start of the tables, of its part of a pair's rows -- the dispersion offset in the low 16 bits, the
+// overlap offset in the high 16 -- in the role it has in the tile (i or j), so that the offsets of a pair's rows are
+// the sum of the two particles' words.  The rows of pair (i, j) are dispersion class pair (d_i, d_j) and overlap
+// class pair (x_j, x_i), the orders pair_energy_table uses when the index order does not matter.
+template <int TERMS>
+__device__ __forceinline__ int fixed_role_word(const TileArgs<float>& a, int cls, bool iRole) {
+    constexpr bool EXCH = (TERMS & IXH_TERM_EXCHANGE) != 0;
+    constexpr int PAIR_BYTES = 4*OVERLAP_PAIR_FLOATS;
+    const int d = cls & 0xff;
+    const int x = (EXCH ? cls >> 8 : cls >> 16) & 0xff;
+    if (iRole)
+        return d*a.numDisp*16 + ((4*(EXCH ? a.exchOffset : a.srOffset)+x*PAIR_BYTES) << 16);
+    return d*16 + ((x*(EXCH ? a.numExch : a.numSr)*PAIR_BYTES) << 16);
+}
+
 // Stage the parameters of a slot whose position is already in LDS; orig is its System index.  Returns the slot's
-// flags (particle index in the current order, for the force buffer).
-template <typename Real, bool TABLES>
+// flags (particle index in the current order, for the force buffer).  FIXED: the i-side role word of the fixed
+// table form in place of the class word.
+template <typename Real, bool TABLES, int TERMS = 0, bool FIXED = false>
 __device__ __forceinline__ int stage_particle(SideLds<Real, TABLES>& s, int lane, const TileArgs<Real>& a, int slot, int orig) {
     if constexpr (TABLES) {
         s.wd[lane] = a.pA[slot].x;
-        s.pw[lane] = __int_as_float(a.cls[slot]);
+        if constexpr (FIXED)
+            s.pw[lane] = __int_as_float(fixed_role_word<TERMS>(a, a.cls[slot], true));
+        else
+            s.pw[lane] = __int_as_float(a.cls[slot]);
     }
     else {
         s.A[lane] = a.pA[slot];
@@ -783,7 +803,9 @@ __device__ __forceinline__ void evaluate_pair(const TileArgs<Real>& a, const Pai
     }
     Real dEdR;
     const bool iFirst = origI < (flagJ & INDEX_MASK);
-    if constexpr (std::is_same<Particle, TableParticle>::value)
+    if constexpr (std::is_same<Particle, FixedPair>::value)
+        pair_energy_fixed<TERMS>(rEff, invREff, P, tb.beta2, e, dEdR);
+    else if constexpr (std::is_same<Particle, TableParticle>::value)
         pair_energy_table<TERMS>(rEff, invREff, P, Q, iFirst, tb, e, dEdR);
     else
         pair_energy<Real, TERMS>(rEff, invREff, P, Q, iFirst, a.settings, e, dEdR);
@@ -981,7 +1003,7 @@ struct QueueLds {
     unsigned short queue[QUEUE];
 };
 
-template <typename Real, int TERMS, bool TABLES, bool ALCH>
+template <typename Real, int TERMS, bool TABLES, bool ALCH, bool FIXED>
 __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const PairTables& tb, QueueLds<Real, TABLES>& q, int head,
                                                int valid, int lane, bool pairImage, double& energy) {
     typedef typename Vec4T<Real>::type R4;
@@ -996,18 +1018,30 @@ __device__ __forceinline__ void evaluate_batch(const TileArgs<Real>& a, const Pa
         min_image<Real>(dx, dy, dz, a.boxR[0], a.boxR[1], a.boxR[2], a.boxR[3], a.boxR[4], a.boxR[5], a.boxR[6], a.boxR[7],
                         a.boxR[8]);
     const Real r2 = dx*dx+dy*dy+dz*dz;
-    Particle P, Q;
-    fetch_particle(q.i, il, P);
-    fetch_particle(q.j, jl, Q);
     // the flags hold the alchemical bits and the System indices, which only an InteraxOrder2 overlap needs (for the
     // Exact scheme the overlap of a pair is the same either way round)
     int flagI = 0, flagJ = 0;
-    if (ALCH || a.ordered) {
+    if (ALCH || (!FIXED && a.ordered)) {
         flagI = q.i.flags[il];
         flagJ = q.j.flags[jl];
     }
     Real en, fs;
-    evaluate_pair<Real, TERMS, ALCH>(a, tb, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
+    if constexpr (FIXED) {
+        // the pair's rows from the sum of the two role words (fixed_role_word)
+        const int word = __float_as_int(q.i.pw[il])+__float_as_int(q.j.pw[jl]);
+        const char* base = (const char*) tb.disp;
+        FixedPair P;
+        P.disp = *(const float4*) (base+(word & 0xffff));
+        P.row = (const OverlapRow*) (base+(word >> 16));
+        P.wsum = q.i.wd[il]+q.j.wd[jl];
+        evaluate_pair<Real, TERMS, ALCH>(a, tb, r2, P, P, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
+    }
+    else {
+        Particle P, Q;
+        fetch_particle(q.i, il, P);
+        fetch_particle(q.j, jl, Q);
+        evaluate_pair<Real, TERMS, ALCH>(a, tb, r2, P, Q, ((flagI >> 30) & 1) != 0, flagJ, flagI & INDEX_MASK, en, fs);
+    }
     energy += (double) en;
     const long long fx = to_fixed(dx*fs), fy = to_fixed(dy*fs), fz = to_fixed(dz*fs);
     atomicAdd(&q.fI[0][il], (unsigned long long) fx);
@@ -1051,7 +1085,7 @@ __device__ __forceinline__ void fetch_partner(const TileArgs<Real>& a, int slot,
 }
 
 // Write a prefetched partner into LDS, moved into the frame of block I (origin cI).
-template <typename Real, bool TABLES>
+template <typename Real, bool TABLES, int TERMS = 0, bool FIXED = false>
 __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const PartnerFetch<Real, TABLES>& f, int I, double cIx, double cIy,
                                               double cIz, int lane, SideLds<Real, TABLES>& s) {
     typedef typename Vec4T<Real>::type R4;
@@ -1072,7 +1106,10 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
         pj.z += Real(sz);
     }
     if constexpr (TABLES) {
-        pj.w = __int_as_float(f.cls);
+        if constexpr (FIXED)
+            pj.w = __int_as_float(fixed_role_word<TERMS>(a, f.cls, false));
+        else
+            pj.w = __int_as_float(f.cls);
         s.wd[lane] = f.A.x;
     }
     else {
@@ -1089,7 +1126,8 @@ __device__ __forceinline__ void store_partner(const TileArgs<Real>& a, const Par
 // wavefronts (measured on the 98 304-atom box: 1 wave, 185.5 us, but four times the partial sums to reduce, 0.209 ms
 // per step; 2 waves, 207 us; 4 waves, 186-190 us and 0.205 ms per step).
 constexpr int QUEUE_GROUP_WAVES = 4;
-template <typename Real, int TERMS, bool TABLES, bool ALCH>
+// FIXED: the fixed table form (pair_energy_fixed), for tables that allow it.
+template <typename Real, int TERMS, bool TABLES, bool ALCH, bool FIXED = false>
 __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_queue(TileArgs<Real> a) {
     typedef typename Vec4T<Real>::type R4;
     __shared__ QueueLds<Real, TABLES> lds[QUEUE_GROUP_WAVES];
@@ -1141,10 +1179,10 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
             cIy = a.centers[3*I+1];
             cIz = a.centers[3*I+2];
             store_pos(q.i, lane, pi);
-            stage_particle<Real, TABLES>(q.i, lane, a, slot, decode_index(pi.w));
+            stage_particle<Real, TABLES, TERMS, FIXED>(q.i, lane, a, slot, decode_index(pi.w));
             q.fI[0][lane] = q.fI[1][lane] = q.fI[2][lane] = 0;
         }
-        store_partner<Real, TABLES>(a, f, I, cIx, cIy, cIz, lane, q.j);
+        store_partner<Real, TABLES, TERMS, FIXED>(a, f, I, cIx, cIy, cIz, lane, q.j);
         q.fJ[0][lane] = q.fJ[1][lane] = q.fJ[2][lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // The tile's candidate list, 256 entries (4 per lane) per step with the next step's entries in flight (each
@@ -1188,7 +1226,7 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
             }
             while (count >= BLOCK) {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, BLOCK, lane, pairImage, energy);
+                evaluate_batch<Real, TERMS, TABLES, ALCH, FIXED>(a, tb, q, head, BLOCK, lane, pairImage, energy);
                 pairs += BLOCK;
                 head = (head+BLOCK) & (QUEUE-1);
                 count -= BLOCK;
@@ -1198,7 +1236,7 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
         }
         if (count > 0) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            evaluate_batch<Real, TERMS, TABLES, ALCH>(a, tb, q, head, count, lane, pairImage, energy);
+            evaluate_batch<Real, TERMS, TABLES, ALCH, FIXED>(a, tb, q, head, count, lane, pairImage, energy);
             pairs += count;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1900,6 +1938,7 @@ struct ixh_pair {
     int numDisp = 1, numExch = 1, numSr = 1, exchOffset = 0, srOffset = 0, tableFloats = 0;
     int highOrder = 0;
     int singleBranch = 0;   // no width pair of the tables mixes the Taylor and closed-form branches
+    bool fixedForm = false;   // the tables allow the fixed table form of the cutoff tile kernel (pair_energy_fixed)
     DeviceArray<int> classes, sortedClasses;
     DeviceArray<float> tables;
     DeviceArray<unsigned> keys, keysOut;
@@ -2146,6 +2185,7 @@ bool form_classes(ixh_pair* p, const double* disp, const double* exch, const dou
 // m1, m2, m3, alpha, width; exchange amplitude, width; short range K, L, width).  Also sets highOrder and singleBranch.
 void form_tables(ixh_pair* p, const double* rec, std::vector<float>& tables) {
     p->highOrder = 0;
+    bool degree3 = false, allDispersion = true;
     tables.assign(p->tableFloats, 0.0f);
     // dispersion rows: C6, C8, C10 combined as in InteraxPairTerms.h:271-279, and the gate alpha_i alpha_j != 0
     if (p->terms & IXH_TERM_DISPERSION)
@@ -2161,6 +2201,8 @@ void form_tables(ixh_pair* p, const double* rec, std::vector<float>& tables) {
                 row[1] = (float) (1.5*pre*(m1i*m2j+m2i*m1j));
                 row[2] = (float) (pre*(2*(m1i*m3j+m3i*m1j)+4.2*m2i*m2j));
                 row[3] = (ai*aj != 0 ? 1.0f : 0.0f);
+                if (ai*aj == 0)
+                    allDispersion = false;
             }
     // overlap rows of a class pair, with the amplitude product (exchange) and K, L averages (short range); a class
     // key is (width, amplitude, K, L)
@@ -2175,12 +2217,21 @@ void form_tables(ixh_pair* p, const double* rec, std::vector<float>& tables) {
                 const auto ki = key(firstRep+ci, exchange, shortRange), kj = key(firstRep+cj, exchange, shortRange);
                 OverlapRow w[2];
                 overlap_rows(std::get<0>(ki), std::get<0>(kj), scheme, w);
+                // the energy of the row's terms as S (a1 + a2 S): A_i A_j S + K S_t + L S_t^2 with S_t = norm S
+                const double wa = std::get<0>(ki), wb = std::get<0>(kj);
+                const double amplitude = std::get<1>(ki)*std::get<1>(kj);
+                const double kk = 0.5*(std::get<2>(ki)+std::get<2>(kj)), ll = 0.5*(std::get<3>(ki)+std::get<3>(kj));
+                const double norm = (wa == 0 || wb == 0 ? 0.0 : EIGHT_PI*(wa+wb)*(wa+wb)*(wa+wb));
                 for (OverlapRow& row : w) {
                     if (row.high != 0.0f)
                         p->highOrder = 1;
-                    row.amplitude = (float) (std::get<1>(ki)*std::get<1>(kj));
-                    row.kk = (float) (0.5*(std::get<2>(ki)+std::get<2>(kj)));
-                    row.ll = (float) (0.5*(std::get<3>(ki)+std::get<3>(kj)));
+                    if (row.p3 != 0.0f)
+                        degree3 = true;
+                    row.amplitude = (float) amplitude;
+                    row.kk = (float) kk;
+                    row.ll = (float) ll;
+                    row.a1 = (float) (amplitude+kk*norm);
+                    row.a2 = (float) (ll*norm*norm);
                 }
                 std::memcpy(&tables[offset+OVERLAP_PAIR_FLOATS*(ci*count+cj)], w, sizeof(w));
             }
@@ -2196,8 +2247,11 @@ void form_tables(ixh_pair* p, const double* rec, std::vector<float>& tables) {
     // rows of a width pair that takes one branch only are equal: the kernel then reads the first without selecting
     p->singleBranch = 1;
     for (int offset = p->exchOffset; offset < p->tableFloats; offset += OVERLAP_PAIR_FLOATS)
-        if (std::memcmp(&tables[offset], &tables[offset+OVERLAP_PAIR_FLOATS/2], sizeof(OverlapRow)) != 0)
+        if (std::memcmp(&tables[offset], &tables[offset+OVERLAP_ROW_FLOATS], sizeof(OverlapRow)) != 0)
             p->singleBranch = 0;
+    const bool bothOverlaps = (p->terms & IXH_TERM_EXCHANGE) && (p->terms & IXH_TERM_SHORT_RANGE);
+    p->fixedForm = p->singleBranch && !p->highOrder && !degree3 && (allDispersion || !(p->terms & IXH_TERM_DISPERSION)) &&
+                   (shared || !bothOverlaps);
 }
 
 // Classes, shared-overlap choice and type-table layout from host copies of the parameters (at creation, and when an
@@ -2772,6 +2826,20 @@ void launch_bound(Kernel kernel, dim3 grid, dim3 block, size_t shared, hipStream
         hipLaunchKernelGGL(kernel, grid, block, shared, stream, args...);
 }
 
+// The tile kernel of a cutoff launch: the fixed table form where the tables allow it (fixed: ixh_pair::fixedForm and
+// no index-order dependence), else the general form.
+template <typename Real, int T, bool TAB, bool ALCH>
+void launch_queue(const TileArgs<Real>& a, bool fixed, int blocks, size_t tableBytes, hipStream_t stream) {
+    const dim3 grid(blocks), group(64*QUEUE_GROUP_WAVES);
+    if constexpr (TAB) {
+        if (fixed) {
+            hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH, true>), grid, group, tableBytes, stream, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH>), grid, group, tableBytes, stream, a);
+}
+
 // Workgroups of a tile launch over numTiles tiles.
 int launch_workgroups(const ixh_pair* p, int numTiles) {
     if (p->method != IXH_NO_CUTOFF && p->numItems > 0 && numTiles == p->numTiles)
@@ -2857,6 +2925,11 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
                 ((p->terms & IXH_TERM_SHORT_RANGE) && !p->settings.short_range_scheme.about_smaller);
     a.beta2 = (float) (2*p->dampingScale);
     const size_t tableBytes = (tables ? sizeof(float)*p->tableFloats : 0);
+    // the fixed form's overlap rows hold the energy of the handle's overlap terms together: a launch takes it when it
+    // evaluates all of them (or none)
+    const int overlapTerms = IXH_TERM_EXCHANGE | IXH_TERM_SHORT_RANGE;
+    const bool fixed = (tables && p->fixedForm && !a.ordered &&
+                        ((terms & overlapTerms) == 0 || (terms & overlapTerms) == (p->terms & overlapTerms)));
     a.tileIndex = (p->ordered ? (const int*) p->tileOrder.ptr : (const int*) nullptr);
     a.items = (p->method != IXH_NO_CUTOFF && p->numItems > 0 && subsetTiles < 0 ? (const int2*) p->items.ptr : nullptr);
     if (subsetTiles >= 0) {
@@ -2893,7 +2966,7 @@ hipEvent_t launch_tiles(ixh_pair* p, const ixh_buffers* b, int terms, int addEne
         switch (terms) {
 #define IXH_LAUNCH(T, TAB, ALCH) \
             if (compact) \
-                hipLaunchKernelGGL((k_pair_queue<Real, T, TAB, ALCH>), dim3(blocks), dim3(64*QUEUE_GROUP_WAVES), tableBytes, stream, a); \
+                launch_queue<Real, T, TAB, ALCH>(a, fixed, blocks, tableBytes, stream); \
             else \
                 hipLaunchKernelGGL((k_pair_rotate<Real, T, TAB, ALCH>), dim3(blocks), dim3(256), tableBytes, stream, a);
 // the variants without alchemical steps are built for the FP32 type-table kernels only

@@ -178,8 +178,16 @@ struct TailPolynomial<Real, M, M> {
 
 // Tang-Toennies f_6, f_8, f_10 and df/dx (InteraxPairTerms.h:223-253).  Below x = (n+1)/2 the closed form
 // 1 - exp(-x) sum_{k<=n} x^k/k! cancels and f_n = exp(-x) sum_{k>n} x^k/k! is used, a sum of positive terms.
+// In two steps: damping_main (branch-free: the closed forms and the derivatives) and damping_tail (the series below
+// x = 5.5, the only divergent step), so that a lane's several pairs can share one block of straight-line code.
 template <typename Real>
-__device__ __forceinline__ void tang_toennies(Real x, Real f[3], Real df[3]) {
+struct Damping {
+    Real x, expX, t6, t7, t8, t9, t10;   // t_n = x^n/n!
+    Real f[3], df[3];                    // df_n = exp(-x) t_n
+};
+
+template <typename Real>
+__device__ __forceinline__ void damping_main(Real x, Damping<Real>& d) {
     const Real expX = fast_exp2(-x*Real(LOG2E));
     // s6 = sum_{k<=6} x^k/k! in Horner form; t_n = x^n/n!
     Real s6 = Real(1.0/720);
@@ -198,22 +206,45 @@ __device__ __forceinline__ void tang_toennies(Real x, Real f[3], Real df[3]) {
     const Real t10 = t9*x*Real(1.0/10);
     const Real s8 = s6+(t7+t8);
     const Real s10 = s8+(t9+t10);
-    f[0] = 1-expX*s6;
-    f[1] = 1-expX*s8;
-    f[2] = 1-expX*s10;
+    d.x = x;
+    d.expX = expX;
+    d.t6 = t6;
+    d.t7 = t7;
+    d.t8 = t8;
+    d.t9 = t9;
+    d.t10 = t10;
+    d.f[0] = 1-expX*s6;
+    d.f[1] = 1-expX*s8;
+    d.f[2] = 1-expX*s10;
+    d.df[0] = expX*t6;
+    d.df[1] = expX*t8;
+    d.df[2] = expX*t10;
+}
+
+template <typename Real>
+__device__ __forceinline__ void damping_tail(Damping<Real>& d) {
+    const Real x = d.x;
     if (x < Real(5.5)) {
-        const Real tail10 = t10*x*TailPolynomial<Real, TangToenniesTail<Real>::TERMS, 1>::eval(x);
-        const Real tail8 = tail10+(t9+t10);
-        const Real tail6 = tail8+(t7+t8);
-        f[2] = expX*tail10;
+        const Real tail10 = d.t10*x*TailPolynomial<Real, TangToenniesTail<Real>::TERMS, 1>::eval(x);
+        const Real tail8 = tail10+(d.t9+d.t10);
+        const Real tail6 = tail8+(d.t7+d.t8);
+        d.f[2] = d.expX*tail10;
         if (x < Real(4.5))
-            f[1] = expX*tail8;
+            d.f[1] = d.expX*tail8;
         if (x < Real(3.5))
-            f[0] = expX*tail6;
+            d.f[0] = d.expX*tail6;
     }
-    df[0] = expX*t6;
-    df[1] = expX*t8;
-    df[2] = expX*t10;
+}
+
+template <typename Real>
+__device__ __forceinline__ void tang_toennies(Real x, Real f[3], Real df[3]) {
+    Damping<Real> d;
+    damping_main(x, d);
+    damping_tail(d);
+    for (int k = 0; k < 3; k++) {
+        f[k] = d.f[k];
+        df[k] = d.df[k];
+    }
 }
 
 // Per-particle parameters of a fused pair force, in the arithmetic type.
@@ -317,38 +348,52 @@ constexpr int MAX_WIDTH_CLASSES = 8;
 // the closed form (:176-187) is S = [4a^2b^2/D^3 (e_a - e_b) + r/D^2 (a e_a + b e_b)]/(8 pi r), i.e.
 // m1 = 4a^2b^2/(8 pi D^3), p_0 = a/(8 pi D^2), q1 = -m1, q0 = b/(8 pi D^2).  Every pair then runs the same
 // instructions whichever branch its widths select: no divergence between Taylor and closed-form pairs of a wave.
-// The fields are laid out for 16-byte LDS reads: the first 48 bytes hold everything a row of degree 3 needs.
+// The fields are laid out for 16-byte LDS reads: the first 48 bytes hold everything a row of degree 2 with one
+// overlap per pair needs (a1, a2: the energy S (a1 + a2 S) of its exchange and short-range terms together).
 struct alignas(16) OverlapRow {
     float k1, k2;            // log2(e)/w1, log2(e)/w2
     float n1, n2;            // 1/w1, 1/w2
     float m1;
-    float p[4];              // p_0 .. p_3
+    float p[3];              // p_0 .. p_2
     float q1, q0;
-    float kTaylor;           // the Taylor row is used where r*kTaylor < txLimit, the closed-form row elsewhere
-    float txLimit;
+    float a1, a2;            // A_i A_j + (K_i+K_j)/2 norm, (L_i+L_j)/2 norm^2 (FP64, rounded once)
+    float p3;
     float amplitude;         // exchange: A_i A_j
     float norm;              // 8 pi (a+b)^3, the short-range normalisation
     float kk, ll;            // short range: (K_i+K_j)/2, (L_i+L_j)/2
-    float gate;              // 0 if a or b is zero: the pair has no such term
+    float kTaylor;           // the Taylor row is used where r*kTaylor < txLimit, the closed-form row elsewhere
+    float txLimit;
     float high;              // 1 if any of p_4 .. p_8 is non-zero (a Taylor row of unequal widths), else 0
     float ph[5];             // p_4 .. p_8
-    float coefficient(int k) const { return k < 4 ? p[k] : ph[k-4]; }
+    float gate;              // 0 if a or b is zero: the pair has no such term
+    float pad[2];
+    float coefficient(int k) const { return k < 3 ? p[k] : k == 3 ? p3 : ph[k-4]; }
     void set_coefficient(int k, float v) {
-        if (k < 4)
+        if (k < 3)
             p[k] = v;
+        else if (k == 3)
+            p3 = v;
         else
             ph[k-4] = v;
     }
 };
-static_assert(sizeof(OverlapRow) == 24*sizeof(float), "OverlapRow layout");
-// Per ordered class pair: the Taylor row, then the closed-form row (both identical when one branch is never taken).
-constexpr int OVERLAP_PAIR_FLOATS = 48;
+static_assert(sizeof(OverlapRow) == 28*sizeof(float), "OverlapRow layout");
+// Per ordered class pair: the Taylor row, then the closed-form row (both identical when one branch is never taken),
+// then 4 floats of padding: class pairs 60 words apart start in different LDS banks, so the lanes of a wave that
+// read different class pairs' rows (ds_read_b128) do not conflict.
+constexpr int OVERLAP_ROW_FLOATS = 28;
+constexpr int OVERLAP_PAIR_FLOATS = 2*OVERLAP_ROW_FLOATS+4;
 
 // Per-particle values the tables do not cover (FP32): the dispersion width (per atom) and the class indices.
 struct TableParticle {
     float wd;
     int cls;                 // dispersion class | exchange class << 8 | short-range class << 16
 };
+
+// The two rows of ordered class pair `index` of an overlap table.
+__device__ __forceinline__ const OverlapRow* overlap_pair(const OverlapRow* table, int index) {
+    return (const OverlapRow*) ((const float*) table+OVERLAP_PAIR_FLOATS*index);
+}
 
 // Pointers into LDS and the sizes of the tables of a launch.
 struct PairTables {
@@ -382,11 +427,11 @@ __device__ __forceinline__ void overlap_from_row(const PairTables& tb, const Ove
             P = P*r + w.ph[k-4];
         }
         dP = dP*r + P;
-        P = P*r + w.p[3];
+        P = P*r + w.p3;
     }
     else {
         // every row of the wave is a closed form or an equal-width expansion: p_4 .. p_8 are zero
-        P = w.p[3];
+        P = w.p3;
     }
 #pragma unroll
     for (int k = 2; k >= 0; k--) {
@@ -405,6 +450,62 @@ __device__ __forceinline__ void overlap_from_row(const PairTables& tb, const Ove
 // Every table read is unconditional, so the compiler can issue them together: a class pair without a term has an
 // all-zero overlap row (S = dS = 0 exactly, amplitude and norm 0) or a zero dispersion row, whose damping argument
 // is replaced by a large constant so that every intermediate stays finite and the term contributes exactly 0.
+// The fixed table form, for tables whose overlap rows take one branch each and are of degree 2 at most, whose
+// dispersion rows are all live, and with at most one overlap per pair (exchange and short range share it, or only one
+// of them is evaluated), independent of the particles' index order.  The kernel finds the pair's rows from offsets
+// staged per particle (FixedPair), so the pair costs no class arithmetic; the overlap's energy terms come together
+// as S (a1 + a2 S).
+struct FixedPair {
+    float4 disp;             // c6, c8, c10, gate
+    const OverlapRow* row;
+    float wsum;              // sum of the two dispersion widths
+};
+
+template <int TERMS>
+__device__ __forceinline__ void pair_energy_fixed(float r, float invR, const FixedPair& fp, float beta2, float& energy, float& dEdR) {
+    constexpr bool DISP = (TERMS & IXH_TERM_DISPERSION) != 0;
+    constexpr bool OVERLAP = (TERMS & (IXH_TERM_EXCHANGE | IXH_TERM_SHORT_RANGE)) != 0;
+    energy = 0;
+    dEdR = 0;
+    const float invR2 = invR*invR;
+    if constexpr (DISP) {
+        // E = -sum_n f_n e_n with e_n = C_n r^-n; dE/dr = sum_n e_n (n f_n / r - f_n' x'), f_n' = exp(-x) x^n/n!:
+        // grouped as dE/dr = h/r - x' exp(-x) g with h = sum_n n f_n e_n and g = sum_n e_n x^n/n!
+        const float dxdr = beta2*fast_rcp(fp.wsum);
+        Damping<float> d;
+        damping_main<float>(dxdr*r, d);
+        damping_tail<float>(d);
+        const float invR6 = invR2*invR2*invR2;
+        const float invR8 = invR6*invR2;
+        const float e6 = fp.disp.x*invR6;
+        const float e8 = fp.disp.y*invR8;
+        const float e10 = fp.disp.z*(invR8*invR2);
+        const float p6 = d.f[0]*e6, p8 = d.f[1]*e8, p10 = d.f[2]*e10;
+        const float h = 6.0f*p6 + 8.0f*p8 + 10.0f*p10;
+        const float g = e6*d.t6 + e8*d.t8 + e10*d.t10;
+        energy = -(p6 + p8 + p10);
+        dEdR = h*invR - (dxdr*d.expX)*g;
+    }
+    if constexpr (OVERLAP) {
+        const OverlapRow& w = *fp.row;
+        const float e1 = fast_exp2(-r*w.k1);
+        const float e2 = fast_exp2(-r*w.k2);
+        // P = p0 + p1 r + p2 r^2, dP = p1 + 2 p2 r
+        const float P1 = w.p[2]*r + w.p[1];
+        const float dP = w.p[2]*r + P1;
+        const float P = P1*r + w.p[0];
+        const float Pa = P + w.m1*invR;
+        const float dPa = dP - w.m1*invR2;
+        const float Pb = w.q1*invR + w.q0;
+        const float dPb = -w.q1*invR2;
+        const float S = e1*Pa + e2*Pb;
+        const float dS = e1*(dPa - w.n1*Pa) + e2*(dPb - w.n2*Pb);
+        const float t = w.a1 + w.a2*S;
+        energy += S*t;
+        dEdR += dS*(t + w.a2*S);
+    }
+}
+
 template <int TERMS>
 __device__ __forceinline__ void pair_energy_table(float r, float invR, const TableParticle& pi, const TableParticle& pj, bool iFirst,
                                                   const PairTables& tb, float& energy, float& dEdR) {
@@ -432,7 +533,7 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     float sx = 0, dsx = 0;
     if (EXCH) {
         const int xi = (pi.cls >> 8) & 0xff, xj = (pj.cls >> 8) & 0xff;
-        const OverlapRow& w = overlap_row(tb, tb.exch+2*(iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi), r);
+        const OverlapRow& w = overlap_row(tb, overlap_pair(tb.exch, iFirst ? __umul24(xi, tb.numExch)+xj : __umul24(xj, tb.numExch)+xi), r);
         overlap_from_row(tb, w, r, invR, sx, dsx);
         const float amp = w.amplitude;
         energy += amp*sx;
@@ -440,7 +541,7 @@ __device__ __forceinline__ void pair_energy_table(float r, float invR, const Tab
     }
     if (SR) {
         const int si = (pi.cls >> 16) & 0xff, sj = (pj.cls >> 16) & 0xff;
-        const OverlapRow& w = overlap_row(tb, tb.sr+2*(iFirst ? __umul24(si, tb.numSr)+sj : __umul24(sj, tb.numSr)+si), r);
+        const OverlapRow& w = overlap_row(tb, overlap_pair(tb.sr, iFirst ? __umul24(si, tb.numSr)+sj : __umul24(sj, tb.numSr)+si), r);
         float ss, dss;
         if (EXCH && tb.shared) {
             ss = sx;
