@@ -240,12 +240,22 @@ __device__ __forceinline__ long long to_fixed(double v) {
 // FP32, five instructions: the high word is floor(v) (v_cvt_flr_i32_f32, one instruction where floor and convert
 // take two), the low word the fraction v - floor(v), taken in FP64 (v_fract_f64), times 2^32.  The FP64 fraction of
 // a 24-bit v is exact unless -2^-29 < v < 0, where it is 1+v rounded to FP64 (kept below 1): the result is then
-// floor(v 2^32) to within one unit.  Valid for |v| < 2^31.  (The FP32 fraction would lose up to 2^8 units there.)
+// floor(v 2^32) to within one unit.  Valid for |v| < 2^31.  (The FP32 fraction, one instruction less, would lose up to
+// 2^8 units for -1/2 < v < 0: 3e-8 kJ/mol/nm, more than the forces of distant pairs can afford.)
 __device__ __forceinline__ long long to_fixed(float v) {
     int hi;
     asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(v));
     const unsigned lo = (unsigned) (__builtin_amdgcn_fract((double) v)*FIXED_POINT_SCALE);
     return (long long) (((unsigned long long) (unsigned) hi << 32) | lo);
+}
+
+// The lanes of the wavefront for which p holds (the compare's lane mask itself, with no copy through a VGPR).
+__device__ __forceinline__ unsigned long long wave_ballot(bool p) {
+    return __builtin_amdgcn_ballot_w64(p);
+}
+// base plus the number of lanes below this one whose bit is set in mask (v_mbcnt_lo and v_mbcnt_hi).
+__device__ __forceinline__ unsigned lanes_below(unsigned long long mask, unsigned base) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned) (mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) mask, base));
 }
 
 __device__ __forceinline__ void add_fixed(long long* p, long long v) {

@@ -14,7 +14,9 @@
 //   3. one wavefront per site A takes the neighbours of its list within the cutoff at this evaluation's centroids
 //      (the reference's FP64 decision), walks their pairs (B, C), tests B-C in FP32 (FP64 near the cutoff or near a
 //      change of periodic image) and evaluates the triple in FP32; the forces on B and C go to LDS as 64-bit fixed
-//      point (integer atomics, so their order does not matter), the force on A stays in registers;
+//      point (integer atomics, so their order does not matter), the force on A stays in registers.  Segmented
+//      handles (clusters, k_triple_groups) walk the pairs by blocks of 64 neighbours, with both neighbours' forces
+//      summed in registers where the blocks are full;
 //   4. the site forces are shared equally among the particles of each site.
 #include "ixh_common.h"
 #include "ixh_internal.h"
@@ -470,12 +472,9 @@ __device__ __forceinline__ void tt3(Real x, Real& f, Real& df) {
         f = Real(1)-expX*(Real(1)+x+x*x*Real(0.5)+term3);
 }
 
-// 3. One workgroup per work item: a site A and all of its neighbour pairs, or (segmented handles) a chunk of them, so
-// that the first sites of a large cluster, with O(n^2) pairs each, are shared out over several workgroups.  WAVES
-// wavefronts share the neighbours staged in LDS and take every WAVES-th step of 64 pairs: segmented handles, whose
-// large clusters need a lot of LDS per item, use 4 and so keep 4 times as many wavefronts per CU in flight.
-template <typename Real, int WAVES>
-__global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
+// 3. Plain handles: one wavefront per site A and all of its neighbour pairs (segmented handles: k_triple_groups).
+template <typename Real>
+__global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     typedef typename Vec4Of<Real>::type R4;
     extern __shared__ double sharedStorage[];
     R4* sDelta = (R4*) sharedStorage;           // AB vector of each neighbour B and its square |AB|^2
@@ -486,10 +485,8 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     int* sMol = sIndex+t.capacity;              // molecule
     Real* sWidth = (Real*) (sMol+t.capacity);   // damping width
     int* sSlot = (int*) (sWidth+t.capacity);    // skin lists: the LDS slot of each list entry, -1 if beyond the cutoff
-    const int item = (t.itemIndex != nullptr ? t.itemIndex[blockIdx.x] : (int) blockIdx.x);
-    const int i = (t.itemSite != nullptr ? t.itemSite[item] : item);
+    const int i = (t.itemIndex != nullptr ? t.itemIndex[blockIdx.x] : (int) blockIdx.x);
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
     const int begin = t.start[i], listLength = (t.count != nullptr ? t.count[i] : t.start[i+1]-begin);
     const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
     const bool alchI = t.siteParams[4*i+3] != 0;
@@ -508,9 +505,9 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
         sForce[3*a] = sForce[3*a+1] = sForce[3*a+2] = 0;
     };
     int m = listLength;
-    if (t.skinLists) {
-        // one wavefront (WAVES == 1): the list's sites within the cutoff now, in list order (ReferenceInteraxThreeBodyIxn.cpp:
-        // 97-103, the reference's FP64 rounding), compacted by ballot
+    {
+        // the list's sites within the cutoff now, in list order (ReferenceInteraxThreeBodyIxn.cpp:97-103, the
+        // reference's FP64 rounding), compacted by ballot
         const double* si = t.sites+3*i;
         const double inv[3] = {1.0/t.box.ax, 1.0/t.box.by, 1.0/t.box.cz};
         int kept = 0;
@@ -537,34 +534,13 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
         }
         m = kept;
     }
-    else
-        for (int a = threadIdx.x; a < m; a += 64*WAVES) {
-            const double4 d4 = t.nbrDelta[begin+a];
-            const double d[3] = {d4.x, d4.y, d4.z};
-            stage(a, t.nbrIndex[begin+a], d);
-        }
     __syncthreads();
     const Real cI = t.scaleFactor*Real(1.5)*aI*wI;
     Real fA[3] = {0, 0, 0};
     double energy = 0;
     int count = 0;
-    // Work items of segmented handles: consecutive shares of the site's neighbour pairs, each share taken by one
-    // wavefront, which sums its forces and energy and adds them in fixed point (forces) or stores them (energy, summed
-    // per system in share order later).  The shares depend on the site's list capacity only, so a site's energy and
-    // forces are the same for any number of wavefronts and any number of shares per item.  Plain handles: one share,
-    // all of the site's pairs.
     // pair indices fit 32 bits: the LDS holds at most a few thousand neighbours
-    int pBegin = 0, pEnd = m*(m-1)/2;
-    int shareSize = pEnd;
-    int numShares = 1;
-    const bool shared = (t.itemShares != nullptr);
-    if (shared) {
-        const int2 sh = t.itemShares[item];
-        pBegin = t.itemFirst[item];
-        numShares = sh.x;
-        shareSize = sh.y;
-        pEnd = min(pEnd, pBegin+numShares*shareSize);
-    }
+    const int pEnd = m*(m-1)/2;
     // Pair p of the site's neighbour pairs.  The first T = M(M-1)/2 (M = min(m, 64)), which are all the pairs within
     // the first 64 neighbours, are taken in rotation order: rotation s holds the pairs (x, (x+s+1) mod M), x < M (the
     // last, for even M, only x < M/2), so that a step's lanes hold each neighbour about twice (exactly twice, once on
@@ -573,10 +549,9 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
     // where a step's lanes share one b or two consecutive ones.
     const int M = min(m, 64);
     const int T = M*(M-1)/2;
-    // A wavefront takes a share's pairs 64 per step, so that in the rows its steps follow each other along a row: the
+    // The wavefront takes the pairs 64 per step, so that in the rows its steps follow each other along a row: the
     // pair 64 further on is in the same row or the next (rows have b >= 64 pairs), and the forces on the row's b are
-    // summed per lane over the steps of the row and over the wavefront only when the wavefront leaves it.  Share sizes
-    // are multiples of 64, so the steps are the same whichever wavefront takes a share.
+    // summed per lane over the steps of the row and over the wavefront only when the wavefront leaves it.
     int R = -1;                 // the row whose forces accR holds (accR1: row R + 1); wavefront-uniform
     bool usedR1 = false;        // some lane has added to accR1
     Real accR[3] = {0, 0, 0}, accR1[3] = {0, 0, 0};
@@ -588,8 +563,8 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
             acc[k] = 0;
         }
     };
-    for (int v = wave; v < numShares; v += WAVES) {
-        const int myBegin = pBegin+shareSize*v, myEnd = min(pEnd, myBegin+shareSize);
+    {
+        const int myBegin = 0, myEnd = pEnd;
         int a = 0, b = 0;
         for (int p0 = myBegin; p0 < myEnd; p0 += 64) {
             const int p = p0+lane;
@@ -779,74 +754,399 @@ __global__ void __launch_bounds__(64*WAVES) k_triples(TripleArgs<Real> t) {
             if (usedR1)
                 flushRow(R+1, accR1);
         }
-        R = -1;
-        usedR1 = false;
-        if (shared) {
-            // the share's force on A and its energy and triple count
-            for (int k = 0; k < 3; k++) {
-                long long f = to_fixed(fA[k]);
-                for (int offset = 32; offset > 0; offset >>= 1)
-                    f += __shfl_xor(f, offset, 64);
-                if (lane == 0)
-                    add_fixed(&t.siteForce[(size_t) k*t.numSites+i], f);
-                fA[k] = 0;
-            }
-            const double e = wave_sum(energy);
-            const int c = (int) wave_sum((double) count);
-            if (lane == 0) {
-                t.itemEnergy[t.itemShareBase[item]+v] = e;
-                t.itemCount[t.itemShareBase[item]+v] = c;
-            }
-            energy = 0;
-            count = 0;
-        }
     }
     __syncthreads();
-    if (t.skinLists) {
-        // plain handles: the neighbour forces as plain stores, one per list entry (zero beyond the cutoff), summed per
-        // site by k_gather_site_forces; the force on A is this workgroup's alone (integer sums: exact in any order)
-        for (int a = lane; a < listLength; a += 64) {
-            const int slot = sSlot[a];
-            for (int k = 0; k < 3; k++)
-                t.contribution[3*((long long) begin+a)+k] = (slot >= 0 ? (long long) sForce[3*slot+k] : 0LL);
-        }
-        for (int k = 0; k < 3; k++) {
-            long long v = to_fixed(fA[k]);
-            for (int offset = 32; offset > 0; offset >>= 1)
-                v += __shfl_xor(v, offset, 64);
-            if (lane == 0)
-                t.siteForce[(size_t) k*t.numSites+i] = v;
-        }
+    // the neighbour forces as plain stores, one per list entry (zero beyond the cutoff), summed per site by
+    // k_gather_site_forces; the force on A is this workgroup's alone (integer sums: exact in any order)
+    for (int a = lane; a < listLength; a += 64) {
+        const int slot = sSlot[a];
+        for (int k = 0; k < 3; k++)
+            t.contribution[3*((long long) begin+a)+k] = (slot >= 0 ? (long long) sForce[3*slot+k] : 0LL);
     }
-    else {
-        // flush the neighbour forces (integer sums: the order of the atomics does not matter)
-        for (int a = threadIdx.x; a < m; a += 64*WAVES) {
-            int j = sIndex[a];
-            for (int k = 0; k < 3; k++)
-                add_fixed(&t.siteForce[(size_t) k*t.numSites+j], (long long) sForce[3*a+k]);
-        }
+    for (int k = 0; k < 3; k++) {
+        long long v = to_fixed(fA[k]);
+        for (int offset = 32; offset > 0; offset >>= 1)
+            v += __shfl_xor(v, offset, 64);
+        if (lane == 0)
+            t.siteForce[(size_t) k*t.numSites+i] = v;
     }
-    if (shared)
-        return;
-    // energy and count: per wavefront, then summed in wavefront order
-    __shared__ double sWaveEnergy[WAVES];
-    __shared__ int sWaveCount[WAVES];
-    double e = wave_sum(energy);
-    int c = (int) wave_sum((double) count);
+    // energy and triple count of the site
+    const double e = wave_sum(energy);
+    const int c = (int) wave_sum((double) count);
     if (lane == 0) {
-        sWaveEnergy[wave] = e;
-        sWaveCount[wave] = c;
+        t.itemEnergy[i] = e;
+        t.itemCount[i] = c;
+    }
+}
+
+// The energy and forces of one triple (A, a, b) of a segmented handle: A the work item's site, a and b two of its
+// neighbours (staged vectors AB and AC with their squares, parameters as the staging writes them; b's width and site
+// index are read only where they are needed), decided within the cutoff in FP32 (in FP64 near it).
+// InteraxThreeBodyTerms.h:51-146, ReferenceInteraxThreeBodyIxn.cpp:104-134 and InteraxThreeBodyForceImpl.h:69-72.
+// Returns false for a triple with no contribution (beyond the cutoff or C9 = 0).  Only the forces on a and b are
+// formed: the three forces of a triple sum to zero, and the force on A is taken as the negated sum of its
+// neighbours' fixed-point forces.
+template <typename Real>
+struct TripleResult {
+    Real e;
+    Real fa[3], fb[3];   // the forces on a and on b
+};
+
+template <typename Real, bool DAMPED>
+__device__ __forceinline__ bool segment_triple(const TripleArgs<Real>& t, const typename Vec4Of<Real>::type& va,
+                                               const typename Vec4Of<Real>::type& pa, Real widthA, int siteA,
+                                               const typename Vec4Of<Real>::type& vb, const typename Vec4Of<Real>::type& pb,
+                                               const Real* sWidth, const int* sIndex, int b, Real cI, Real sI, bool alchI,
+                                               TripleResult<Real>& out) {
+    Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
+    Real w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+    bool ok;
+    if (fabs(w-t.cutoff2r) < t.band*t.cutoff2r) {
+        // segments are nonperiodic: BC from the two centroids in FP64
+        double d[3];
+        ref_delta(t.sites+3*siteA, t.sites+3*sIndex[b], 0, t.box, d);
+        ok = ref_norm2(d) < t.cutoff2;
+        c[0] = Real(d[0]);
+        c[1] = Real(d[1]);
+        c[2] = Real(d[2]);
+        w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+    }
+    else
+        ok = w < t.cutoff2r;
+    if (!ok)
+        return false;
+    Real c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.z*(pa.y+pb.y)*pb.z);
+    if (t.lambda != Real(1) && (alchI || pa.w < 0 || pb.w < 0))
+        c9 *= t.lambda;
+    if (c9 == 0)
+        return false;
+    const Real A[3] = {va.x, va.y, va.z}, B[3] = {vb.x, vb.y, vb.z};
+    const Real u = va.w, v = vb.w;
+    const Real pp = A[0]*B[0]+A[1]*B[1]+A[2]*B[2];
+    const Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
+    const Real s = B[0]*c[0]+B[1]*c[1]+B[2]*c[2];
+    const Real uv = u*v;
+    const Real rs = fast_rsqrt(uv*w);
+    const Real invP = rs*rs;
+    const Real invP32 = invP*rs;
+    const Real invP52 = invP32*invP;
+    const Real pqs = pp*q*s;
+    Real e = c9*(invP32-3*pqs*invP52);
+    const Real radial = c9*(-3*invP32+15*pqs*invP52);
+    const Real cross = -3*c9*invP52;
+    const Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*(invP*uv);
+    const Real X = pp*s*cross, Y = pp*q*cross, Z = q*s*cross;
+    Real aA = X-ru, aB = Y-Z, aC = rw-X;
+    Real bA = -(Z+X), bB = -(rv+Y), bC = -(Y+rw);
+    if constexpr (DAMPED) {
+        const Real widthB = sWidth[b];
+        Real len[3] = {sqrt(u), sqrt(v), sqrt(w)};
+        Real rates[3] = {2*t.dampingScale/(sI+widthA), 2*t.dampingScale/(sI+widthB), 2*t.dampingScale/(widthA+widthB)};
+        Real f[3], dfdr[3];
+        for (int k = 0; k < 3; k++) {
+            if (t.useDamping) {
+                tt3<Real>(rates[k]*len[k], f[k], dfdr[k]);
+                dfdr[k] *= rates[k];
+            }
+            else {
+                f[k] = 1;
+                dfdr[k] = 0;
+            }
+            if (t.invDampingRadius > 0) {
+                Real x = len[k]*t.invDampingRadius;
+                Real x6 = x*x*x;
+                x6 *= x6;
+                Real g = -expm1(-x6);
+                Real dg = 6*x6*exp(-x6)/len[k];
+                dfdr[k] = dfdr[k]*g + f[k]*dg;
+                f[k] *= g;
+            }
+        }
+        const Real damping = f[0]*f[1]*f[2];
+        const Real sa = e*dfdr[0]*f[1]*f[2]/len[0];
+        const Real sb = e*f[0]*dfdr[1]*f[2]/len[1];
+        const Real sc = e*f[0]*f[1]*dfdr[2]/len[2];
+        aA = aA*damping-sa;
+        aB *= damping;
+        aC = aC*damping+sc;
+        bA *= damping;
+        bB = bB*damping-sb;
+        bC = bC*damping-sc;
+        e *= damping;
+    }
+    out.e = e;
+    for (int k = 0; k < 3; k++) {
+        out.fa[k] = A[k]*aA+B[k]*aB+c[k]*aC;
+        out.fb[k] = A[k]*bA+B[k]*bB+c[k]*bC;
+    }
+    return true;
+}
+
+// Lane l receives the value of lane (l+1) mod 64 (DPP wave_rol:1; every lane active).
+__device__ __forceinline__ float lane_rotate(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_rotate(double v) {
+    return wave_rotate_down(v);
+}
+
+// Sum over the 16 lanes of each row, every lane of the row receiving the same order-fixed total.
+__device__ __forceinline__ float row_sum(float v) {
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x141, 0xF, 0xF, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x140, 0xF, 0xF, false));
+    return v;
+}
+__device__ __forceinline__ double row_sum(double v) {
+    for (int offset = 1; offset < 16; offset <<= 1)
+        v += __shfl_xor(v, offset, 64);
+    return v;
+}
+
+// Steps (64 pairs each) of the triple kernel of segmented handles for a site with m neighbours: every group but the
+// last has 32 rotation steps and m - 64 (k + 1) later-block steps, the last group the rotation steps of its nb
+// neighbours.
+__host__ __device__ __forceinline__ long long segment_steps(long long m) {
+    if (m < 2)
+        return 0;
+    const long long last = (m-1)/64, nb = m-64*last;
+    return last*m-32*last*last+(nb*(nb-1)/2+63)/64;
+}
+
+// 3b. Segmented handles (clusters): one workgroup per work item, a site A and consecutive shares of its neighbour
+// pairs, one wavefront per share (blockDim.x / 64 wavefronts take every so-many-th share).  The pairs a < b of the
+// site's m neighbours come in steps of 64 (one per lane), grouped by blocks of 64 neighbours: group k holds first the
+// pairs within block k (neighbours 64 k .. 64 k + nb - 1) in rotation order -- rotation r holds (x, (x+r+1) mod nb)
+// -- then the pairs of block k with each later block l, 64 steps for a full block l (step s: lane x holds
+// a = 64 k + x and b = 64 l + (x+s) mod 64) and one step per neighbour b of a last, partial block (lane x holds
+// a = 64 k + x and the one b).  Group k starts at step k m - 32 k^2.
+//   In a full block k lane x holds a = 64 k + x in every step of the group: its loads are made once and the force on
+// a is summed in the lane's registers over the group.  In the rotation steps of a full block and in the steps with a
+// full block l, the b of lane x is the previous step's b of lane x+1, so the force on b is summed in a register that
+// moves down one lane per step (DPP wave rotation) and is added to LDS in fixed point once per block pair; with a
+// partial block l the one b's force is summed over each row of 16 lanes, and in the rotation steps of a partial
+// block every lane adds its own a and b.  The force on A is the negated sum of its neighbours' forces (fixed point,
+// exact).  The shares depend on the site's list capacity only, so the results do not depend on the launch shape (one
+// instantiation for every number of wavefronts).
+template <typename Real, bool DAMPED>
+__global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
+    typedef typename Vec4Of<Real>::type R4;
+    extern __shared__ double sharedStorage[];
+    R4* sDelta = (R4*) sharedStorage;
+    R4* sParam = sDelta+t.capacity;
+    unsigned long long* sForce = (unsigned long long*) (sParam+t.capacity);
+    int* sIndex = (int*) (sForce+3*(size_t) t.capacity);
+    int* sMol = sIndex+t.capacity;
+    Real* sWidth = (Real*) (sMol+t.capacity);
+    const int item = (t.itemIndex != nullptr ? t.itemIndex[blockIdx.x] : (int) blockIdx.x);
+    const int i = t.itemSite[item];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6, waves = (int) blockDim.x >> 6;
+    const int begin = t.start[i], m = t.count[i];
+    const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
+    const bool alchI = t.siteParams[4*i+3] != 0;
+    for (int a = threadIdx.x; a < m; a += blockDim.x) {
+        const double4 d4 = t.nbrDelta[begin+a];
+        const int j = t.nbrIndex[begin+a];
+        const Real x = Real(d4.x), y = Real(d4.y), z = Real(d4.z);
+        const Real u = x*x+y*y+z*z;
+        R4 dr = {x, y, z, u};
+        sDelta[a] = dr;
+        const Real alpha = Real(t.siteParams[4*j]), w = Real(t.siteParams[4*j+1]);
+        R4 pr = {alpha*w, w, wI+w, (t.siteParams[4*j+3] != 0 ? -fast_rcp(u) : fast_rcp(u))};
+        sParam[a] = pr;
+        sIndex[a] = j;
+        sMol[a] = t.molecule[j];
+        sWidth[a] = Real(t.siteParams[4*j+2]);
+        sForce[3*a] = sForce[3*a+1] = sForce[3*a+2] = 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double total = 0;
-        int triples = 0;
-        for (int k = 0; k < WAVES; k++) {
-            total += sWaveEnergy[k];
-            triples += sWaveCount[k];
+    const Real cI = t.scaleFactor*Real(1.5)*aI*wI;
+    const int2 sh = t.itemShares[item];
+    const int numShares = sh.x, shareSteps = sh.y/64;
+    const int firstStep = t.itemFirst[item]/64;
+    const int K = (m+63)/64;
+    const int total = (int) segment_steps(m);
+    // add a register sum of forces to neighbour b in fixed point
+    auto addForce = [&](int b, const Real (&f)[3]) {
+        for (int c = 0; c < 3; c++) {
+            const long long v = to_fixed(f[c]);
+            if (v != 0)
+                atomicAdd(&sForce[3*b+c], (unsigned long long) v);
         }
-        t.itemEnergy[item] = total;
-        t.itemCount[item] = triples;
+    };
+    for (int v = wave; v < numShares; v += waves) {
+        const int g0 = firstStep+v*shareSteps, g1 = min(total, g0+shareSteps);
+        double energy = 0;
+        int count = 0;
+        int g = g0;
+        // the group of step g0 and the step's place in it
+        int k = 0, groupStart = 0;
+        while (k < K-1 && g0 >= groupStart+32+(m-64*(k+1))) {
+            groupStart += 32+(m-64*(k+1));
+            k++;
+        }
+        int local = g0-groupStart;
+        while (g < g1) {
+            const int nb = min(64, m-64*k);
+            const bool full = (nb == 64);
+            const int rotSteps = (nb*(nb-1)/2+63)/64;
+            const int groupSteps = rotSteps+max(0, m-64*(k+1));
+            // the lane's own neighbour a = 64 k + lane (full blocks)
+            R4 oDelta = {0, 0, 0, 0}, oParam = {0, 0, 0, 0};
+            Real oWidth = 0;
+            int oMol = 0, oSite = 0;
+            if (full) {
+                const int a = 64*k+lane;
+                oDelta = sDelta[a];
+                oParam = sParam[a];
+                oWidth = sWidth[a];
+                oMol = sMol[a];
+                oSite = sIndex[a];
+            }
+            Real accA[3] = {0, 0, 0};
+            while (g < g1 && local < groupSteps) {
+                TripleResult<Real> r;
+                if (local < rotSteps && full) {
+                    // rotation steps of a full block: lane x holds (x, (x+rot+1) mod 64), rotation 31 only x < 32
+                    const int r0 = local, r1 = min(rotSteps, local+(g1-g));
+                    Real accB[3] = {0, 0, 0};
+                    for (int rot = r0; rot < r1; rot++) {
+                        const int b = 64*k+((lane+rot+1) & 63);
+                        if ((rot < 31 || lane < 32) && oMol != sMol[b] &&
+                            segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, sDelta[b], sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                            energy += (double) r.e;
+                            count++;
+                            for (int c = 0; c < 3; c++) {
+                                accA[c] += r.fa[c];
+                                accB[c] += r.fb[c];
+                            }
+                        }
+                        for (int c = 0; c < 3; c++)
+                            accB[c] = lane_rotate(accB[c]);
+                    }
+                    addForce(64*k+((lane+r1+1) & 63), accB);
+                    g += r1-r0;
+                    local = r1;
+                }
+                else if (local < rotSteps) {
+                    // rotation steps of a partial block: pair p of its nb (nb - 1) / 2; rot = floor(p / nb) from
+                    // (p + 1/2) / nb in FP32, which lies at least 1/(2 nb) from an integer (p < 2016, nb < 64)
+                    const int r1 = min(rotSteps, local+(g1-g));
+                    const float invNb = 1.0f/(float) nb;
+                    for (int step = local; step < r1; step++) {
+                        const int p = 64*step+lane;
+                        if (p < nb*(nb-1)/2) {
+                            const int rot = (int) (((float) p+0.5f)*invNb);
+                            const int x = p-rot*nb;
+                            int y = x+rot+1;
+                            if (y >= nb)
+                                y -= nb;
+                            const int a = 64*k+x, b = 64*k+y;
+                            if (sMol[a] != sMol[b] &&
+                                segment_triple<Real, DAMPED>(t, sDelta[a], sParam[a], sWidth[a], sIndex[a], sDelta[b], sParam[b], sWidth, sIndex, b, cI,
+                                                     sI, alchI, r)) {
+                                energy += (double) r.e;
+                                count++;
+                                addForce(a, r.fa);
+                                addForce(b, r.fb);
+                            }
+                        }
+                    }
+                    g += r1-local;
+                    local = r1;
+                }
+                else {
+                    const int t0 = local-rotSteps;
+                    const int l = k+1+(t0 >> 6), s0 = t0 & 63;
+                    const int nbl = min(64, m-64*l);
+                    const int s1 = min(nbl, s0+(g1-g));
+                    if (nbl == 64) {
+                        // block pair (k, l): lane x holds a = 64 k + x and b = 64 l + (x+s) mod 64
+                        Real accB[3] = {0, 0, 0};
+                        for (int step = s0; step < s1; step++) {
+                            const int b = 64*l+((lane+step) & 63);
+                            if (oMol != sMol[b] &&
+                                segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, sDelta[b], sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                                energy += (double) r.e;
+                                count++;
+                                for (int c = 0; c < 3; c++) {
+                                    accA[c] += r.fa[c];
+                                    accB[c] += r.fb[c];
+                                }
+                            }
+                            for (int c = 0; c < 3; c++)
+                                accB[c] = lane_rotate(accB[c]);
+                        }
+                        addForce(64*l+((lane+s1) & 63), accB);
+                    }
+                    else {
+                        // a partial block l: one b per step, its force summed over each row of 16 lanes; lane
+                        // (step mod 16) of each row keeps the row's sum, added to LDS for 16 steps at a time
+                        Real keep[3] = {0, 0, 0};
+                        for (int step = s0; step < s1; step++) {
+                            const int b = 64*l+step;
+                            Real fb[3] = {0, 0, 0};
+                            if (oMol != sMol[b] &&
+                                segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, sDelta[b], sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                                energy += (double) r.e;
+                                count++;
+                                for (int c = 0; c < 3; c++) {
+                                    accA[c] += r.fa[c];
+                                    fb[c] = r.fb[c];
+                                }
+                            }
+                            const bool slot = ((lane & 15) == (step & 15));
+                            for (int c = 0; c < 3; c++) {
+                                const Real sum = row_sum(fb[c]);
+                                if (slot)
+                                    keep[c] = sum;
+                            }
+                            if ((step & 15) == 15 || step == s1-1) {
+                                // lanes of slots without a step of this window keep zero and add nothing
+                                addForce(64*l+(step & ~15)+(lane & 15), keep);
+                                for (int c = 0; c < 3; c++)
+                                    keep[c] = 0;
+                            }
+                        }
+                    }
+                    g += s1-s0;
+                    local += s1-s0;
+                }
+            }
+            if (full)
+                addForce(64*k+lane, accA);
+            if (local == groupSteps) {
+                k++;
+                local = 0;
+            }
+        }
+        // the share's energy and triple count
+        const double e = wave_sum(energy);
+        const int n = (int) wave_sum((double) count);
+        if (lane == 0) {
+            t.itemEnergy[t.itemShareBase[item]+v] = e;
+            t.itemCount[t.itemShareBase[item]+v] = n;
+        }
+    }
+    __syncthreads();
+    // the neighbour forces (integer sums: the order of the atomics does not matter) and, negated, their sum on A
+    long long sum[3] = {0, 0, 0};
+    for (int a = threadIdx.x; a < m; a += blockDim.x) {
+        const int j = sIndex[a];
+        for (int c = 0; c < 3; c++) {
+            const long long f = (long long) sForce[3*a+c];
+            add_fixed(&t.siteForce[(size_t) c*t.numSites+j], f);
+            sum[c] += f;
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        long long f = sum[c];
+        for (int offset = 32; offset > 0; offset >>= 1)
+            f += __shfl_xor(f, offset, 64);
+        if (lane == 0)
+            add_fixed(&t.siteForce[(size_t) c*t.numSites+i], -f);
     }
 }
 
@@ -1146,8 +1446,8 @@ struct ixh_threebody {
 
 namespace {
 
-// Shares of a site's neighbour pairs (segmented handles): about a quarter of the pairs its list can hold, a multiple of
-// 64 between SHARE_MIN and SHARE_MAX pairs.  They depend on the capacity alone, never on the launch shape.
+// Shares of a site's neighbour pairs (segmented handles): about a quarter of the steps of 64 pairs its list can hold
+// (segment_steps), between SHARE_MIN and SHARE_MAX pairs.  They depend on the capacity alone, never on the launch shape.
 constexpr long long SHARE_MIN = 64, SHARE_MAX = 4096;
 
 // The work items of a segmented handle's triple kernel: every site's shares, about chunkPairs pairs of them per item
@@ -1162,7 +1462,7 @@ void build_items(ixh_threebody* h) {
     for (int g = 0; g < h->numSegments; g++) {
         for (int i = h->segOffsets[g]; i < h->segOffsets[g+1]; i++) {
             const long long cap = h->segOffsets[g+1]-i-1;
-            const long long pairs = (cap >= 2 ? cap*(cap-1)/2 : 0);
+            const long long pairs = 64*segment_steps(cap);
             if (pairs > INT32_MAX)
                 throw Error(IXH_ERROR_CAPACITY, "ixh_threebody_create_segmented: a segment has too many sites for the triple kernel");
             const long long size = std::min(SHARE_MAX, std::max(SHARE_MIN, ((pairs+3)/4+63)/64*64));
@@ -1426,16 +1726,12 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     bool timed = h->timer.start(stream);
     const int workgroups = (subsetItems >= 0 ? subsetItems : (items ? h->numItems : h->numSites));
     if (workgroups > 0) {
-        if (items && h->tripleWaves == 1)
-            hipLaunchKernelGGL((k_triples<Real, 1>), dim3(workgroups), dim3(64), shared, stream, t);
-        else if (items && h->tripleWaves == 2)
-            hipLaunchKernelGGL((k_triples<Real, 2>), dim3(workgroups), dim3(128), shared, stream, t);
-        else if (items && h->tripleWaves == 8)
-            hipLaunchKernelGGL((k_triples<Real, 8>), dim3(workgroups), dim3(512), shared, stream, t);
+        if (items && (t.useDamping || t.invDampingRadius > 0))
+            hipLaunchKernelGGL((k_triple_groups<Real, true>), dim3(workgroups), dim3(64*h->tripleWaves), shared, stream, t);
         else if (items)
-            hipLaunchKernelGGL((k_triples<Real, 4>), dim3(workgroups), dim3(256), shared, stream, t);
+            hipLaunchKernelGGL((k_triple_groups<Real, false>), dim3(workgroups), dim3(64*h->tripleWaves), shared, stream, t);
         else
-            hipLaunchKernelGGL((k_triples<Real, 1>), dim3(workgroups), dim3(64), shared, stream, t);
+            hipLaunchKernelGGL((k_triples<Real>), dim3(workgroups), dim3(64), shared, stream, t);
     }
     IXH_HIP_CHECK(hipGetLastError());
     if (timed)
