@@ -784,13 +784,24 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
 // index are read only where they are needed), decided within the cutoff in FP32 (in FP64 near it).
 // InteraxThreeBodyTerms.h:51-146, ReferenceInteraxThreeBodyIxn.cpp:104-134 and InteraxThreeBodyForceImpl.h:69-72.
 // Returns false for a triple with no contribution (beyond the cutoff or C9 = 0).  Only the forces on a and b are
-// formed: the three forces of a triple sum to zero, and the force on A is taken as the negated sum of its
-// neighbours' fixed-point forces.
+// formed, as combinations of AB and AC (BC = AC - AB): the three forces of a triple sum to zero, and the force on A
+// is taken as the negated sum of its neighbours' fixed-point forces.  The staged parameters of neighbour j are
+// (alpha_j w_j / (w_A + w_j), w_j, w_A + w_j, +-1/|Aj|^2), so that C9 = 1.5 c alpha_A w_A P_a P_b (w_A+w_a+w_b) /
+// (w_a+w_b) with P_j the first of them.
 template <typename Real>
 struct TripleResult {
     Real e;
-    Real fa[3], fb[3];   // the forces on a and on b
+    Real aA, aB;   // the force on a: AB aA + AC aB
+    Real bA, bB;   // the force on b: AB bA + AC bB
 };
+
+// acc += AB kA + AC kB
+template <typename Real, typename R4>
+__device__ __forceinline__ void add_combination(Real (&acc)[3], const R4& A, const R4& B, Real kA, Real kB) {
+    acc[0] += A.x*kA+B.x*kB;
+    acc[1] += A.y*kA+B.y*kB;
+    acc[2] += A.z*kA+B.z*kB;
+}
 
 template <typename Real, bool DAMPED>
 __device__ __forceinline__ bool segment_triple(const TripleArgs<Real>& t, const typename Vec4Of<Real>::type& va,
@@ -815,7 +826,7 @@ __device__ __forceinline__ bool segment_triple(const TripleArgs<Real>& t, const 
         ok = w < t.cutoff2r;
     if (!ok)
         return false;
-    Real c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.z*(pa.y+pb.y)*pb.z);
+    Real c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.y+pb.y);
     if (t.lambda != Real(1) && (alchI || pa.w < 0 || pb.w < 0))
         c9 *= t.lambda;
     if (c9 == 0)
@@ -875,10 +886,10 @@ __device__ __forceinline__ bool segment_triple(const TripleArgs<Real>& t, const 
         e *= damping;
     }
     out.e = e;
-    for (int k = 0; k < 3; k++) {
-        out.fa[k] = A[k]*aA+B[k]*aB+c[k]*aC;
-        out.fb[k] = A[k]*bA+B[k]*bB+c[k]*bC;
-    }
+    out.aA = aA-aC;
+    out.aB = aB+aC;
+    out.bA = bA-bC;
+    out.bB = bB+bC;
     return true;
 }
 
@@ -946,6 +957,10 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
     const int begin = t.start[i], m = t.count[i];
     const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
     const bool alchI = t.siteParams[4*i+3] != 0;
+    // the work item's forces on the neighbours, summed over 16 slots per component (few lanes per LDS address)
+    __shared__ unsigned long long sSiteSum[3][16];
+    if (threadIdx.x < 48)
+        sSiteSum[threadIdx.x >> 4][threadIdx.x & 15] = 0;
     for (int a = threadIdx.x; a < m; a += blockDim.x) {
         const double4 d4 = t.nbrDelta[begin+a];
         const int j = t.nbrIndex[begin+a];
@@ -954,7 +969,7 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
         R4 dr = {x, y, z, u};
         sDelta[a] = dr;
         const Real alpha = Real(t.siteParams[4*j]), w = Real(t.siteParams[4*j+1]);
-        R4 pr = {alpha*w, w, wI+w, (t.siteParams[4*j+3] != 0 ? -fast_rcp(u) : fast_rcp(u))};
+        R4 pr = {alpha*w/(wI+w), w, wI+w, (t.siteParams[4*j+3] != 0 ? -fast_rcp(u) : fast_rcp(u))};
         sParam[a] = pr;
         sIndex[a] = j;
         sMol[a] = t.molecule[j];
@@ -979,7 +994,7 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
     for (int v = wave; v < numShares; v += waves) {
         const int g0 = firstStep+v*shareSteps, g1 = min(total, g0+shareSteps);
         double energy = 0;
-        int count = 0;
+        int count = 0;   // the share's triples (wavefront-uniform)
         int g = g0;
         // the group of step g0 and the step's place in it
         int k = 0, groupStart = 0;
@@ -1014,15 +1029,16 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
                     Real accB[3] = {0, 0, 0};
                     for (int rot = r0; rot < r1; rot++) {
                         const int b = 64*k+((lane+rot+1) & 63);
+                        const R4 vb = sDelta[b];
+                        bool hit = false;
                         if ((rot < 31 || lane < 32) && oMol != sMol[b] &&
-                            segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, sDelta[b], sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                            segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, vb, sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                            hit = true;
                             energy += (double) r.e;
-                            count++;
-                            for (int c = 0; c < 3; c++) {
-                                accA[c] += r.fa[c];
-                                accB[c] += r.fb[c];
-                            }
+                            add_combination(accA, oDelta, vb, r.aA, r.aB);
+                            add_combination(accB, oDelta, vb, r.bA, r.bB);
                         }
+                        count += __popcll(wave_ballot(hit));
                         for (int c = 0; c < 3; c++)
                             accB[c] = lane_rotate(accB[c]);
                     }
@@ -1037,6 +1053,7 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
                     const float invNb = 1.0f/(float) nb;
                     for (int step = local; step < r1; step++) {
                         const int p = 64*step+lane;
+                        bool hit = false;
                         if (p < nb*(nb-1)/2) {
                             const int rot = (int) (((float) p+0.5f)*invNb);
                             const int x = p-rot*nb;
@@ -1044,15 +1061,19 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
                             if (y >= nb)
                                 y -= nb;
                             const int a = 64*k+x, b = 64*k+y;
+                            const R4 va = sDelta[a], vb = sDelta[b];
                             if (sMol[a] != sMol[b] &&
-                                segment_triple<Real, DAMPED>(t, sDelta[a], sParam[a], sWidth[a], sIndex[a], sDelta[b], sParam[b], sWidth, sIndex, b, cI,
-                                                     sI, alchI, r)) {
+                                segment_triple<Real, DAMPED>(t, va, sParam[a], sWidth[a], sIndex[a], vb, sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                                hit = true;
                                 energy += (double) r.e;
-                                count++;
-                                addForce(a, r.fa);
-                                addForce(b, r.fb);
+                                Real fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0};
+                                add_combination(fa, va, vb, r.aA, r.aB);
+                                add_combination(fb, va, vb, r.bA, r.bB);
+                                addForce(a, fa);
+                                addForce(b, fb);
                             }
                         }
+                        count += __popcll(wave_ballot(hit));
                     }
                     g += r1-local;
                     local = r1;
@@ -1067,15 +1088,16 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
                         Real accB[3] = {0, 0, 0};
                         for (int step = s0; step < s1; step++) {
                             const int b = 64*l+((lane+step) & 63);
+                            const R4 vb = sDelta[b];
+                            bool hit = false;
                             if (oMol != sMol[b] &&
-                                segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, sDelta[b], sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                                segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, vb, sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                                hit = true;
                                 energy += (double) r.e;
-                                count++;
-                                for (int c = 0; c < 3; c++) {
-                                    accA[c] += r.fa[c];
-                                    accB[c] += r.fb[c];
-                                }
+                                add_combination(accA, oDelta, vb, r.aA, r.aB);
+                                add_combination(accB, oDelta, vb, r.bA, r.bB);
                             }
+                            count += __popcll(wave_ballot(hit));
                             for (int c = 0; c < 3; c++)
                                 accB[c] = lane_rotate(accB[c]);
                         }
@@ -1087,16 +1109,17 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
                         Real keep[3] = {0, 0, 0};
                         for (int step = s0; step < s1; step++) {
                             const int b = 64*l+step;
+                            const R4 vb = sDelta[b];
                             Real fb[3] = {0, 0, 0};
+                            bool hit = false;
                             if (oMol != sMol[b] &&
-                                segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, sDelta[b], sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                                segment_triple<Real, DAMPED>(t, oDelta, oParam, oWidth, oSite, vb, sParam[b], sWidth, sIndex, b, cI, sI, alchI, r)) {
+                                hit = true;
                                 energy += (double) r.e;
-                                count++;
-                                for (int c = 0; c < 3; c++) {
-                                    accA[c] += r.fa[c];
-                                    fb[c] = r.fb[c];
-                                }
+                                add_combination(accA, oDelta, vb, r.aA, r.aB);
+                                add_combination(fb, oDelta, vb, r.bA, r.bB);
                             }
+                            count += __popcll(wave_ballot(hit));
                             const bool slot = ((lane & 15) == (step & 15));
                             for (int c = 0; c < 3; c++) {
                                 const Real sum = row_sum(fb[c]);
@@ -1124,10 +1147,9 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
         }
         // the share's energy and triple count
         const double e = wave_sum(energy);
-        const int n = (int) wave_sum((double) count);
         if (lane == 0) {
             t.itemEnergy[t.itemShareBase[item]+v] = e;
-            t.itemCount[t.itemShareBase[item]+v] = n;
+            t.itemCount[t.itemShareBase[item]+v] = count;
         }
     }
     __syncthreads();
@@ -1141,12 +1163,15 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
             sum[c] += f;
         }
     }
-    for (int c = 0; c < 3; c++) {
-        long long f = sum[c];
-        for (int offset = 32; offset > 0; offset >>= 1)
-            f += __shfl_xor(f, offset, 64);
-        if (lane == 0)
-            add_fixed(&t.siteForce[(size_t) c*t.numSites+i], -f);
+    for (int c = 0; c < 3; c++)
+        if (sum[c] != 0)
+            atomicAdd(&sSiteSum[c][lane & 15], (unsigned long long) sum[c]);
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long total = 0;
+        for (int k = 0; k < 16; k++)
+            total += sSiteSum[threadIdx.x][k];
+        add_fixed(&t.siteForce[(size_t) threadIdx.x*t.numSites+i], -(long long) total);
     }
 }
 
