@@ -795,12 +795,15 @@ struct TripleResult {
     Real bA, bB;   // the force on b: AB bA + AC bB
 };
 
-// acc += AB kA + AC kB
+// acc += AB kA + AC kB, as two multiply-adds per component
 template <typename Real, typename R4>
 __device__ __forceinline__ void add_combination(Real (&acc)[3], const R4& A, const R4& B, Real kA, Real kB) {
-    acc[0] += A.x*kA+B.x*kB;
-    acc[1] += A.y*kA+B.y*kB;
-    acc[2] += A.z*kA+B.z*kB;
+    acc[0] = fma(A.x, kA, acc[0]);
+    acc[1] = fma(A.y, kA, acc[1]);
+    acc[2] = fma(A.z, kA, acc[2]);
+    acc[0] = fma(B.x, kB, acc[0]);
+    acc[1] = fma(B.y, kB, acc[1]);
+    acc[2] = fma(B.z, kB, acc[2]);
 }
 
 template <typename Real, bool DAMPED>
