@@ -514,12 +514,12 @@ def bench_batch(args, sample_every=16):
         "plain": {"value": plain_rate, "unit": "cluster evals/s", "ms_per_1024": plain_s*1e3,
                   "form": "1024 sequential evaluations, one pair and one three-body handle per cluster"},
         "speedup_vs_plain": batch_rate/plain_rate,
-        "roofline": {"bound": "valu", "note": "the dominant kernel, k_triples (ATM triples of the batch): 130 "
+        "roofline": {"bound": "valu", "note": "the dominant kernel, k_triple_groups (ATM triples of the batch): 130 "
                                               "algorithmic FP32 flops per triple over its HIP-event duration (eager "
                                               "evaluations); no MFMA on this path",
                      "achieved": 130*triples/triple_s/1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": 130*triples/triple_s/1e12/PEAK_FP32_TFLOPS,
-                     "traffic": traffic_from_profiles("k_triples_batch"), "kernel": "k_triples",
+                     "traffic": traffic_from_profiles("k_triple_groups_batch"), "kernel": "k_triple_groups",
                      "kernel_avg_us": triple_s*1e6, "launches": triple_n.value},
         "roofline_evaluation": {"bound": "valu", "note": "240 flops per pair and 130 per triple over the whole batched "
                                                          "evaluation (all kernels, graph replay)",
