@@ -779,6 +779,244 @@ __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     }
 }
 
+// The energy and the forces on a and b of one triple (A, a, b) of a plain handle (neighbours staged as k_triples
+// stages them): BC with the reference's minimum image, decided in FP64 near the cutoff or a change of image; the same
+// arithmetic as k_triples.  Returns false for a triple with no contribution.
+template <typename Real, bool DAMPED>
+__device__ __forceinline__ bool plain_triple(const TripleArgs<Real>& t, const typename Vec4Of<Real>::type& va,
+                                             const typename Vec4Of<Real>::type& pa, Real widthA, int siteA,
+                                             const typename Vec4Of<Real>::type& vb, const typename Vec4Of<Real>::type& pb,
+                                             Real widthB, int siteB, Real cI, Real sI, bool alchI, Real& energy, Real (&fa)[3],
+                                             Real (&fb)[3]) {
+    Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
+    bool exact = false;
+    if (t.periodic) {
+        Real u2 = c[2]*t.boxR[8]+Real(0.5);
+        Real s2 = floor(u2);
+        c[0] -= s2*t.boxR[3];
+        c[1] -= s2*t.boxR[4];
+        c[2] -= s2*t.boxR[5];
+        Real u1 = c[1]*t.boxR[7]+Real(0.5);
+        Real s1 = floor(u1);
+        c[0] -= s1*t.boxR[1];
+        c[1] -= s1*t.boxR[2];
+        Real u0 = c[0]*t.boxR[6]+Real(0.5);
+        Real s0 = floor(u0);
+        c[0] -= s0*t.boxR[0];
+        Real f2 = u2-s2, f1 = u1-s1, f0 = u0-s0;
+        exact = (f2 < t.band || f2 > 1-t.band || f1 < t.band || f1 > 1-t.band || f0 < t.band || f0 > 1-t.band);
+    }
+    Real w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+    bool ok;
+    if (exact || fabs(w-t.cutoff2r) < t.band*t.cutoff2r) {
+        double d[3];
+        ref_delta(t.sites+3*siteA, t.sites+3*siteB, t.periodic, t.box, d);
+        ok = ref_norm2(d) < t.cutoff2;
+        c[0] = Real(d[0]);
+        c[1] = Real(d[1]);
+        c[2] = Real(d[2]);
+        w = c[0]*c[0]+c[1]*c[1]+c[2]*c[2];
+    }
+    else
+        ok = w < t.cutoff2r;
+    if (!ok)
+        return false;
+    Real c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.z*(pa.y+pb.y)*pb.z);
+    if (t.lambda != Real(1) && (alchI || pa.w < 0 || pb.w < 0))
+        c9 *= t.lambda;
+    if (c9 == 0)
+        return false;
+    const Real A[3] = {va.x, va.y, va.z}, B[3] = {vb.x, vb.y, vb.z};
+    const Real u = va.w, v = vb.w;
+    const Real pp = A[0]*B[0]+A[1]*B[1]+A[2]*B[2];
+    const Real q = A[0]*c[0]+A[1]*c[1]+A[2]*c[2];
+    const Real s = B[0]*c[0]+B[1]*c[1]+B[2]*c[2];
+    const Real uv = u*v;
+    const Real rs = fast_rsqrt(uv*w);
+    const Real invP = rs*rs;
+    const Real invP32 = invP*rs;
+    const Real invP52 = invP32*invP;
+    const Real pqs = pp*q*s;
+    Real e = c9*(invP32-3*pqs*invP52);
+    const Real radial = c9*(-3*invP32+15*pqs*invP52);
+    const Real cross = -3*c9*invP52;
+    const Real ru = radial*fabs(pa.w), rv = radial*fabs(pb.w), rw = radial*(invP*uv);
+    const Real X = pp*s*cross, Y = pp*q*cross, Z = q*s*cross;
+    Real aA = X-ru, aB = Y-Z, aC = rw-X;
+    Real bA = -(Z+X), bB = -(rv+Y), bC = -(Y+rw);
+    if constexpr (DAMPED) {
+        Real len[3] = {sqrt(u), sqrt(v), sqrt(w)};
+        Real rates[3] = {2*t.dampingScale/(sI+widthA), 2*t.dampingScale/(sI+widthB), 2*t.dampingScale/(widthA+widthB)};
+        Real f[3], dfdr[3];
+        for (int k = 0; k < 3; k++) {
+            if (t.useDamping) {
+                tt3<Real>(rates[k]*len[k], f[k], dfdr[k]);
+                dfdr[k] *= rates[k];
+            }
+            else {
+                f[k] = 1;
+                dfdr[k] = 0;
+            }
+            if (t.invDampingRadius > 0) {
+                Real x = len[k]*t.invDampingRadius;
+                Real x6 = x*x*x;
+                x6 *= x6;
+                Real g = -expm1(-x6);
+                Real dg = 6*x6*exp(-x6)/len[k];
+                dfdr[k] = dfdr[k]*g + f[k]*dg;
+                f[k] *= g;
+            }
+        }
+        const Real damping = f[0]*f[1]*f[2];
+        const Real sa = e*dfdr[0]*f[1]*f[2]/len[0];
+        const Real sb = e*f[0]*dfdr[1]*f[2]/len[1];
+        const Real sc = e*f[0]*f[1]*dfdr[2]/len[2];
+        aA = aA*damping-sa;
+        aB *= damping;
+        aC = aC*damping+sc;
+        bA *= damping;
+        bB = bB*damping-sb;
+        bC = bC*damping-sc;
+        e *= damping;
+    }
+    energy = e;
+    for (int k = 0; k < 3; k++) {
+        fa[k] = A[k]*aA+B[k]*aB+c[k]*aC;
+        fb[k] = A[k]*bA+B[k]*bB+c[k]*bC;
+    }
+    return true;
+}
+
+// Sum over the GROUP lanes of each lane group (GROUP a power of two up to 64), every lane of a group receiving the
+// same order-fixed total.
+template <int GROUP, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+    for (int offset = GROUP/2; offset > 0; offset >>= 1)
+        v += __shfl_xor(v, offset, 64);
+    return v;
+}
+
+constexpr int PACKED_GROUP = 32;
+
+// 3a. Plain handles whose lists hold at most 64 entries: 64 / GROUP sites per wavefront, GROUP lanes each, so that
+// the fixed costs of a site (filtering its list in FP64, staging, the contributions and reductions) are shared out
+// over several sites (a site's list holds a few tens of entries, its ~100 pairs a fraction of a wavefront's work).
+// Each site has its own LDS region; its pairs are taken in rotation order (rotation r: (x, (x+r+1) mod m)), GROUP
+// per step, the forces on a and b added to LDS in fixed point; the force on A is the negated sum of its
+// neighbours' (exact), written with the per-entry contributions that k_gather_site_forces sums.
+template <typename Real, int GROUP, bool DAMPED>
+__global__ void __launch_bounds__(64) k_triples_packed(TripleArgs<Real> t, int numItems) {
+    typedef typename Vec4Of<Real>::type R4;
+    constexpr int SITES = 64/GROUP;
+    extern __shared__ double sharedStorage[];
+    const int lane = threadIdx.x, g = lane/GROUP, l = lane%GROUP;
+    const int cap = t.capacity;
+    R4* sDelta = (R4*) sharedStorage+(size_t) g*cap;
+    R4* sParam = (R4*) sharedStorage+(size_t) (SITES+g)*cap;
+    unsigned long long* sForce = (unsigned long long*) ((R4*) sharedStorage+(size_t) 2*SITES*cap)+(size_t) 3*g*cap;
+    int* sIndexAll = (int*) ((unsigned long long*) ((R4*) sharedStorage+(size_t) 2*SITES*cap)+(size_t) 3*SITES*cap);
+    int* sIndex = sIndexAll+(size_t) g*cap;
+    int* sMol = sIndexAll+(size_t) (SITES+g)*cap;
+    int* sSlot = sIndexAll+(size_t) (2*SITES+g)*cap;
+    Real* sWidth = (Real*) (sIndexAll+(size_t) 4*SITES*cap)+(size_t) g*cap;
+    const int item = SITES*(int) blockIdx.x+g;
+    const bool valid = (item < numItems);
+    const int i = (valid ? (t.itemIndex != nullptr ? t.itemIndex[item] : item) : 0);
+    const int begin = (valid ? t.start[i] : 0);
+    const int listLength = (valid ? (t.count != nullptr ? t.count[i] : t.start[i+1]-begin) : 0);
+    const Real aI = Real(t.siteParams[4*i]), wI = Real(t.siteParams[4*i+1]), sI = Real(t.siteParams[4*i+2]);
+    const bool alchI = t.siteParams[4*i+3] != 0;
+    // the list's sites within the cutoff now, in list order (ReferenceInteraxThreeBodyIxn.cpp:97-103, the reference's
+    // FP64 rounding), compacted by ballot within the group
+    const double* si = t.sites+3*i;
+    const double inv[3] = {1.0/t.box.ax, 1.0/t.box.by, 1.0/t.box.cz};
+    const unsigned long long groupMask = (GROUP == 64 ? ~0ULL : ((1ULL << GROUP)-1));
+    int m = 0;
+    for (int a0 = 0; __any(a0 < listLength); a0 += GROUP) {
+        const int a = a0+l;
+        int j = -1;
+        double d[3] = {0, 0, 0};
+        bool hit = false;
+        if (a < listLength) {
+            j = t.nbrIndex[begin+a];
+            if (t.exactImage)
+                ref_delta(si, t.sites+3*j, t.periodic, t.box, d);
+            else
+                fast_delta(si, t.sites+3*j, t.periodic, t.box, inv, d);
+            hit = ref_norm2(d) < t.cutoff2;
+        }
+        const unsigned long long mine = (wave_ballot(hit) >> (g*GROUP)) & groupMask;
+        const int slot = m+__popcll(mine & ((1ULL << l)-1));
+        if (hit) {
+            const Real x = Real(d[0]), y = Real(d[1]), z = Real(d[2]);
+            const Real u = x*x+y*y+z*z;
+            R4 dr = {x, y, z, u};
+            sDelta[slot] = dr;
+            const Real alpha = Real(t.siteParams[4*j]), w = Real(t.siteParams[4*j+1]);
+            R4 pr = {alpha*w, w, wI+w, (t.siteParams[4*j+3] != 0 ? -fast_rcp(u) : fast_rcp(u))};
+            sParam[slot] = pr;
+            sIndex[slot] = j;
+            sMol[slot] = t.molecule[j];
+            sWidth[slot] = Real(t.siteParams[4*j+2]);
+            sForce[3*slot] = sForce[3*slot+1] = sForce[3*slot+2] = 0;
+        }
+        if (a < listLength)
+            sSlot[a] = (hit ? slot : -1);
+        m += __popcll(mine);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const Real cI = t.scaleFactor*Real(1.5)*aI*wI;
+    double energy = 0;
+    int count = 0;
+    // the pairs in rotation order; rot = floor(p / m) from (p + 1/2) / m in FP32 (p < 2016, m <= 64: at least
+    // 1/(2 m) from an integer)
+    const int P = m*(m-1)/2;
+    const float invM = (m > 0 ? 1.0f/(float) m : 0.0f);
+    for (int p0 = 0; __any(p0 < P); p0 += GROUP) {
+        const int p = p0+l;
+        if (p < P) {
+            const int rot = (int) (((float) p+0.5f)*invM);
+            const int a = p-rot*m;
+            int b = a+rot+1;
+            if (b >= m)
+                b -= m;
+            Real e, fa[3], fb[3];
+            if (sMol[a] != sMol[b] &&
+                plain_triple<Real, DAMPED>(t, sDelta[a], sParam[a], sWidth[a], sIndex[a], sDelta[b], sParam[b], sWidth[b], sIndex[b], cI, sI,
+                                           alchI, e, fa, fb)) {
+                energy += (double) e;
+                count++;
+                for (int k = 0; k < 3; k++) {
+                    atomicAdd(&sForce[3*a+k], (unsigned long long) to_fixed(fa[k]));
+                    atomicAdd(&sForce[3*b+k], (unsigned long long) to_fixed(fb[k]));
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // the neighbour forces as plain stores, one per list entry (zero beyond the cutoff), summed per site by
+    // k_gather_site_forces, and their negated sum, the force on A (integer sums: exact in any order)
+    long long sum[3] = {0, 0, 0};
+    for (int a = l; a < listLength; a += GROUP) {
+        const int slot = sSlot[a];
+        for (int k = 0; k < 3; k++) {
+            const long long f = (slot >= 0 ? (long long) sForce[3*slot+k] : 0LL);
+            t.contribution[3*((long long) begin+a)+k] = f;
+            sum[k] += f;
+        }
+    }
+    for (int k = 0; k < 3; k++)
+        sum[k] = group_sum<GROUP>(sum[k]);
+    const double e = group_sum<GROUP>(energy);
+    const int c = group_sum<GROUP>(count);
+    if (valid && l == 0) {
+        for (int k = 0; k < 3; k++)
+            t.siteForce[(size_t) k*t.numSites+i] = -sum[k];
+        t.itemEnergy[i] = e;
+        t.itemCount[i] = c;
+    }
+}
+
 // The energy and forces of one triple (A, a, b) of a segmented handle: A the work item's site, a and b two of its
 // neighbours (staged vectors AB and AC with their squares, parameters as the staging writes them; b's width and site
 // index are read only where they are needed), decided within the cutoff in FP32 (in FP64 near it).
@@ -1758,6 +1996,16 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
             hipLaunchKernelGGL((k_triple_groups<Real, true>), dim3(workgroups), dim3(64*h->tripleWaves), shared, stream, t);
         else if (items)
             hipLaunchKernelGGL((k_triple_groups<Real, false>), dim3(workgroups), dim3(64*h->tripleWaves), shared, stream, t);
+        else if (t.capacity <= 64) {
+            // several sites per wavefront (PACKED_GROUP lanes each)
+            constexpr int SITES = 64/PACKED_GROUP;
+            const size_t packed = (size_t) SITES*t.capacity*(2*sizeof(R4)+3*sizeof(long long)+4*sizeof(int)+sizeof(Real));
+            const int blocks = (workgroups+SITES-1)/SITES;
+            if (t.useDamping || t.invDampingRadius > 0)
+                hipLaunchKernelGGL((k_triples_packed<Real, PACKED_GROUP, true>), dim3(blocks), dim3(64), packed, stream, t, workgroups);
+            else
+                hipLaunchKernelGGL((k_triples_packed<Real, PACKED_GROUP, false>), dim3(blocks), dim3(64), packed, stream, t, workgroups);
+        }
         else
             hipLaunchKernelGGL((k_triples<Real>), dim3(workgroups), dim3(64), shared, stream, t);
     }
