@@ -280,6 +280,12 @@ __global__ void __launch_bounds__(64) k_sort_cells(const int* cellStart, const i
             sortedSites[begin+k] = v[k];
 }
 
+__global__ void k_iota_int(int* p, int n) {
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n)
+        p[i] = i;
+}
+
 __global__ void k_fill_int(int* p, int n, int value) {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i < n)
@@ -1696,6 +1702,10 @@ struct ixh_threebody {
     // subsets of segments: site list and segment list of the last call
     StagingRing subsetStaging;
     DeviceArray<int> subsetSites, subsetSegments;
+    // plain handles with lists of at most 64 entries: the sites by list length, longest first (the packed triple
+    // kernel's order, so that the sites of a wavefront have similar work); valid after a list build
+    DeviceArray<int> lengthOrder, lengthKeys, siteIota;
+    bool lengthOrderValid = false;
     DeviceArray<double> paramsDev, sites, siteEnergy;
     DeviceArray<double4> nbrDelta;
     DeviceArray<long long> siteForce, tripleCount;
@@ -1956,7 +1966,7 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.itemFirst = (items ? h->itemFirst.ptr : nullptr);
     t.itemShares = (items ? h->itemShares.ptr : nullptr);
     t.itemShareBase = (items ? h->itemShareBase.ptr : nullptr);
-    t.itemIndex = (subsetItems >= 0 ? h->subsetItems.ptr : (items ? h->itemOrder.ptr : nullptr));
+    t.itemIndex = (subsetItems >= 0 ? h->subsetItems.ptr : (items ? h->itemOrder.ptr : (h->lengthOrderValid ? h->lengthOrder.ptr : nullptr)));
     t.nbrIndex = h->nbrIndex.ptr;
     t.nbrDelta = h->nbrDelta.ptr;
     t.numSites = h->numSites;
@@ -2467,6 +2477,24 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
         IXH_HIP_CHECK(hipMemcpyAsync(h->buildSites.ptr, h->sites.ptr, sizeof(double)*3*(size_t) n, hipMemcpyDeviceToDevice, stream));
         h->listBuilt = true;
         h->listMax = maxCount;
+        h->lengthOrderValid = false;
+        if (maxCount <= 64) {
+            // the packed triple kernel's site order: longest list first
+            h->lengthOrder.reserve(n);
+            h->lengthKeys.reserve(n);
+            h->siteIota.reserve(n);
+            hipLaunchKernelGGL(k_iota_int, dim3(grid_of(n, 256)), dim3(256), 0, stream, h->siteIota.ptr, n);
+            size_t orderBytes = 0;
+            IXH_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, orderBytes, h->count.ptr, h->lengthKeys.ptr, h->siteIota.ptr,
+                                                         h->lengthOrder.ptr, (size_t) n, 0, 7, stream));
+            if (orderBytes > h->sortTemp.count) {
+                IXH_HIP_CHECK(hipStreamSynchronize(stream));   // the old buffer may still be in use by the search
+                h->sortTemp.reserve(orderBytes);
+            }
+            IXH_HIP_CHECK(rocprim::radix_sort_pairs_desc(h->sortTemp.ptr, orderBytes, h->count.ptr, h->lengthKeys.ptr, h->siteIota.ptr,
+                                                         h->lengthOrder.ptr, (size_t) n, 0, 7, stream));
+            h->lengthOrderValid = true;
+        }
         // the reverse lists: which list entries name each site
         {
             const long long entries = (long long) n*h->listCapacity;
