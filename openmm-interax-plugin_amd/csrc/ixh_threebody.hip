@@ -453,6 +453,8 @@ struct TripleArgs {
     int skinLists;               // plain handles: the lists hold the sites within cutoff + skin, the neighbours within the
                                  // cutoff are found at each evaluation (nbrDelta is not used)
     int exactImage;              // skin lists: the reference's minimum image with divisions (small boxes)
+    int bcSameImage;             // no periodic boundaries, or every box length above 4 cutoffs: BC = AC - AB needs no
+                                 // image (|AC - AB| < 2 cutoffs < half a box length)
     long long* contribution;     // skin lists: the force on the site of list entry e from the triples of the list's
                                  // owner, component k at [3 e + k] (summed by k_gather_site_forces)
     long long* siteForce;        // 3 x numSites fixed point
@@ -796,7 +798,7 @@ __device__ __forceinline__ bool plain_triple(const TripleArgs<Real>& t, const ty
                                              Real (&fb)[3]) {
     Real c[3] = {vb.x-va.x, vb.y-va.y, vb.z-va.z};
     bool exact = false;
-    if (t.periodic) {
+    if (!t.bcSameImage) {
         Real u2 = c[2]*t.boxR[8]+Real(0.5);
         Real s2 = floor(u2);
         c[0] -= s2*t.boxR[3];
@@ -827,7 +829,8 @@ __device__ __forceinline__ bool plain_triple(const TripleArgs<Real>& t, const ty
         ok = w < t.cutoff2r;
     if (!ok)
         return false;
-    Real c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.z*(pa.y+pb.y)*pb.z);
+    // staged parameters (alpha_j w_j / (w_A + w_j), w_j, w_A + w_j, +-1/|Aj|^2), as k_triple_groups stages them
+    Real c9 = cI*pa.x*pb.x*(pa.z+pb.y)*fast_rcp(pa.y+pb.y);
     if (t.lambda != Real(1) && (alchI || pa.w < 0 || pb.w < 0))
         c9 *= t.lambda;
     if (c9 == 0)
@@ -909,7 +912,7 @@ constexpr int PACKED_GROUP = 32;
 // over several sites (a site's list holds a few tens of entries, its ~100 pairs a fraction of a wavefront's work).
 // Each site has its own LDS region; its pairs are taken in rotation order (rotation r: (x, (x+r+1) mod m)), GROUP
 // per step, the forces on a and b added to LDS in fixed point; the force on A is the negated sum of its
-// neighbours' (exact), written with the per-entry contributions that k_gather_site_forces sums.
+// neighbours' (exact), formed by k_gather_site_forces from the per-entry contributions.
 template <typename Real, int GROUP, bool DAMPED>
 __global__ void __launch_bounds__(64) k_triples_packed(TripleArgs<Real> t, int numItems) {
     typedef typename Vec4Of<Real>::type R4;
@@ -959,7 +962,7 @@ __global__ void __launch_bounds__(64) k_triples_packed(TripleArgs<Real> t, int n
             R4 dr = {x, y, z, u};
             sDelta[slot] = dr;
             const Real alpha = Real(t.siteParams[4*j]), w = Real(t.siteParams[4*j+1]);
-            R4 pr = {alpha*w, w, wI+w, (t.siteParams[4*j+3] != 0 ? -fast_rcp(u) : fast_rcp(u))};
+            R4 pr = {alpha*w/(wI+w), w, wI+w, (t.siteParams[4*j+3] != 0 ? -fast_rcp(u) : fast_rcp(u))};
             sParam[slot] = pr;
             sIndex[slot] = j;
             sMol[slot] = t.molecule[j];
@@ -1000,24 +1003,16 @@ __global__ void __launch_bounds__(64) k_triples_packed(TripleArgs<Real> t, int n
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // the neighbour forces as plain stores, one per list entry (zero beyond the cutoff), summed per site by
-    // k_gather_site_forces, and their negated sum, the force on A (integer sums: exact in any order)
-    long long sum[3] = {0, 0, 0};
+    // the neighbour forces as plain stores, one per list entry (zero beyond the cutoff): k_gather_site_forces sums
+    // them per site, and the force on A as their negated sum (integer sums: exact in any order)
     for (int a = l; a < listLength; a += GROUP) {
         const int slot = sSlot[a];
-        for (int k = 0; k < 3; k++) {
-            const long long f = (slot >= 0 ? (long long) sForce[3*slot+k] : 0LL);
-            t.contribution[3*((long long) begin+a)+k] = f;
-            sum[k] += f;
-        }
+        for (int k = 0; k < 3; k++)
+            t.contribution[3*((long long) begin+a)+k] = (slot >= 0 ? (long long) sForce[3*slot+k] : 0LL);
     }
-    for (int k = 0; k < 3; k++)
-        sum[k] = group_sum<GROUP>(sum[k]);
     const double e = group_sum<GROUP>(energy);
     const int c = group_sum<GROUP>(count);
     if (valid && l == 0) {
-        for (int k = 0; k < 3; k++)
-            t.siteForce[(size_t) k*t.numSites+i] = -sum[k];
         t.itemEnergy[i] = e;
         t.itemCount[i] = c;
     }
@@ -1519,7 +1514,8 @@ __global__ void k_apply_site_forces(const long long* siteForce, int numSites, co
 // triple counts of its sites in site order into one partial (groupEnergy, groupCount), for k_reduce_sites.
 constexpr int GATHER_LANES = 16;
 constexpr int GATHER_THREADS = 256;
-__global__ void __launch_bounds__(GATHER_THREADS) k_gather_site_forces(const long long* own, const long long* contribution,
+__global__ void __launch_bounds__(GATHER_THREADS) k_gather_site_forces(const long long* own, const int* listStart,
+                                                                       const int* listCount, const long long* contribution,
                                                                        const int* revStart, const int* revEnd,
                                                                        const int* revList, int numSites, const int* offsets,
                                                                        const int* particles, long long* force, int stride,
@@ -1553,9 +1549,18 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_site_forces(const lon
         return;   // uniform over the launch: the lane exchanges below need every lane of the wave
     long long total[3] = {0, 0, 0};
     if (i < numSites) {
-        if (sub == 0)
-            for (int k = 0; k < 3; k++)
-                total[k] = own[(size_t) k*numSites+i];
+        if (listStart == nullptr) {
+            if (sub == 0)
+                for (int k = 0; k < 3; k++)
+                    total[k] = own[(size_t) k*numSites+i];
+        }
+        else
+            // the force on site i from its own triples: the negated sum of its list's contributions
+            for (int a = sub; a < listCount[i]; a += GATHER_LANES) {
+                const long long e = (long long) listStart[i]+a;
+                for (int k = 0; k < 3; k++)
+                    total[k] -= contribution[3*e+k];
+            }
         for (int r = revStart[i]+sub; r < revEnd[i]; r += GATHER_LANES) {
             const long long e = revList[r];
             for (int k = 0; k < 3; k++)
@@ -1993,6 +1998,7 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.skinLists = (h->numSegments == 0);
     t.contribution = h->contribution.ptr;
     t.exactImage = (h->periodic && 2*h->cutoff >= std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
+    t.bcSameImage = (!h->periodic || 4*h->cutoff < 0.999*std::min(box.ax, std::min(box.by, box.cz))) ? 1 : 0;
     t.siteForce = h->siteForce.ptr;
     t.itemEnergy = h->siteEnergy.ptr;
     t.itemCount = h->siteCount.ptr;
@@ -2034,7 +2040,10 @@ void finish_plain(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, doubl
     const int groups = grid_of((long long) n*GATHER_LANES, GATHER_THREADS);
     h->groupEnergy.reserve(groups);
     h->groupCount.reserve(groups);
+    // the packed triple kernel (lists of at most 64 entries) leaves the sites' own forces to the gather
+    const bool packed = (h->listMax <= 64);
     hipLaunchKernelGGL(k_gather_site_forces, dim3(groups), dim3(GATHER_THREADS), 0, stream, (const long long*) h->siteForce.ptr,
+                       packed ? (const int*) h->start.ptr : nullptr, packed ? (const int*) h->count.ptr : nullptr,
                        (const long long*) h->contribution.ptr, (const int*) h->revStart.ptr, (const int*) h->revEnd.ptr,
                        (const int*) h->revList.ptr, n, (const int*) h->offsetsDev.ptr, (const int*) h->particlesDev.ptr, b->force,
                        b->force_stride, (flags & IXH_INCLUDE_FORCES) != 0 ? 1 : 0, (const double*) h->siteEnergy.ptr,
