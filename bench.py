@@ -361,7 +361,7 @@ def bench_triplet(args):
                                               "HIP-event time (ms_per_step covers the whole evaluation: centroids, "
                                               "neighbour search, triples, site forces)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved/PEAK_FP32_TFLOPS,
-                     "traffic": traffic_from_profiles("k_triples"), "kernel": "k_triples",
+                     "traffic": traffic_from_profiles("k_triples_packed"), "kernel": "k_triples_packed",
                      "kernel_avg_us": avg_kernel_s*1e6, "launches": launches.value,
                      "timed_launches": f"HIP events recorded around every {TIMER_STRIDE}th launch of the timed loop"},
     }
