@@ -332,7 +332,7 @@ __device__ __forceinline__ void fast_delta(const double* from, const double* to,
 // the search repeated with more room.
 __global__ void k_neighbors(const double* sites, const int* molecule, int numSites, int periodic, BoxT box, double cutoff2, CellGrid g,
                             int exactImage, const int* cellStart, const int* cellEnd, const int* sortedSites, int* count,
-                            int capacity, int* nbrIndex, double4* nbrDelta) {
+                            int capacity, int* nbrIndex) {
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const double* si = sites+3*i;
@@ -395,11 +395,8 @@ __global__ void k_neighbors(const double* sites, const int* molecule, int numSit
         }
         const unsigned long long ballot = __ballot(hit);
         const int pos = written+__popcll(ballot & ((1ULL << lane)-1));
-        if (hit && pos < capacity) {
+        if (hit && pos < capacity)
             nbrIndex[(size_t) i*capacity+pos] = j;
-            if (nbrDelta != nullptr)
-                nbrDelta[(size_t) i*capacity+pos] = make_double4(d[0], d[1], d[2], 0.0);
-        }
         written += __popcll(ballot);
     }
     if (lane == 0)
@@ -436,7 +433,6 @@ struct TripleArgs {
     const int* itemShareBase;    // work items: the index of item w's first share among all shares
     const int* itemIndex;        // subsets: the item of every workgroup (nullptr: item = workgroup index)
     const int* nbrIndex;
-    const double4* nbrDelta;
     int numSites;
     int periodic;
     BoxT box;
@@ -451,7 +447,7 @@ struct TripleArgs {
     Real invDampingRadius;       // 0: no centroid damping
     int capacity;                // LDS entries
     int skinLists;               // plain handles: the lists hold the sites within cutoff + skin, the neighbours within the
-                                 // cutoff are found at each evaluation (nbrDelta is not used)
+                                 // cutoff are found at each evaluation
     int exactImage;              // skin lists: the reference's minimum image with divisions (small boxes)
     int bcSameImage;             // no periodic boundaries, or every box length above 4 cutoffs: BC = AC - AB needs no
                                  // image (|AC - AB| < 2 cutoffs < half a box length)
@@ -1203,10 +1199,12 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
     __shared__ unsigned long long sSiteSum[3][16];
     if (threadIdx.x < 48)
         sSiteSum[threadIdx.x >> 4][threadIdx.x & 15] = 0;
+    const double* si = t.sites+3*i;
     for (int a = threadIdx.x; a < m; a += blockDim.x) {
-        const double4 d4 = t.nbrDelta[begin+a];
+        // AB from the two centroids, as the neighbour search rounds it (segments are nonperiodic)
         const int j = t.nbrIndex[begin+a];
-        const Real x = Real(d4.x), y = Real(d4.y), z = Real(d4.z);
+        const Real x = Real(__dsub_rn(t.sites[3*j], si[0])), y = Real(__dsub_rn(t.sites[3*j+1], si[1])),
+                   z = Real(__dsub_rn(t.sites[3*j+2], si[2]));
         const Real u = x*x+y*y+z*z;
         R4 dr = {x, y, z, u};
         sDelta[a] = dr;
@@ -1420,7 +1418,7 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
 // Segmented neighbour lists: site i's candidates are the later sites of its own segment, [i+1, siteEnd[i]); the hits
 // (another molecule, within the cutoff, no periodic images) are written from start[i], their number to count[i].
 __global__ void k_neighbors_segments(const double* sites, const int* molecule, const int* siteEnd, double cutoff2, const int* start,
-                                     int* count, int* nbrIndex, double4* nbrDelta, const int* siteIndex) {
+                                     int* count, int* nbrIndex, const int* siteIndex) {
     const int i = (siteIndex != nullptr ? siteIndex[blockIdx.x] : (int) blockIdx.x);
     const int lane = threadIdx.x;
     const double* si = sites+3*i;
@@ -1438,11 +1436,8 @@ __global__ void k_neighbors_segments(const double* sites, const int* molecule, c
             hit = ref_norm2(d) < cutoff2;
         }
         const unsigned long long ballot = __ballot(hit);
-        if (hit) {
-            const int pos = start[i]+written+__popcll(ballot & ((1ULL << lane)-1));
-            nbrIndex[pos] = j;
-            nbrDelta[pos] = make_double4(d[0], d[1], d[2], 0.0);
-        }
+        if (hit)
+            nbrIndex[start[i]+written+__popcll(ballot & ((1ULL << lane)-1))] = j;
         written += __popcll(ballot);
     }
     if (lane == 0)
@@ -1712,7 +1707,6 @@ struct ixh_threebody {
     DeviceArray<int> lengthOrder, lengthKeys, siteIota;
     bool lengthOrderValid = false;
     DeviceArray<double> paramsDev, sites, siteEnergy;
-    DeviceArray<double4> nbrDelta;
     DeviceArray<long long> siteForce, tripleCount;
     hipStream_t lastStream = nullptr;
     hipEvent_t done = nullptr;   // recorded at the end of every execute: calls on other streams wait for it
@@ -1973,7 +1967,6 @@ void launch_triples(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, dou
     t.itemShareBase = (items ? h->itemShareBase.ptr : nullptr);
     t.itemIndex = (subsetItems >= 0 ? h->subsetItems.ptr : (items ? h->itemOrder.ptr : (h->lengthOrderValid ? h->lengthOrder.ptr : nullptr)));
     t.nbrIndex = h->nbrIndex.ptr;
-    t.nbrDelta = h->nbrDelta.ptr;
     t.numSites = h->numSites;
     t.periodic = h->periodic;
     t.box = box;
@@ -2158,7 +2151,6 @@ int ixh_threebody_create_segmented(int device, const ixh_threebody_desc* d, int 
                 h->siteEnd.upload(end.data(), std::max<size_t>(end.size(), 1), nullptr);
                 h->start.upload(start.data(), start.size(), nullptr);
                 h->nbrIndex.reserve((size_t) std::max<long long>(total, 1));
-                h->nbrDelta.reserve((size_t) std::max<long long>(total, 1));
             }
             IXH_HIP_CHECK(hipEventCreateWithFlags(&h->updateDone, hipEventDisableTiming));
             IXH_HIP_CHECK(hipHostMalloc((void**) &h->hostInvalid, sizeof(int), hipHostMallocDefault));
@@ -2325,7 +2317,7 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
             const int workgroups = (subsetSites >= 0 ? subsetSites : n);
             if (workgroups > 0)
                 hipLaunchKernelGGL(k_neighbors_segments, dim3(workgroups), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr,
-                                   h->siteEnd.ptr, cutoff2, h->start.ptr, h->count.ptr, h->nbrIndex.ptr, h->nbrDelta.ptr,
+                                   h->siteEnd.ptr, cutoff2, h->start.ptr, h->count.ptr, h->nbrIndex.ptr,
                                    subsetSites >= 0 ? (const int*) h->subsetSites.ptr : (const int*) nullptr);
             IXH_HIP_CHECK(hipGetLastError());
             hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
@@ -2464,7 +2456,7 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
             }
             hipLaunchKernelGGL(k_neighbors, dim3(n), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr, n, h->periodic, box,
                                listCutoff*listCutoff, g, exactImage, h->cellStart.ptr, h->cellEnd.ptr, h->sortedSites.ptr, h->count.ptr,
-                               h->listCapacity, h->nbrIndex.ptr, (double4*) nullptr);
+                               h->listCapacity, h->nbrIndex.ptr);
             IXH_HIP_CHECK(hipGetLastError());
             IXH_HIP_CHECK(rocprim::reduce(h->sortTemp.ptr, maxBytes, h->count.ptr, h->listTotals.ptr, 0, (size_t) n,
                                           rocprim::maximum<int>(), stream));
