@@ -41,9 +41,9 @@ def test_batch_against_oracle(precision):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["mixed", "double"])
 def test_large_clusters_against_oracle(precision):
-    # sites with more than 64 later neighbours take the triple kernel's row order (pairs (a, b), b >= 64, in rows,
-    # the forces on b summed per row), and the 200-molecule cluster's first sites (more than 16 384 neighbour pairs)
-    # are split into several work items
+    # sites with more than 64 later neighbours take the triple kernel's blocks of 64 neighbours (full blocks with the
+    # forces summed in registers, a partial last block), and the 200-molecule cluster's first sites (more than
+    # 16 384 neighbour pairs) are split into several work items
     systems, positions = clusters([200, 130, 66], seed=11)
     batch = ih.Batch(systems, precision)
     energies, forces = batch.evaluate(positions)
@@ -119,6 +119,51 @@ def test_every_launch_shape_gives_the_same_results(precision):
                 continue
             assert np.array_equal(e, reference[0])
             for a, b in zip(f, reference[1]):
+                assert np.array_equal(a, b)
+    finally:
+        batch.close()
+
+
+def damped_cluster(num_molecules, seed):
+    """A water-like cluster whose ATM force has Tang-Toennies and centroid damping, a second (two-particle) site on
+    every fourth molecule and a cutoff inside the cluster, so that the segmented triple kernel's damped form, its
+    molecule filter and its cutoff test all take part."""
+    rng = np.random.default_rng(seed)
+    radius = (3*num_molecules/(4*np.pi*33.4))**(1.0/3.0)
+    centers = workloads.rsa_points(rng, num_molecules, 2*radius+2.0, 0.25, periodic=False, sphere_radius=radius)
+    positions = workloads.water_sites(rng, centers)
+    system = ih.System(len(positions))
+    workloads.add_water_forces(system, num_molecules, rng, ih.InteraxPairForce.NoCutoff, 1.0)
+    tb = ih.InteraxThreeBodyForce()
+    for m in range(num_molecules):
+        tb.addSite([3*m], 1.45e-3, 1150.0, 0.05+0.01*(m % 3), m)
+        if m % 4 == 0:
+            tb.addSite([3*m+1, 3*m+2], 0.4e-3, 1300.0, 0.04, m)
+    tb.setDampingMethod(ih.InteraxThreeBodyForce.TangToennies)
+    tb.setDampingScale(0.9)
+    tb.setCentroidDampingRadius(0.2)
+    tb.setCutoffDistance(1.0)
+    system.addForce(tb)
+    return system, positions
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed", "double"])
+def test_damped_sites_against_oracle(precision):
+    # more than 64 sites per cluster (full and partial neighbour blocks) with damping, sites of one molecule and
+    # pairs beyond the cutoff; the same results for every launch shape
+    out = [damped_cluster(n, seed=20+k) for k, n in enumerate([150, 9, 70])]
+    systems, positions = [s for s, _ in out], [p for _, p in out]
+    batch = ih.Batch(systems, precision)
+    try:
+        energies, forces = batch.evaluate(positions)
+        check_against_oracle(systems, positions, energies, forces, precision)
+        for workgroups, waves, pairs in [(256, 1, 1024), (1024, 8, 65536), (512, 2, 8192)]:
+            print(f"shape: {workgroups} pair workgroups, {waves} waves, {pairs} pairs per item", flush=True)
+            batch.set_shape(workgroups, waves, pairs)
+            e, f = batch.evaluate(positions)
+            assert np.array_equal(e, energies)
+            for a, b in zip(f, forces):
                 assert np.array_equal(a, b)
     finally:
         batch.close()
