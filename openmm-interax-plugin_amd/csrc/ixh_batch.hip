@@ -67,7 +67,7 @@ struct ixh_batch {
     // The pair half of an evaluation runs on the state's stream and the three-body half beside it on `side` (forked
     // and joined by events, also inside the captured graph), with its own energies, added at the join.
     hipStream_t side = nullptr;
-    hipEvent_t forked = nullptr, joined = nullptr;
+    hipEvent_t forked = nullptr, joined = nullptr, searched = nullptr;
     DeviceArray<double> threeEnergies, threeTotal;
     double* hostEnergies = nullptr;   // pinned
     int evaluated = 0;                // eager evaluations so far (the graph is captured after the first)
@@ -87,8 +87,10 @@ void release_side(ixh_batch* b) {
         (void) hipEventDestroy(b->forked);
     if (b->joined != nullptr)
         (void) hipEventDestroy(b->joined);
+    if (b->searched != nullptr)
+        (void) hipEventDestroy(b->searched);
     b->side = nullptr;
-    b->forked = b->joined = nullptr;
+    b->forked = b->joined = b->searched = nullptr;
 }
 
 void destroy_graph(ixh_batch* b) {
@@ -120,11 +122,14 @@ void enqueue(ixh_batch* b, int flags) {
     IXH_HIP_CHECK(hipGetLastError());
     IXH_HIP_CHECK(hipEventRecord(b->forked, stream));
     IXH_HIP_CHECK(hipStreamWaitEvent(b->side, b->forked, 0));
-    check(ixh_pair_execute_segments(b->pair, &b->buffers, b->pairTerms, flags, b->energies.ptr));
     ixh_buffers three = b->buffers;
     three.stream = b->side;
     three.energy = b->threeTotal.ptr;
     check(ixh_threebody_execute_segments(b->threebody, &three, flags, b->threeEnergies.ptr));
+    // the pair half starts once the three-body half has its neighbour lists, so that the (latency-bound) search runs
+    // alone and the pair tiles share the device with the triple kernel only
+    IXH_HIP_CHECK(hipStreamWaitEvent(stream, b->searched, 0));
+    check(ixh_pair_execute_segments(b->pair, &b->buffers, b->pairTerms, flags, b->energies.ptr));
     IXH_HIP_CHECK(hipEventRecord(b->joined, b->side));
     IXH_HIP_CHECK(hipStreamWaitEvent(stream, b->joined, 0));
     hipLaunchKernelGGL(k_add_three_body, grid, dim3(256), 0, stream, b->energies.ptr, (const double*) b->threeEnergies.ptr,
@@ -204,6 +209,8 @@ int ixh_batch_create(int device, const ixh_batch_desc* d, ixh_batch** out) {
                 IXH_HIP_CHECK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
                 IXH_HIP_CHECK(hipEventCreateWithFlags(&b->forked, hipEventDisableTiming));
                 IXH_HIP_CHECK(hipEventCreateWithFlags(&b->joined, hipEventDisableTiming));
+                IXH_HIP_CHECK(hipEventCreateWithFlags(&b->searched, hipEventDisableTiming));
+                threebody_set_search_event(b->threebody, b->searched);
             }
             IXH_HIP_CHECK(hipHostMalloc((void**) &b->hostEnergies, sizeof(double)*b->numSystems, hipHostMallocDefault));
         }

@@ -4,6 +4,8 @@
 
 #include "interax_hip.h"
 
+#include <hip/hip_runtime.h>
+
 namespace ixh {
 
 // Largest number of workgroups of a pair tile launch (each wave then takes a contiguous range of tiles).
@@ -17,6 +19,9 @@ void pair_lambda_collect(const ixh_pair* pair, int count, double* energies);
 // Launch shape of a segmented three-body handle's triple kernel: wavefronts per work item (1, 2, 4, 8) and neighbour
 // pairs per work item (a multiple of 64).  Waits for the device when the work items change.
 void threebody_set_launch_shape(ixh_threebody* threebody, int waves, int chunkPairs);
+// An event a segmented three-body handle records on its stream after each evaluation's neighbour search (NULL: none);
+// the caller owns it.
+void threebody_set_search_event(ixh_threebody* threebody, hipEvent_t event);
 void threebody_launch_shape(const ixh_threebody* threebody, int* waves, int* chunkPairs);
 
 }  // namespace ixh

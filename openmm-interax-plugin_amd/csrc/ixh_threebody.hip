@@ -1417,10 +1417,15 @@ __global__ void __launch_bounds__(512) k_triple_groups(TripleArgs<Real> t) {
 
 // Segmented neighbour lists: site i's candidates are the later sites of its own segment, [i+1, siteEnd[i]); the hits
 // (another molecule, within the cutoff, no periodic images) are written from start[i], their number to count[i].
-__global__ void k_neighbors_segments(const double* sites, const int* molecule, const int* siteEnd, double cutoff2, const int* start,
-                                     int* count, int* nbrIndex, const int* siteIndex) {
-    const int i = (siteIndex != nullptr ? siteIndex[blockIdx.x] : (int) blockIdx.x);
-    const int lane = threadIdx.x;
+constexpr int SEGMENT_SEARCH_WAVES = 4;   // sites per workgroup of k_neighbors_segments, one wavefront each
+__global__ void __launch_bounds__(64*SEGMENT_SEARCH_WAVES) k_neighbors_segments(const double* sites, const int* molecule, const int* siteEnd,
+                                                                             double cutoff2, const int* start, int* count, int* nbrIndex,
+                                                                             const int* siteIndex, int numSearched) {
+    const int w = SEGMENT_SEARCH_WAVES*(int) blockIdx.x+(int) (threadIdx.x >> 6);
+    if (w >= numSearched)
+        return;   // a whole wavefront
+    const int i = (siteIndex != nullptr ? siteIndex[w] : w);
+    const int lane = threadIdx.x & 63;
     const double* si = sites+3*i;
     const int mi = molecule[i];
     const int end = siteEnd[i];
@@ -1710,6 +1715,7 @@ struct ixh_threebody {
     DeviceArray<long long> siteForce, tripleCount;
     hipStream_t lastStream = nullptr;
     hipEvent_t done = nullptr;   // recorded at the end of every execute: calls on other streams wait for it
+    hipEvent_t searched = nullptr;   // segmented handles: recorded after the neighbour search, if set (not owned)
     // parameter updates: FP64 alpha, frequency, width of the host path, the update's end, and its width check
     DeviceArray<double> rawSites;
     DeviceArray<int> invalid;
@@ -2052,6 +2058,10 @@ void finish_plain(ixh_threebody* h, const ixh_buffers* b, const BoxT& box, doubl
 
 namespace ixh {
 
+void threebody_set_search_event(ixh_threebody* h, hipEvent_t event) {
+    h->searched = event;
+}
+
 void threebody_set_launch_shape(ixh_threebody* h, int waves, int chunkPairs) {
     if (waves != 1 && waves != 2 && waves != 4 && waves != 8)
         throw Error(IXH_ERROR_INVALID_ARGUMENT, "ixh_threebody: the triple kernel takes 1, 2, 4 or 8 wavefronts per work item");
@@ -2316,12 +2326,15 @@ int ixh_threebody_execute_subset(ixh_threebody* h, const ixh_buffers* b, int fla
             }
             const int workgroups = (subsetSites >= 0 ? subsetSites : n);
             if (workgroups > 0)
-                hipLaunchKernelGGL(k_neighbors_segments, dim3(workgroups), dim3(64), 0, stream, h->sites.ptr, h->moleculeDev.ptr,
-                                   h->siteEnd.ptr, cutoff2, h->start.ptr, h->count.ptr, h->nbrIndex.ptr,
-                                   subsetSites >= 0 ? (const int*) h->subsetSites.ptr : (const int*) nullptr);
+                hipLaunchKernelGGL(k_neighbors_segments, dim3((workgroups+SEGMENT_SEARCH_WAVES-1)/SEGMENT_SEARCH_WAVES),
+                                   dim3(64*SEGMENT_SEARCH_WAVES), 0, stream, h->sites.ptr, h->moleculeDev.ptr, h->siteEnd.ptr, cutoff2,
+                                   h->start.ptr, h->count.ptr, h->nbrIndex.ptr,
+                                   subsetSites >= 0 ? (const int*) h->subsetSites.ptr : (const int*) nullptr, workgroups);
             IXH_HIP_CHECK(hipGetLastError());
             hipLaunchKernelGGL(k_zero_long, dim3(grid_of(3LL*n, 256)), dim3(256), 0, stream, h->siteForce.ptr, 3LL*n);
             IXH_HIP_CHECK(hipGetLastError());
+            if (h->searched != nullptr)
+                IXH_HIP_CHECK(hipEventRecord(h->searched, stream));
             if (h->precision == IXH_DOUBLE)
                 launch_triples<double>(h, b, box, cutoff2, h->maxSegment, stream, subsetItemCount);
             else
