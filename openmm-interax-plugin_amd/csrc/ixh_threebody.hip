@@ -14,9 +14,9 @@
 //   3. one wavefront per site A takes the neighbours of its list within the cutoff at this evaluation's centroids
 //      (the reference's FP64 decision), walks their pairs (B, C), tests B-C in FP32 (FP64 near the cutoff or near a
 //      change of periodic image) and evaluates the triple in FP32; the forces on B and C go to LDS as 64-bit fixed
-//      point (integer atomics, so their order does not matter), the force on A stays in registers.  Segmented
-//      handles (clusters, k_triple_groups) walk the pairs by blocks of 64 neighbours, with both neighbours' forces
-//      summed in registers where the blocks are full;
+//      point (integer atomics, so their order does not matter).  Lists of at most 64 entries take two sites per
+//      wavefront (k_triples_packed).  Segmented handles (clusters, k_triple_groups) walk the pairs by blocks of 64
+//      neighbours, with both neighbours' forces summed in registers where the blocks are full;
 //   4. the site forces are shared equally among the particles of each site.
 #include "ixh_common.h"
 #include "ixh_internal.h"
@@ -476,7 +476,8 @@ __device__ __forceinline__ void tt3(Real x, Real& f, Real& df) {
         f = Real(1)-expX*(Real(1)+x+x*x*Real(0.5)+term3);
 }
 
-// 3. Plain handles: one wavefront per site A and all of its neighbour pairs (segmented handles: k_triple_groups).
+// 3. Plain handles with lists longer than 64 entries: one wavefront per site A and all of its neighbour pairs
+// (shorter lists: k_triples_packed; segmented handles: k_triple_groups).
 template <typename Real>
 __global__ void __launch_bounds__(64) k_triples(TripleArgs<Real> t) {
     typedef typename Vec4Of<Real>::type R4;
