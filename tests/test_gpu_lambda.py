@@ -47,7 +47,8 @@ def test_matches_separate_evaluations(model, precision):
 @pytest.mark.parametrize("softcore", [0.0, 0.3])
 def test_alchemical_box(softcore):
     """The alchemical job of the served configuration: a 12-atom solute in 1000 water-like molecules, all three pair
-    forces fused, long range correction on."""
+    forces fused, fused windows against separate evaluations (without the long range correction here; the windows
+    with it are checked against the oracle below)."""
     system, pos = workloads.alchemical_box(softcore, seed=5)
     context = ih.Context(system, {"Precision": "mixed"})
     context.setPositions(pos)
