@@ -1137,7 +1137,8 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
     if constexpr (TABLES)
         tb = load_tables<Real>(a, tableLds);
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    // the wavefront's index as a wave-uniform value (the work item and tile indices derived from it are then uniform)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     QueueLds<Real, TABLES>& q = lds[w];
     const long long wave = (long long) blockIdx.x*QUEUE_GROUP_WAVES+w;
     const long long numWaves = (long long) gridDim.x*QUEUE_GROUP_WAVES;
@@ -1158,13 +1159,32 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
     double energy = 0;
     int pairs = 0;
 
+    // A tile's partners and first candidate entries are loaded before the previous tile's j-side flush: vector memory
+    // operations are counted in order, so a load issued after the flush's atomics could only be waited for together
+    // with them; issued first, they are in flight together with the flush.
+    long long t = 0;
+    int tileX = 0, cBegin = 0, cEnd = 0;
+    PartnerFetch<Real, TABLES> f;
+    uint2 firstChunk = {0, 0};
+    bool firstValid = false;
+    auto prefetch_tile = [&](long long item) {
+        t = (a.tileIndex != nullptr ? a.tileIndex[item] : item);
+        // the tile word, the lane's partner slot and the list bounds together, then what depends on them
+        tileX = a.tiles[t].x;
+        const int slot = a.tileSlots[(size_t) t*BLOCK+lane];
+        cBegin = a.listOffsets[t];
+        cEnd = a.listOffsets[t+1];
+        // every lane loads (from the tile's last four entries, or the start of the array, when past the end) and keeps
+        // a validity flag; offsets are multiples of 4 and the array holds at least 4
+        firstValid = (cBegin+4*lane < cEnd);
+        firstChunk = *(const uint2*) (a.lists+(firstValid ? cBegin+4*lane : (cEnd >= 4 ? cEnd-4 : 0)));
+        fetch_partner<Real, TABLES>(a, slot, f);
+    };
+    if (begin < end)
+        prefetch_tile(begin);
     for (long long item = begin; item < end; item++) {
-        const long long t = (a.tileIndex != nullptr ? a.tileIndex[item] : item);
-        const int2 tile = a.tiles[t];
-        PartnerFetch<Real, TABLES> f;
-        fetch_partner<Real, TABLES>(a, a.tileSlots[(size_t) t*BLOCK+lane], f);
-        const int I = tile.x & TILE_BLOCK_MASK;
-        const bool pairImage = (tile.x & TILE_PAIR_IMAGE) != 0;
+        const int I = tileX & TILE_BLOCK_MASK;
+        const bool pairImage = (tileX & TILE_PAIR_IMAGE) != 0;
         if (I != curI) {
             if (curI >= 0 && flagI >= 0) {
                 int oi = flagI & INDEX_MASK;
@@ -1194,11 +1214,10 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
         // Every lane loads (from the tile's last four entries, or the start of the array, when past the end) and
         // keeps a validity flag instead of branching, so that the wait for a step's entries can count the next
         // step's load as outstanding on every path.  Offsets are multiples of 4 and the array holds at least 4.
-        const int cBegin = a.listOffsets[t], cEnd = a.listOffsets[t+1];
         const int cLast = (cEnd >= 4 ? cEnd-4 : 0);
         int head = 0, count = 0;
-        bool chunkValid = (cBegin+4*lane < cEnd);
-        uint2 chunk = *(const uint2*) (a.lists+(chunkValid ? cBegin+4*lane : cLast));
+        bool chunkValid = firstValid;
+        uint2 chunk = firstChunk;
         for (int base = cBegin; base < cEnd; base += 4*BLOCK) {
             const int nextIndex = base+4*BLOCK+4*lane;
             const bool nextValid = (nextIndex < cEnd);
@@ -1243,6 +1262,8 @@ __global__ void __launch_bounds__(64*QUEUE_GROUP_WAVES, TABLES ? 5 : 1) k_pair_q
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         {
             const int flagJ = f.flags;
+            if (item+1 < end)
+                prefetch_tile(item+1);
             if (flagJ >= 0) {
                 int oj = flagJ & INDEX_MASK;
                 add_fixed(&a.force[oj], -(long long) q.fJ[0][lane]);
